@@ -1,0 +1,193 @@
+/*
+ * mms_gpu.h -- C ABI of the MI355X (gfx950) field-evaluation + LLG RK4 library
+ * (libmmsgpu.so).
+ *
+ * Part 1 declares drop-in replacements for the reference's own entry points,
+ * with the same argument meaning and 0/1 (EXIT_SUCCESS/EXIT_FAILURE) return
+ * convention.  Each cites the reference declaration it replaces.  Differences a
+ * caller can observe:
+ *   - plain C linkage; Vector3 arrays are passed as `float *` holding x,y,z
+ *     per cell (the 12-byte AoS layout of the reference's struct Vector3);
+ *   - all computation runs on the GPU whatever `use_gpu` says (the argument is
+ *     accepted for signature compatibility); there is no CPU path;
+ *   - errors (HIP failures, bad shapes) return EXIT_FAILURE with a message in
+ *     mms_last_error() instead of calling exit();
+ *   - device buffers are cached per problem shape inside the library instead of
+ *     function-level statics sized by the first call, so any sequence of shapes
+ *     is accepted;
+ *   - fmm_calc accepts any xdim, ydim (zero-padded to a power-of-two square)
+ *     and expansion orders P = 1..12 (the reference asserts P <= 4).
+ *
+ * Part 2 is the resident API: an explicit context that owns the state on the
+ * device, runs fixed-step RK4 without host round trips, and selects the
+ * potential solver and the M2L precision.
+ *
+ * Thread safety: calls on one context must not overlap; distinct contexts may
+ * be used from distinct host threads.  The Part-1 entry points share an
+ * internal cache and are serialised by a mutex.
+ */
+#ifndef MMS_GPU_H
+#define MMS_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Part 1: the reference's entry points                                     */
+/* ------------------------------------------------------------------------ */
+
+/* O(N^2) scalar potential phi_i = sum_{j != i} charge_j dx dy dz / |r_i - r_j|
+ * over the whole xdim*ydim*zdim grid (cell index z*ydim*xdim + y*xdim + x).
+ * `potential` is fully overwritten.
+ * Replaces: int calc_potential_exact(...)  helper_functions.hpp:31-35,
+ *           defined fmm_calc.cu:412-461. */
+int calc_potential_exact(const float *charge, int xdim, int ydim, int zdim,
+                         float dx, float dy, float dz, float *potential,
+                         int use_gpu, int verbosity);
+
+/* Same computation as calc_potential_exact.
+ * Replaces: int calc_potential_exact_gpu(...)  helper_functions.hpp:37-40,
+ *           defined potential_calc.cu:120-188. */
+int calc_potential_exact_gpu(const float *charge, int xdim, int ydim, int zdim,
+                             float dx, float dy, float dz, float *potential);
+
+/* Fast-multipole scalar potential of order P (same physics as
+ * calc_potential_exact, truncation error falling geometrically with P).
+ * `potential` is fully overwritten.
+ * Replaces: int fmm_calc(...)  helper_functions.hpp:24-29,
+ *           defined fmm_calc.cu:295-408. */
+int fmm_calc(const float *charge, float *potential, int xdim, int ydim, int zdim,
+             float dx, float dy, float dz, int P, int use_gpu, int verbosity);
+
+/* Effective field H (A/m) of magnetisation M (A/m): demag (1/4pi grad phi of
+ * charge = div M), exchange (2A/(mu0 Ms^2) lap M) and the external ramp
+ * (1/mu0) max(0, 0.3 - 0.3 t/1ns) x.  charge/potential receive the
+ * intermediate fields as in the reference (when demag != 0).
+ * Replaces: int Hfield(...)  ode_functions.hpp:8-15, ode_functions.cpp:21-70. */
+int Hfield(const float *M, float *H, float *charge, float *potential, float t,
+           int xdim, int ydim, int zdim, float dx, float dy, float dz, int P,
+           float mu_0, float Ms, float Aexch, int demag, int exchange, int external,
+           int use_fmm, int use_gpu, int verbosity);
+
+/* One classic RK4 step of the Landau-Lifshitz equation from M (with H = field
+ * at M on entry) to M2, updating material cells only; renormalises |M2| = Ms
+ * when `normalize`.  On return H holds the field at the last stage, slope =
+ * k1 + 2 k2 + 2 k3 + k4, this_slope = k4, *t2 = t + dt.  Stage combination and
+ * renormalisation are carried out in double precision.
+ * Replaces: int rk4_step(...)  ode_functions.cpp:76-148 (no header). */
+int rk4_step(float t, float dt, float *t2, const unsigned char *material,
+             const float *M, float *M2, float *H, float *slope, float *this_slope,
+             float *charge, float *potential, int xdim, int ydim, int zdim,
+             float dx, float dy, float dz, int P, float mu_0, float Ms, float Aexch,
+             float alfa, float gamma, int demag, int exchange, int external,
+             int use_fmm, int normalize, int use_gpu, int verbosity);
+
+/* Adaptive time marching with the reference policy: energy-increase
+ * rejection (dt /= 2.5, stop after 50 consecutive rejections), accepted-step
+ * dt = 1e-14 / torque / 2, stop when torque < 1e-3 or t > finaltime; writes
+ * <sim_name>/dynamics.dat, Mdynamics.dat and panic.dat in the reference's
+ * text formats.  M is overwritten with the final state.  The interactive
+ * pause after step 1 is not reproduced.
+ * Replaces: int time_marching(...)  ode_functions.hpp:17-24,
+ *           ode_functions.cpp:158-365. */
+int time_marching(unsigned char *material, float *M, float finaltime, float timestep,
+                  int xdim, int ydim, int zdim, float dx, float dy, float dz, int P,
+                  float mu_0, float Ms, float Aexch, float alfa, float gamma,
+                  int demag, int exchange, int external, int use_fmm, int use_gpu,
+                  const char *sim_name, int verbosity);
+
+/* ------------------------------------------------------------------------ */
+/* Part 2: resident API (no counterpart in the reference: it replaces the     */
+/* reference's function-level statics with an explicit handle)              */
+/* ------------------------------------------------------------------------ */
+
+typedef struct mms_ctx mms_ctx;
+
+enum mms_solver {
+    MMS_SOLVER_DIRECT = 0, /* O(N^2) direct sum                                */
+    MMS_SOLVER_FMM = 1     /* P2M, M2M, M2L, L2L, L2P + near-field P2P          */
+};
+enum mms_m2l_precision {
+    MMS_M2L_FP32 = 0,   /* FP32 operands, FP32 accumulation                   */
+    MMS_M2L_BF16 = 1,   /* BF16 operands, FP32 accumulation (MFMA)            */
+    MMS_M2L_BF16X3 = 2  /* hi/lo BF16 split of both operands, 3 MFMA products */
+};
+enum mms_external {
+    MMS_EXT_NONE = 0,
+    MMS_EXT_RAMP = 1,   /* the reference ramp, ode_functions.cpp:61 */
+    MMS_EXT_CONST = 2   /* constant hext[3] in A/m                  */
+};
+
+typedef struct {
+    int xdim, ydim, zdim;       /* grid; cell index z*ydim*xdim + y*xdim + x   */
+    float dx, dy, dz;           /* cell size (m)                               */
+    int solver;                 /* enum mms_solver                             */
+    int P;                      /* FMM expansion order                         */
+    int m2l_precision;          /* enum mms_m2l_precision                      */
+    int leaf_width;             /* FMM leaf box width in cells (2, 4 or 8; 0 = 4) */
+    float mu_0, Ms, Aexch, alfa, gamma;
+    int demag, exchange;        /* 0/1 switches                                */
+    int external;               /* enum mms_external                           */
+    float hext[3];              /* A/m, for MMS_EXT_CONST                      */
+} mms_params;
+
+/* Create a context owning device state for one film.  material: one byte per
+ * cell (non-zero = magnetic), may be NULL (all magnetic). */
+int mms_create(mms_ctx **ctx, const mms_params *p, const unsigned char *material);
+int mms_destroy(mms_ctx *ctx);
+
+/* State transfer (host AoS x,y,z float, A/m).  set resets t to 0. */
+int mms_set_m(mms_ctx *ctx, const float *M);
+int mms_get_m(mms_ctx *ctx, float *M);
+/* Set the simulation time used by the ramp field. */
+int mms_set_time(mms_ctx *ctx, double t);
+
+/* Advance nsteps fixed-dt RK4 steps on the device (4 field evaluations per
+ * step; the stage-1 field of each step is the post-step field of the previous
+ * one, as in the reference time loop).  Enqueued on the context's stream;
+ * returns once the work is enqueued.  use_graph != 0 replays one captured
+ * step as a hipGraph. */
+int mms_steps(mms_ctx *ctx, int nsteps, float dt, int use_graph);
+int mms_sync(mms_ctx *ctx);
+
+/* Field at the current state (host AoS output, A/m); also energy (eV) and
+ * max torque |M x H|/Ms^2 as in the reference time loop. */
+int mms_hfield(mms_ctx *ctx, float *H);
+int mms_energy_torque(mms_ctx *ctx, double *E, double *torque);
+
+/* Device-pointer potential solve: rho_dev, phi_dev are device float arrays
+ * of xdim*ydim*zdim cells (charge density in A/m^2, potential as
+ * calc_potential_exact).  Enqueued on the context's stream. */
+int mms_potential_dev(mms_ctx *ctx, const float *rho_dev, float *phi_dev);
+
+/* Host-array potential through the context's solver (synchronous). */
+int mms_potential(mms_ctx *ctx, const float *charge, float *potential);
+
+/* FP64 direct-sum potential at selected cells (device FP64 arithmetic), used
+ * to measure solver error on large grids: out[i] = phi(probe[i]). */
+int mms_potential_probes_f64(mms_ctx *ctx, const float *charge, const int64_t *probes,
+                             int nprobe, double *out);
+
+/* Timing: mean device time (ms) per launch of each kernel class over the last
+ * mms_profile_end() window; names are returned through mms_profile_name(). */
+int mms_profile_begin(mms_ctx *ctx);
+int mms_profile_end(mms_ctx *ctx);
+int mms_profile_count(mms_ctx *ctx);
+const char *mms_profile_name(mms_ctx *ctx, int i);
+double mms_profile_ms(mms_ctx *ctx, int i);
+long mms_profile_launches(mms_ctx *ctx, int i);
+
+/* Message for the last failing call on this host thread ("" if none). */
+const char *mms_last_error(void);
+/* Library build identifier (target arch, version). */
+const char *mms_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MMS_GPU_H */

@@ -1,0 +1,717 @@
+// api.cpp -- C ABI of libmmsgpu.so: contexts, the reference-mirroring entry
+// points (calc_potential_exact, fmm_calc, Hfield, rk4_step, time_marching) and
+// the resident fixed-step driver.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <sys/time.h>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mms_internal.hpp"
+
+namespace mms {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+const char *kclass_name(int k) {
+    static const char *names[K_COUNT] = {"div",      "stage",   "direct",  "direct_reduce", "p2m", "m2m",
+                                         "m2l",      "p2p_l2p", "hfield",  "energy",        "probes", "misc"};
+    return (k >= 0 && k < K_COUNT) ? names[k] : "?";
+}
+
+int Profiler::begin(int k, hipEvent_t *a, hipEvent_t *b) {
+    (void)k;
+    for (hipEvent_t *e : {a, b}) {
+        if (!pool.empty()) {
+            *e = pool.back();
+            pool.pop_back();
+        } else if (hipEventCreate(e) != hipSuccess) {
+            *a = nullptr;
+            return 1;
+        }
+    }
+    if (hipEventRecord(*a, stream) != hipSuccess) { *a = nullptr; return 1; }
+    return 0;
+}
+int Profiler::end_record(int k, hipEvent_t a, hipEvent_t b) {
+    if (hipEventRecord(b, stream) != hipSuccess) return 1;
+    pending.push_back({k, a, b});
+    if (pending.size() > 4096) return flush();
+    return 0;
+}
+int Profiler::flush() {
+    for (const Pending &p : pending) {
+        float t = 0.0f;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) {
+            ms[p.k] += t;
+            count[p.k]++;
+        }
+        pool.push_back(p.a);
+        pool.push_back(p.b);
+    }
+    pending.clear();
+    return 0;
+}
+void Profiler::reset() {
+    for (int k = 0; k < K_COUNT; k++) { ms[k] = 0; count[k] = 0; }
+}
+Profiler::~Profiler() {
+    for (const Pending &p : pending) { pool.push_back(p.a); pool.push_back(p.b); }
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
+}  // namespace mms
+
+using namespace mms;
+
+// ============================================================== context ==
+static int ctx_alloc(mms_ctx *c, const unsigned char *material) {
+    const long n = c->g.n();
+    for (int i = 0; i < 3; i++) MMS_HIP(hipMalloc(&c->m32[i], sizeof(float) * 3 * n));
+    MMS_HIP(hipMalloc(&c->mn64, sizeof(double) * 3 * n));
+    MMS_HIP(hipMalloc(&c->mnew64, sizeof(double) * 3 * n));
+    MMS_HIP(hipMalloc(&c->acc64, sizeof(double) * 3 * n));
+    MMS_HIP(hipMalloc(&c->rho, sizeof(float) * n));
+    MMS_HIP(hipMalloc(&c->phi, sizeof(float) * n));
+    MMS_HIP(hipMalloc(&c->phi_n, sizeof(float) * n));
+    MMS_HIP(hipMalloc(&c->matl, n));
+    MMS_HIP(hipMalloc(&c->t_dev, sizeof(double)));
+    MMS_HIP(hipMalloc(&c->h_aos, sizeof(float) * 3 * n));
+    MMS_HIP(hipMalloc(&c->aux_aos, sizeof(float) * 9 * n));
+    c->red_blocks = 1024;
+    MMS_HIP(hipMalloc(&c->red, sizeof(double) * 5 * c->red_blocks));
+    MMS_HIP(hipMemsetAsync(c->m32[0], 0, sizeof(float) * 3 * n, c->stream));
+    MMS_HIP(hipMemsetAsync(c->m32[1], 0, sizeof(float) * 3 * n, c->stream));
+    MMS_HIP(hipMemsetAsync(c->m32[2], 0, sizeof(float) * 3 * n, c->stream));
+    MMS_HIP(hipMemsetAsync(c->mn64, 0, sizeof(double) * 3 * n, c->stream));
+    MMS_HIP(hipMemsetAsync(c->acc64, 0, sizeof(double) * 3 * n, c->stream));
+    MMS_HIP(hipMemsetAsync(c->t_dev, 0, sizeof(double), c->stream));
+    if (material) {
+        MMS_HIP(hipMemcpyAsync(c->matl, material, n, hipMemcpyHostToDevice, c->stream));
+    } else {
+        MMS_HIP(hipMemsetAsync(c->matl, 1, n, c->stream));
+    }
+    MMS_HIP(hipStreamSynchronize(c->stream));
+    if (material) {
+        c->host_matl.assign(material, material + n);
+    } else {
+        c->host_matl.assign(n, 1);
+    }
+    return 0;
+}
+
+static void ctx_free(mms_ctx *c) {
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+    void *bufs[] = {c->m32[0], c->m32[1], c->m32[2], c->mn64, c->mnew64, c->acc64, c->rho, c->phi,
+                    c->phi_n, c->matl, c->t_dev, c->h_aos, c->aux_aos, c->red};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    c->direct.release();
+    c->fmm.release();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    c->stream = nullptr;
+}
+
+int mms_ctx::potential(const float *rho_d, float *phi_d) {
+    if (p.solver == MMS_SOLVER_FMM) return fmm_potential(g, fmm, rho_d, phi_d, stream, &prof);
+    return direct_potential(g, direct, rho_d, phi_d, stream, &prof);
+}
+
+// one fixed-dt RK4 step on the device: four (div, potential, stage) passes;
+// the stage-0 field is evaluated at M_n.  State returns to m32[cur].
+int mms_ctx::one_step(float dt) {
+    const int a = cur, b = 1 - cur;  // ping-pong between m32[0] and m32[1]
+    const float *in[4] = {m32[a], m32[b], m32[a], m32[b]};
+    float *out[4] = {m32[b], m32[a], m32[b], m32[a]};
+    for (int s = 0; s < 4; s++) {
+        if (p.demag) {
+            if (launch_div(g, in[s], rho, stream, &prof)) return 1;
+            if (potential(rho, phi)) return 1;
+        }
+        if (launch_stage(g, mat, s, (double)dt, phi, in[s], out[s], mn64, acc64, matl, t_dev, stream, &prof))
+            return 1;
+    }
+    if (p.external == MMS_EXT_RAMP && launch_add_time(t_dev, (double)dt, stream)) return 1;
+    return 0;
+}
+
+static int validate_params(const mms_params *p) {
+    MMS_CHECK(p != nullptr, "null parameters");
+    MMS_CHECK(p->xdim > 0 && p->ydim > 0 && p->zdim > 0, "grid dimensions must be positive");
+    MMS_CHECK((long)p->xdim * p->ydim * p->zdim < (1L << 31), "grid too large (>= 2^31 cells)");
+    MMS_CHECK(p->dx > 0 && p->dy > 0 && p->dz > 0, "cell sizes must be positive");
+    MMS_CHECK(p->solver == MMS_SOLVER_DIRECT || p->solver == MMS_SOLVER_FMM, "unknown solver");
+    MMS_CHECK(p->m2l_precision >= MMS_M2L_FP32 && p->m2l_precision <= MMS_M2L_BF16X3, "unknown M2L precision");
+    return 0;
+}
+
+extern "C" int mms_create(mms_ctx **out, const mms_params *p, const unsigned char *material) {
+    if (!out) { set_error("null context pointer"); return 1; }
+    *out = nullptr;
+    if (validate_params(p)) return 1;
+    std::unique_ptr<mms_ctx> c(new mms_ctx());
+    c->p = *p;
+    if (c->p.leaf_width == 0) c->p.leaf_width = 4;
+    c->g.xd = p->xdim; c->g.yd = p->ydim; c->g.zd = p->zdim;
+    c->g.dx = p->dx; c->g.dy = p->dy; c->g.dz = p->dz;
+    c->g.ay = p->dy / p->dx; c->g.az = p->dz / p->dx;
+    c->mat.mu_0 = p->mu_0; c->mat.Ms = p->Ms; c->mat.Aexch = p->Aexch; c->mat.alfa = p->alfa;
+    c->mat.gamma = p->gamma; c->mat.demag = p->demag; c->mat.exchange = p->exchange;
+    c->mat.external = p->external;
+    for (int i = 0; i < 3; i++) c->mat.hext[i] = p->hext[i];
+    MMS_HIP(hipGetDevice(&c->device));
+    MMS_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->prof.stream = c->stream;
+    if (ctx_alloc(c.get(), material)) { ctx_free(c.get()); return 1; }
+    int st = 0;
+    if (c->p.solver == MMS_SOLVER_FMM) st = c->fmm.init(c->g, c->p.P, c->p.leaf_width, c->p.m2l_precision);
+    else st = c->direct.init(c->g);
+    if (st) { ctx_free(c.get()); return 1; }
+    *out = c.release();
+    return 0;
+}
+
+extern "C" int mms_destroy(mms_ctx *c) {
+    if (!c) return 0;
+    ctx_free(c);
+    delete c;
+    return 0;
+}
+
+static int upload_m(mms_ctx *c, const float *M) {
+    const long n = c->g.n();
+    float *staging = c->aux_aos;  // device AoS staging
+    MMS_HIP(hipMemcpyAsync(staging, M, sizeof(float) * 3 * n, hipMemcpyHostToDevice, c->stream));
+    if (launch_soa_from_aos(staging, c->m32[c->cur], c->mn64, n, c->stream)) return 1;
+    return 0;
+}
+
+extern "C" int mms_set_m(mms_ctx *c, const float *M) {
+    MMS_CHECK(c && M, "null argument");
+    if (upload_m(c, M)) return 1;
+    c->t_host = 0.0;
+    MMS_HIP(hipMemsetAsync(c->t_dev, 0, sizeof(double), c->stream));
+    MMS_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int mms_get_m(mms_ctx *c, float *M) {
+    MMS_CHECK(c && M, "null argument");
+    const long n = c->g.n();
+    if (launch_aos_from_soa(c->m32[c->cur], c->aux_aos, n, c->stream)) return 1;
+    MMS_HIP(hipMemcpyAsync(M, c->aux_aos, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    MMS_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int mms_set_time(mms_ctx *c, double t) {
+    MMS_CHECK(c, "null context");
+    c->t_host = t;
+    MMS_HIP(hipMemcpyAsync(c->t_dev, &c->t_host, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    MMS_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int mms_steps(mms_ctx *c, int nsteps, float dt, int use_graph) {
+    MMS_CHECK(c, "null context");
+    MMS_CHECK(nsteps >= 0, "negative step count");
+    if (!use_graph || c->prof.on) {
+        for (int i = 0; i < nsteps; i++)
+            if (c->one_step(dt)) return 1;
+        c->t_host += (double)nsteps * dt;
+        return 0;
+    }
+    if (!c->graph || c->graph_dt != dt) {
+        if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+        hipGraph_t gr;
+        MMS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int st = c->one_step(dt);
+        hipError_t e = hipStreamEndCapture(c->stream, &gr);
+        if (st) return 1;
+        MMS_HIP(e);
+        hipError_t e2 = hipGraphInstantiate(&c->graph, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        MMS_HIP(e2);
+        c->graph_dt = dt;
+    }
+    for (int i = 0; i < nsteps; i++) MMS_HIP(hipGraphLaunch(c->graph, c->stream));
+    c->t_host += (double)nsteps * dt;
+    return 0;
+}
+
+extern "C" int mms_sync(mms_ctx *c) {
+    MMS_CHECK(c, "null context");
+    MMS_HIP(hipStreamSynchronize(c->stream));
+    if (c->prof.on) c->prof.flush();
+    return 0;
+}
+
+// field at the current state into c->h_aos (and rho/phi)
+static int field_now(mms_ctx *c, const float *m) {
+    if (c->p.demag) {
+        if (launch_div(c->g, m, c->rho, c->stream, &c->prof)) return 1;
+        if (c->potential(c->rho, c->phi)) return 1;
+    }
+    return launch_hfield(c->g, c->mat, c->phi, m, c->t_dev, c->h_aos, c->stream, &c->prof);
+}
+
+extern "C" int mms_hfield(mms_ctx *c, float *H) {
+    MMS_CHECK(c && H, "null argument");
+    if (field_now(c, c->m32[c->cur])) return 1;
+    MMS_HIP(hipMemcpyAsync(H, c->h_aos, sizeof(float) * 3 * c->g.n(), hipMemcpyDeviceToHost, c->stream));
+    MMS_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// energy (eV) and torque from h_aos at state m (reference units)
+static int energy_torque_of(mms_ctx *c, const float *m, double *E, double *T) {
+    const long n = c->g.n();
+    if (launch_energy_h(m, c->h_aos, c->matl, n, c->red, c->red_blocks, c->stream, &c->prof)) return 1;
+    std::vector<double> part(2 * c->red_blocks);
+    MMS_HIP(hipMemcpyAsync(part.data(), c->red, sizeof(double) * 2 * c->red_blocks, hipMemcpyDeviceToHost,
+                           c->stream));
+    MMS_HIP(hipStreamSynchronize(c->stream));
+    double e = 0, t = 0;
+    for (int i = 0; i < c->red_blocks; i++) {
+        e += part[2 * i];
+        t = t > part[2 * i + 1] ? t : part[2 * i + 1];
+    }
+    const double dV = (double)(c->g.dx * c->g.dy * c->g.dz);
+    *E = e * (-0.5 * (double)c->p.mu_0 * dV / 1.6e-19);
+    *T = t / c->p.Ms / c->p.Ms;
+    return 0;
+}
+
+extern "C" int mms_energy_torque(mms_ctx *c, double *E, double *T) {
+    MMS_CHECK(c && E && T, "null argument");
+    if (field_now(c, c->m32[c->cur])) return 1;
+    return energy_torque_of(c, c->m32[c->cur], E, T);
+}
+
+extern "C" int mms_potential_dev(mms_ctx *c, const float *rho_dev, float *phi_dev) {
+    MMS_CHECK(c && rho_dev && phi_dev, "null argument");
+    return c->potential(rho_dev, phi_dev);
+}
+
+extern "C" int mms_potential(mms_ctx *c, const float *charge, float *potential) {
+    MMS_CHECK(c && charge && potential, "null argument");
+    const long n = c->g.n();
+    MMS_HIP(hipMemcpyAsync(c->rho, charge, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    if (c->potential(c->rho, c->phi)) return 1;
+    MMS_HIP(hipMemcpyAsync(potential, c->phi, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    MMS_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int mms_potential_probes_f64(mms_ctx *c, const float *charge, const int64_t *probes, int np,
+                                        double *out) {
+    MMS_CHECK(c && charge && probes && out && np >= 0, "bad argument");
+    const long n = c->g.n();
+    for (int i = 0; i < np; i++) MMS_CHECK(probes[i] >= 0 && probes[i] < n, "probe index out of range");
+    int64_t *dp = nullptr;
+    double *dout = nullptr;
+    MMS_HIP(hipMemcpyAsync(c->rho, charge, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    MMS_HIP(hipMalloc(&dp, sizeof(int64_t) * (np > 0 ? np : 1)));
+    MMS_HIP(hipMalloc(&dout, sizeof(double) * (np > 0 ? np : 1)));
+    int st = 0;
+    if (hipMemcpyAsync(dp, probes, sizeof(int64_t) * np, hipMemcpyHostToDevice, c->stream) != hipSuccess) st = 1;
+    if (!st) st = direct_probes_f64(c->g, c->rho, dp, np, dout, c->stream, &c->prof);
+    if (!st && hipMemcpyAsync(out, dout, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream) != hipSuccess) st = 1;
+    if (!st && hipStreamSynchronize(c->stream) != hipSuccess) st = 1;
+    (void)hipFree(dp);
+    (void)hipFree(dout);
+    if (st) set_error("probe evaluation failed");
+    return st;
+}
+
+extern "C" int mms_profile_begin(mms_ctx *c) {
+    MMS_CHECK(c, "null context");
+    c->prof.flush();
+    c->prof.reset();
+    c->prof.on = true;
+    return 0;
+}
+extern "C" int mms_profile_end(mms_ctx *c) {
+    MMS_CHECK(c, "null context");
+    MMS_HIP(hipStreamSynchronize(c->stream));
+    c->prof.flush();
+    c->prof.on = false;
+    return 0;
+}
+extern "C" int mms_profile_count(mms_ctx *c) { (void)c; return K_COUNT; }
+extern "C" const char *mms_profile_name(mms_ctx *c, int i) { (void)c; return kclass_name(i); }
+extern "C" double mms_profile_ms(mms_ctx *c, int i) {
+    if (!c || i < 0 || i >= K_COUNT || c->prof.count[i] == 0) return 0.0;
+    return c->prof.ms[i] / (double)c->prof.count[i];
+}
+extern "C" long mms_profile_launches(mms_ctx *c, int i) {
+    if (!c || i < 0 || i >= K_COUNT) return 0;
+    return c->prof.count[i];
+}
+
+extern "C" const char *mms_last_error(void) { return g_err.c_str(); }
+extern "C" const char *mms_version(void) { return "mmsgpu 0.1 gfx950"; }
+
+// ============================================== reference entry points ==
+// Contexts for the Part-1 calls are cached per (shape, spacing, solver, P,
+// material constants); at most a few are kept, least recently used evicted.
+namespace {
+struct CacheEntry {
+    mms_params p;
+    std::vector<unsigned char> material;
+    mms_ctx *ctx;
+    unsigned long used;
+};
+std::mutex g_cache_mu;
+std::vector<CacheEntry> g_cache;
+unsigned long g_clock = 0;
+
+bool same_params(const mms_params &a, const mms_params &b) { return memcmp(&a, &b, sizeof(mms_params)) == 0; }
+
+mms_ctx *cached_ctx(const mms_params &p, const unsigned char *material) {
+    const long n = (long)p.xdim * p.ydim * p.zdim;
+    for (CacheEntry &e : g_cache) {
+        if (!same_params(e.p, p)) continue;
+        if (material) {
+            if (e.material.size() != (size_t)n || memcmp(e.material.data(), material, n) != 0) continue;
+        } else if (!e.material.empty()) {
+            continue;
+        }
+        e.used = ++g_clock;
+        return e.ctx;
+    }
+    if (g_cache.size() >= 4) {
+        size_t victim = 0;
+        for (size_t i = 1; i < g_cache.size(); i++)
+            if (g_cache[i].used < g_cache[victim].used) victim = i;
+        mms_destroy(g_cache[victim].ctx);
+        g_cache.erase(g_cache.begin() + victim);
+    }
+    mms_ctx *c = nullptr;
+    if (mms_create(&c, &p, material)) return nullptr;
+    CacheEntry e{p, {}, c, ++g_clock};
+    if (material) e.material.assign(material, material + n);
+    g_cache.push_back(e);
+    return c;
+}
+
+mms_params base_params(int xd, int yd, int zd, float dx, float dy, float dz) {
+    mms_params p;
+    memset(&p, 0, sizeof(p));
+    p.xdim = xd; p.ydim = yd; p.zdim = zd;
+    p.dx = dx; p.dy = dy; p.dz = dz;
+    p.solver = MMS_SOLVER_DIRECT;
+    p.P = 4;
+    p.leaf_width = 4;
+    p.demag = 1;
+    return p;
+}
+
+// choose the FMM leaf width for a grid (the smallest padded square needs at
+// least 4 leaf boxes per side); returns 0 when the grid is too small for the
+// FMM and the direct sum should be used instead.
+int fmm_leaf_for(int xd, int yd) {
+    const int side = xd > yd ? xd : yd;
+    int s = 1;
+    while (s < side) s <<= 1;
+    if (s >= 16) return 4;
+    if (s >= 8) return 2;
+    return 0;
+}
+
+void set_solver(mms_params &p, int use_fmm, int P) {
+    if (use_fmm) {
+        const int w = fmm_leaf_for(p.xdim, p.ydim);
+        if (w > 0) {
+            p.solver = MMS_SOLVER_FMM;
+            p.P = P;
+            p.leaf_width = w;
+            return;
+        }
+    }
+    p.solver = MMS_SOLVER_DIRECT;
+    p.P = 0;
+}
+}  // namespace
+
+extern "C" int calc_potential_exact(const float *charge, int xdim, int ydim, int zdim, float dx, float dy,
+                                    float dz, float *potential, int use_gpu, int verbosity) {
+    (void)use_gpu;
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    timeval t1, t2;
+    gettimeofday(&t1, nullptr);
+    mms_params p = base_params(xdim, ydim, zdim, dx, dy, dz);
+    mms_ctx *c = cached_ctx(p, nullptr);
+    if (!c || mms_potential(c, charge, potential)) return EXIT_FAILURE;
+    gettimeofday(&t2, nullptr);
+    if (verbosity >= 10)
+        printf("Exact: took %f seconds\n", (t2.tv_sec + t2.tv_usec / 1e6) - (t1.tv_sec + t1.tv_usec / 1e6));
+    return EXIT_SUCCESS;
+}
+
+extern "C" int calc_potential_exact_gpu(const float *charge, int xdim, int ydim, int zdim, float dx, float dy,
+                                        float dz, float *potential) {
+    return calc_potential_exact(charge, xdim, ydim, zdim, dx, dy, dz, potential, 1, 0);
+}
+
+extern "C" int fmm_calc(const float *charge, float *potential, int xdim, int ydim, int zdim, float dx, float dy,
+                        float dz, int P, int use_gpu, int verbosity) {
+    (void)use_gpu;
+    if (P < 1 || P > 10) { set_error("fmm_calc: P must be in 1..10"); return EXIT_FAILURE; }
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    timeval t1, t2;
+    gettimeofday(&t1, nullptr);
+    mms_params p = base_params(xdim, ydim, zdim, dx, dy, dz);
+    set_solver(p, 1, P);
+    mms_ctx *c = cached_ctx(p, nullptr);
+    if (!c || mms_potential(c, charge, potential)) return EXIT_FAILURE;
+    gettimeofday(&t2, nullptr);
+    if (verbosity >= 10)
+        printf("FMM: took %f seconds\n", (t2.tv_sec + t2.tv_usec / 1e6) - (t1.tv_sec + t1.tv_usec / 1e6));
+    return EXIT_SUCCESS;
+}
+
+static int mms_set_time_nolock(mms_ctx *c, double t) { return mms_set_time(c, t); }
+
+static mms_params field_params(int xdim, int ydim, int zdim, float dx, float dy, float dz, int P, float mu_0,
+                               float Ms, float Aexch, int demag, int exchange, int external, int use_fmm) {
+    mms_params p = base_params(xdim, ydim, zdim, dx, dy, dz);
+    set_solver(p, use_fmm, P);
+    p.mu_0 = mu_0; p.Ms = Ms; p.Aexch = Aexch;
+    p.demag = demag; p.exchange = exchange;
+    p.external = external ? MMS_EXT_RAMP : MMS_EXT_NONE;
+    return p;
+}
+
+extern "C" int Hfield(const float *M, float *H, float *charge, float *potential, float t, int xdim, int ydim,
+                      int zdim, float dx, float dy, float dz, int P, float mu_0, float Ms, float Aexch, int demag,
+                      int exchange, int external, int use_fmm, int use_gpu, int verbosity) {
+    (void)use_gpu; (void)verbosity;
+    if (use_fmm && (P < 1 || P > 10)) { set_error("Hfield: P must be in 1..10"); return EXIT_FAILURE; }
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    mms_params p = field_params(xdim, ydim, zdim, dx, dy, dz, P, mu_0, Ms, Aexch, demag, exchange, external, use_fmm);
+    mms_ctx *c = cached_ctx(p, nullptr);
+    if (!c) return EXIT_FAILURE;
+    const long n = c->g.n();
+    if (upload_m(c, M) || mms_set_time_nolock(c, (double)t)) return EXIT_FAILURE;
+    if (field_now(c, c->m32[c->cur])) return EXIT_FAILURE;
+    if (hipMemcpyAsync(H, c->h_aos, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        return EXIT_FAILURE;
+    if (demag) {
+        if (charge && hipMemcpyAsync(charge, c->rho, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            return EXIT_FAILURE;
+        if (potential &&
+            hipMemcpyAsync(potential, c->phi, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            return EXIT_FAILURE;
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) { set_error("Hfield: stream failure"); return EXIT_FAILURE; }
+    return EXIT_SUCCESS;
+}
+
+extern "C" int rk4_step(float t, float dt, float *t2, const unsigned char *material, const float *M, float *M2,
+                        float *H, float *slope, float *this_slope, float *charge, float *potential, int xdim,
+                        int ydim, int zdim, float dx, float dy, float dz, int P, float mu_0, float Ms, float Aexch,
+                        float alfa, float gamma, int demag, int exchange, int external, int use_fmm, int normalize,
+                        int use_gpu, int verbosity) {
+    (void)use_gpu; (void)verbosity;
+    if (use_fmm && (P < 1 || P > 10)) { set_error("rk4_step: P must be in 1..10"); return EXIT_FAILURE; }
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    mms_params p = field_params(xdim, ydim, zdim, dx, dy, dz, P, mu_0, Ms, Aexch, demag, exchange, external, use_fmm);
+    p.alfa = alfa; p.gamma = gamma;
+    mms_ctx *c = cached_ctx(p, material);
+    if (!c) return EXIT_FAILURE;
+    const long n = c->g.n();
+    hipStream_t s = c->stream;
+    // device staging: aux_aos = [H in | k out | slope out]
+    float *h_in = c->aux_aos, *k_out = c->aux_aos + 3 * n, *s_out = c->aux_aos + 6 * n;
+    if (mms_set_time_nolock(c, (double)t)) return EXIT_FAILURE;
+    if (hipMemcpyAsync(c->h_aos, M, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        launch_soa_from_aos(c->h_aos, c->m32[0], c->mn64, n, s) ||
+        hipMemcpyAsync(h_in, H, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s) != hipSuccess) {
+        set_error("rk4_step: upload failed");
+        return EXIT_FAILURE;
+    }
+    StageExtra ex{nullptr, nullptr, nullptr, nullptr, normalize, nullptr};
+    const float *in[4] = {c->m32[0], c->m32[1], c->m32[2], c->m32[1]};
+    float *outp[4] = {c->m32[1], c->m32[2], c->m32[1], c->m32[2]};
+    for (int st = 0; st < 4; st++) {
+        ex.h_given = (st == 0) ? h_in : nullptr;
+        ex.h_out = (st == 3) ? c->h_aos : nullptr;
+        ex.k_out = (st == 3) ? k_out : nullptr;
+        ex.slope_out = (st == 3) ? s_out : nullptr;
+        if (st > 0 && demag) {
+            if (launch_div(c->g, in[st], c->rho, s, &c->prof) || c->potential(c->rho, c->phi)) return EXIT_FAILURE;
+        }
+        if (launch_stage_ex(c->g, c->mat, st, (double)dt, c->phi, in[st], outp[st], c->mn64, c->acc64, c->matl,
+                            c->t_dev, ex, s, &c->prof))
+            return EXIT_FAILURE;
+    }
+    // outputs
+    float *m_aos = c->aux_aos;  // h_in no longer needed
+    if (launch_aos_from_soa(c->m32[2], m_aos, n, s)) return EXIT_FAILURE;
+    bool ok = hipMemcpyAsync(M2, m_aos, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s) == hipSuccess &&
+              hipMemcpyAsync(H, c->h_aos, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s) == hipSuccess;
+    if (ok && slope) ok = hipMemcpyAsync(slope, s_out, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s) == hipSuccess;
+    if (ok && this_slope)
+        ok = hipMemcpyAsync(this_slope, k_out, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s) == hipSuccess;
+    if (ok && demag && charge)
+        ok = hipMemcpyAsync(charge, c->rho, sizeof(float) * n, hipMemcpyDeviceToHost, s) == hipSuccess;
+    if (ok && demag && potential)
+        ok = hipMemcpyAsync(potential, c->phi, sizeof(float) * n, hipMemcpyDeviceToHost, s) == hipSuccess;
+    if (ok) ok = hipStreamSynchronize(s) == hipSuccess;
+    if (!ok) { set_error("rk4_step: download failed"); return EXIT_FAILURE; }
+    // the reference leaves M2 untouched on non-material cells
+    (void)material;
+    if (t2) *t2 = t + dt;
+    return EXIT_SUCCESS;
+}
+
+// ---------------------------------------------------------- time loop --
+static int mavg_of(mms_ctx *c, const float *m, double out[5]) {
+    if (launch_mavg(m, c->matl, c->g.n(), c->red, c->red_blocks, c->stream)) return 1;
+    std::vector<double> part(5 * c->red_blocks);
+    MMS_HIP(hipMemcpyAsync(part.data(), c->red, sizeof(double) * 5 * c->red_blocks, hipMemcpyDeviceToHost, c->stream));
+    MMS_HIP(hipStreamSynchronize(c->stream));
+    for (int q = 0; q < 5; q++) out[q] = 0;
+    for (int i = 0; i < c->red_blocks; i++)
+        for (int q = 0; q < 5; q++) out[q] += part[5 * i + q];
+    return 0;
+}
+
+static int write_vector3d(FILE *fh, const float *M, int zd, int yd, int xd) {
+    for (int z = 0; z < zd; z++)
+        for (int x = 0; x < xd; x++)
+            for (int y = 0; y < yd; y++) {
+                const float *v = M + 3 * ((long)z * yd * xd + (long)y * xd + x);
+                fprintf(fh, "%g %g %g \n", v[0], v[1], v[2]);
+            }
+    return 0;
+}
+
+extern "C" int time_marching(unsigned char *material, float *M, float finaltime, float timestep, int xdim, int ydim,
+                             int zdim, float dx, float dy, float dz, int P, float mu_0, float Ms, float Aexch,
+                             float alfa, float gamma, int demag, int exchange, int external, int use_fmm,
+                             int use_gpu, const char *sim_name, int verbosity) {
+    (void)use_gpu;
+    if (use_fmm && (P < 1 || P > 10)) { set_error("time_marching: P must be in 1..10"); return EXIT_FAILURE; }
+    mms_params p = field_params(xdim, ydim, zdim, dx, dy, dz, P, mu_0, Ms, Aexch, demag, exchange, external, use_fmm);
+    p.alfa = alfa; p.gamma = gamma;
+    mms_ctx *c = nullptr;
+    if (mms_create(&c, &p, material)) return EXIT_FAILURE;
+    std::unique_ptr<mms_ctx, int (*)(mms_ctx *)> guard(c, mms_destroy);
+    const long n = c->g.n();
+    hipStream_t s = c->stream;
+    char fname[1100];
+    const char *dir = sim_name ? sim_name : ".";
+    mkdir(dir, 0755);
+    snprintf(fname, sizeof fname, "%s/%s", dir, "dynamics.dat");
+    FILE *fh = fopen(fname, "w");
+    snprintf(fname, sizeof fname, "%s/%s", dir, "Mdynamics.dat");
+    FILE *fhM = fopen(fname, "w");
+    snprintf(fname, sizeof fname, "%s/%s", dir, "panic.dat");
+    FILE *fhp = fopen(fname, "w");
+    if (!fh || !fhM || !fhp) {
+        if (fh) fclose(fh);
+        if (fhM) fclose(fhM);
+        if (fhp) fclose(fhp);
+        set_error(std::string("time_marching: cannot open output files in ") + dir);
+        return EXIT_FAILURE;
+    }
+    int status = 0;
+    if (upload_m(c, M)) status = 1;
+    // buffers: m32[cur] = M_n (f32), mn64 = M_n (f64); phi_n = potential at M_n
+    double dt = timestep, t = 0.0;
+    int tindex = 0, consecutive_error_count = 0;
+    std::vector<float> hostM(3 * n);
+    auto field_into = [&](const float *m, float *phi_out) -> int {
+        if (demag) {
+            if (launch_div(c->g, m, c->rho, s, &c->prof) || c->potential(c->rho, phi_out)) return 1;
+        }
+        return launch_hfield(c->g, c->mat, phi_out, m, c->t_dev, c->h_aos, s, &c->prof);
+    };
+    float E = 0;
+    double Ed = 0, torque0 = 0;
+    if (!status && mms_set_time_nolock(c, 0.0)) status = 1;
+    if (!status && field_into(c->m32[c->cur], c->phi_n)) status = 1;
+    bool h_valid = true;  // h_aos holds the field at M_n
+    while (!status && t <= finaltime) {
+        if (mms_set_time_nolock(c, t)) { status = 1; break; }
+        if (!h_valid && launch_hfield(c->g, c->mat, c->phi_n, c->m32[c->cur], c->t_dev, c->h_aos, s, &c->prof)) {
+            status = 1;
+            break;
+        }
+        if (energy_torque_of(c, c->m32[c->cur], &Ed, &torque0)) { status = 1; break; }
+        E = (float)Ed;
+        // RK4 step: stages read/write m32 slots other than cur; stage 3 proposes mnew64
+        const int a = c->cur, b = (c->cur + 1) % 3, d = (c->cur + 2) % 3;
+        const float *in[4] = {c->m32[a], c->m32[b], c->m32[d], c->m32[b]};
+        float *outp[4] = {c->m32[b], c->m32[d], c->m32[b], c->m32[d]};
+        StageExtra ex{nullptr, nullptr, nullptr, nullptr, 1, c->mnew64};
+        for (int st = 0; st < 4 && !status; st++) {
+            const float *phi_use = (st == 0) ? c->phi_n : c->phi;
+            if (st > 0 && demag) {
+                if (launch_div(c->g, in[st], c->rho, s, &c->prof) || c->potential(c->rho, c->phi)) { status = 1; break; }
+            }
+            if (launch_stage_ex(c->g, c->mat, st, dt, phi_use, in[st], outp[st], c->mn64, c->acc64, c->matl, c->t_dev,
+                                ex, s, &c->prof))
+                status = 1;
+        }
+        if (status) break;
+        // post-step field at the proposal (m32[d]); potential kept in phi
+        double E2d = 0, torqued = 0;
+        if (field_into(c->m32[d], c->phi) || energy_torque_of(c, c->m32[d], &E2d, &torqued)) { status = 1; break; }
+        const float E2 = (float)E2d, torque = (float)torqued;
+        if (E2 - E > .1) {
+            h_valid = false;
+            dt = (float)(dt / 2.5);
+            printf("PANIC!! energy is increasing, so halving the step.\n");
+            fprintf(fhp, "PANIC!! energy is increasing, so halving the step. %d, %g, %g, %g, %g \n", tindex, t, dt,
+                    E2, torque);
+            consecutive_error_count++;
+            if (consecutive_error_count > 50) {
+                fprintf(stdout, "PANIC!! Stuck in energy violation errors. Terminating the simulation.\n");
+                fprintf(fhp, "PANIC!! Stuck in energy violation errors. Terminating the simulation.\n");
+                break;
+            }
+        } else {
+            consecutive_error_count = 0;
+            h_valid = true;
+            const double t2 = (double)((float)t + (float)dt);
+            dt = (float)(1e-14 / torque / 2);
+            t = t2;
+            tindex++;
+            c->cur = d;
+            std::swap(c->mn64, c->mnew64);
+            std::swap(c->phi_n, c->phi);
+            // log line
+            double avg[5];
+            if (mavg_of(c, c->m32[c->cur], avg)) { status = 1; break; }
+            if (mms_get_m(c, hostM.data())) { status = 1; break; }
+            write_vector3d(fhM, hostM.data(), zdim, ydim, xdim);
+            const double cnt = avg[4] > 0 ? avg[4] : 1;
+            fprintf(fh, "%d, %g, %g, %g, %g, %g, %g, %g, %g \n", tindex, t, dt, E2, avg[0] / cnt / Ms,
+                    avg[1] / cnt / Ms, avg[2] / cnt / Ms, avg[3] / cnt / Ms, torque);
+            if (verbosity >= 1)
+                fprintf(stdout, "%d, %g, %g, %g, %g, %g, %g, %g, %g \n", tindex, t, dt, E2, avg[0] / cnt / Ms,
+                        avg[1] / cnt / Ms, avg[2] / cnt / Ms, avg[3] / cnt / Ms, torque);
+            if (torque < 1e-3) {
+                printf("Torque is too low. Breaking simulation...\n");
+                break;
+            }
+        }
+        fflush(NULL);
+    }
+    if (!status && mms_get_m(c, M)) status = 1;
+    fclose(fh);
+    fclose(fhM);
+    fclose(fhp);
+    return status ? EXIT_FAILURE : EXIT_SUCCESS;
+}

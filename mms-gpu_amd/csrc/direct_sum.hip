@@ -1,0 +1,156 @@
+// direct_sum.hip -- O(N^2) scalar potential on gfx950.
+//
+// phi_i = (dV/dx) * sum_{j != i} rho_j / |r_i - r_j|_u   with distances in units
+// of dx (r_u = (dxi, dyi*ay, dzi*az)), the computation of the reference's
+// calc_potential_exact (fmm_calc.cu:412-461) and calc_potential_exact_kernel
+// (potential_calc.cu:18-116).
+//
+// Work split: a 2D grid of (target block of 256) x (source chunk).  A block
+// stages its chunk through LDS in tiles of 256 sources as float4 (x, y, z,
+// rho) built from the linear index once per tile; every lane then walks the
+// tile with LDS broadcast reads (one ds_read_b128 per source per wave).  Each
+// lane accumulates a tile in FP32 and folds the tile sum into an FP64
+// accumulator, so summation error grows with the tile, not with N.  Chunk
+// partials go to an FP64 scratch [nchunk][n] and a second kernel adds them in
+// fixed order (bitwise reproducible run to run).  The self pair is excluded by
+// r^2 > 0.
+#include "mms_internal.hpp"
+
+namespace mms {
+namespace {
+
+constexpr int TB = 256;    // targets per block (one per lane)
+constexpr int TILE = 256;  // sources per LDS tile
+
+__global__ __launch_bounds__(TB) void direct_partial_kernel(const float *__restrict__ rho, double *__restrict__ part,
+                                                            int xd, int yd, int zd, float ay, float az,
+                                                            long n, long chunk) {
+    __shared__ float4 src[TILE];
+    const long t = (long)blockIdx.x * TB + threadIdx.x;
+    const long c0 = (long)blockIdx.y * chunk;
+    long c1 = c0 + chunk;
+    if (c1 > n) c1 = n;
+    const long plane = (long)xd * yd;
+    float tx = 0.f, ty = 0.f, tz = 0.f;
+    if (t < n) {
+        tx = (float)(t % xd);
+        ty = (float)((t / xd) % yd) * ay;
+        tz = (float)(t / plane) * az;
+    }
+    double acc = 0.0;
+    for (long base = c0; base < c1; base += TILE) {
+        const long j = base + threadIdx.x;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (j < c1) {
+            v.x = (float)(j % xd);
+            v.y = (float)((j / xd) % yd) * ay;
+            v.z = (float)(j / plane) * az;
+            v.w = rho[j];
+        }
+        __syncthreads();
+        src[threadIdx.x] = v;
+        __syncthreads();
+        const int cnt = (int)((c1 - base) < TILE ? (c1 - base) : TILE);
+        float s = 0.f;
+#pragma unroll 8
+        for (int k = 0; k < cnt; k++) {
+            const float4 q = src[k];
+            const float ddx = tx - q.x, ddy = ty - q.y, ddz = tz - q.z;
+            const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            const float inv = (r2 > 0.f) ? rsqrtf(r2) : 0.f;
+            s = fmaf(q.w, inv, s);
+        }
+        acc += (double)s;
+    }
+    if (t < n) part[(long)blockIdx.y * n + t] = acc;
+}
+
+__global__ __launch_bounds__(256) void direct_reduce_kernel(const double *__restrict__ part, float *__restrict__ phi,
+                                                            long n, int nchunk, double scale) {
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    double s = 0.0;
+    for (int c = 0; c < nchunk; c++) s += part[(long)c * n + t];
+    phi[t] = (float)(s * scale);
+}
+
+// FP64 probe oracle on the device: one block per probe, all sources.
+__global__ __launch_bounds__(256) void probes_f64_kernel(const float *__restrict__ rho, const int64_t *__restrict__ probes,
+                                                         double *__restrict__ out, int xd, int yd, int zd,
+                                                         double dx, double dy, double dz) {
+    __shared__ double red[256];
+    const long n = (long)xd * yd * zd, plane = (long)xd * yd;
+    const long i = probes[blockIdx.x];
+    const int tx = (int)(i % xd), ty = (int)((i / xd) % yd), tz = (int)(i / plane);
+    double acc = 0.0;
+    for (long j = threadIdx.x; j < n; j += 256) {
+        const int sx = (int)(j % xd), sy = (int)((j / xd) % yd), sz = (int)(j / plane);
+        const double rx = (double)(tx - sx) * dx, ry = (double)(ty - sy) * dy, rz = (double)(tz - sz) * dz;
+        const double r2 = rx * rx + ry * ry + rz * rz;
+        if (r2 > 0.0) acc += (double)rho[j] / sqrt(r2);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0] * dx * dy * dz;
+}
+
+}  // namespace
+
+int DirectPlan::init(const Grid &g) {
+    release();
+    const long n = g.n();
+    const long tblocks = (n + TB - 1) / TB;
+    // aim for >= 1024 blocks in total (4 per CU) without chunks below one tile
+    long want = (1024 + tblocks - 1) / tblocks;
+    long maxchunks = (n + TILE - 1) / TILE;
+    if (want > maxchunks) want = maxchunks;
+    if (want < 1) want = 1;
+    chunk = (n + want - 1) / want;
+    chunk = ((chunk + TILE - 1) / TILE) * TILE;
+    nchunk = (int)((n + chunk - 1) / chunk);
+    MMS_HIP(hipMalloc(&part, sizeof(double) * (size_t)n * nchunk));
+    return 0;
+}
+
+void DirectPlan::release() {
+    if (part) (void)hipFree(part);
+    part = nullptr;
+    nchunk = 0;
+}
+
+int direct_potential(const Grid &g, DirectPlan &plan, const float *rho, float *phi, hipStream_t s,
+                     Profiler *prof) {
+    const long n = g.n();
+    MMS_CHECK(plan.part != nullptr, "direct solver plan not initialised");
+    {
+        KTimer kt(prof, K_DIRECT);
+        dim3 grid((unsigned)((n + TB - 1) / TB), (unsigned)plan.nchunk);
+        hipLaunchKernelGGL(direct_partial_kernel, grid, dim3(TB), 0, s, rho, plan.part, g.xd, g.yd, g.zd,
+                           g.ay, g.az, n, plan.chunk);
+        MMS_HIP(hipGetLastError());
+    }
+    {
+        KTimer kt(prof, K_DIRECT_REDUCE);
+        const double scale = (double)(g.dx * g.dy * g.dz) / (double)g.dx;
+        hipLaunchKernelGGL(direct_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, plan.part,
+                           phi, n, plan.nchunk, scale);
+        MMS_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
+int direct_probes_f64(const Grid &g, const float *rho, const int64_t *probes, int np, double *out,
+                      hipStream_t s, Profiler *prof) {
+    KTimer kt(prof, K_PROBES);
+    if (np <= 0) return 0;
+    hipLaunchKernelGGL(probes_f64_kernel, dim3(np), dim3(256), 0, s, rho, probes, out, g.xd, g.yd, g.zd,
+                       (double)g.dx, (double)g.dy, (double)g.dz);
+    MMS_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace mms

@@ -1,0 +1,191 @@
+// mms_internal.hpp -- shared declarations of the gfx950 library (not installed).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include <map>
+
+#include "../../include/mms_gpu.h"
+
+namespace mms {
+
+// ---------------------------------------------------------------- errors --
+void set_error(const std::string &msg);
+#define MMS_HIP(call)                                                            \
+    do {                                                                         \
+        hipError_t e_ = (call);                                                  \
+        if (e_ != hipSuccess) {                                                  \
+            ::mms::set_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+            return 1;                                                            \
+        }                                                                        \
+    } while (0)
+#define MMS_CHECK(cond, msg)           \
+    do {                                \
+        if (!(cond)) {                  \
+            ::mms::set_error(msg);      \
+            return 1;                   \
+        }                               \
+    } while (0)
+
+// ------------------------------------------------------------- geometry --
+// Cell (x,y,z) at linear index z*ydim*xdim + y*xdim + x.  Distances are
+// measured in units of dx; ay = dy/dx, az = dz/dx.
+struct Grid {
+    int xd, yd, zd;
+    float dx, dy, dz;
+    float ay, az;
+    long n() const { return (long)xd * yd * zd; }
+    long plane() const { return (long)xd * yd; }
+};
+
+// ------------------------------------------------------------- profiling --
+// Kernel classes timed with hipEvents when profiling is on.
+enum KClass {
+    K_DIV = 0, K_STAGE, K_DIRECT, K_DIRECT_REDUCE, K_P2M, K_M2M, K_M2L, K_P2P_L2P,
+    K_HFIELD, K_ENERGY, K_PROBES, K_MISC, K_COUNT
+};
+const char *kclass_name(int k);
+
+struct Profiler {
+    bool on = false;
+    hipStream_t stream = nullptr;
+    struct Pending { int k; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    double ms[K_COUNT] = {0};
+    long count[K_COUNT] = {0};
+    int begin(int k, hipEvent_t *a, hipEvent_t *b);
+    int end_record(int k, hipEvent_t a, hipEvent_t b);
+    int flush();
+    void reset();
+    ~Profiler();
+};
+
+// Helper that brackets a launch with profiler events (no-op when off).
+struct KTimer {
+    Profiler *p; int k; hipEvent_t a = nullptr, b = nullptr;
+    KTimer(Profiler *p_, int k_) : p(p_), k(k_) { if (p && p->on) p->begin(k, &a, &b); }
+    ~KTimer() { if (p && p->on && a) p->end_record(k, a, b); }
+};
+
+// --------------------------------------------------------- direct solver --
+struct DirectPlan {
+    int nchunk = 0;      // source chunks
+    long chunk = 0;      // sources per chunk
+    double *part = nullptr;  // [nchunk][n] partial sums
+    int init(const Grid &g);
+    void release();
+};
+int direct_potential(const Grid &g, DirectPlan &plan, const float *rho, float *phi,
+                     hipStream_t s, Profiler *prof);
+int direct_probes_f64(const Grid &g, const float *rho, const int64_t *probes, int np,
+                      double *out, hipStream_t s, Profiler *prof);
+
+// ------------------------------------------------------------ FMM solver --
+// Regular-grid FMM: per-layer quadtrees over a zero-padded 2^Lmax square,
+// leaf boxes of w x w cells at level Lleaf; expansions are truncated at order
+// P and keep the (n+m even) coefficients only (in-plane sources and targets),
+// stored as K = (P+1)(P+2)/2 reals per box and layer.
+struct FmmOps;  // host FP64 operators (fmm_operators.cpp)
+struct FmmPlan {
+    int P = 0, K = 0, w = 4, Lmax = 0, Lleaf = 0, zd = 0;
+    int precision = MMS_M2L_FP32;
+    // device buffers
+    float *mult = nullptr;   // all levels, [level][z][by][bx][K]
+    float *loc = nullptr;    // all levels, same layout
+    std::vector<long> lvl_off;  // float offset of each level in mult/loc
+    float *d_p2m = nullptr;  // [K][w*w]
+    float *d_l2p = nullptr;  // [w*w][K]
+    float *d_m2m = nullptr;  // [4][K][K]
+    float *d_l2l = nullptr;  // [4][K][K]
+    float *d_m2l = nullptr;  // [level][dz][49 offsets][K][K] (level 2..Lleaf)
+    uint16_t *d_m2l_bf = nullptr;  // bf16 hi (and lo) copies for MFMA paths
+    float *d_near = nullptr; // near-field kernel table 1/r over offsets
+    int near_side = 0;       // 4w-1
+    long m2l_level_stride = 0;
+    int init(const Grid &g, int P, int w, int precision);
+    void release();
+};
+int fmm_potential(const Grid &g, FmmPlan &plan, const float *rho, float *phi,
+                  hipStream_t s, Profiler *prof);
+
+// Host-side operator construction (FP64), real bases as described above.
+struct FmmHostOps {
+    int P, K, w;
+    std::vector<double> p2m;   // [K][w*w]
+    std::vector<double> l2p;   // [w*w][K]
+    std::vector<double> m2m;   // [4][K][K]
+    std::vector<double> l2l;   // [4][K][K]
+    // M2L for one level: [2zd-1][49][K][K] (offset index (oy+3)*7 + (ox+3))
+    std::vector<double> m2l_level(int level, int Lmax, int zd, double ay, double az) const;
+};
+int fmm_build_ops(int P, int w, double ay, FmmHostOps &ops);
+int fmm_ncoef(int P);
+
+// -------------------------------------------------------- stencil / LLG --
+struct Material {
+    float mu_0, Ms, Aexch, alfa, gamma;
+    int demag, exchange, external;
+    float hext[3];
+};
+int launch_div(const Grid &g, const float *m, float *rho, hipStream_t s, Profiler *prof);
+// stage in 0..3; m_in/m_out FP32 SoA [3][n]; mn/acc FP64 SoA [3][n]; t_dev: device double
+int launch_stage(const Grid &g, const Material &mat, int stage, double dt, const float *phi,
+                 const float *m_in, float *m_out, double *mn, double *acc,
+                 const uint8_t *matl, const double *t_dev, hipStream_t s, Profiler *prof);
+// H (AoS float) at state m (SoA float); phi may be null if !demag
+int launch_hfield(const Grid &g, const Material &mat, const float *phi, const float *m,
+                  const double *t_dev, float *H_aos, hipStream_t s, Profiler *prof);
+// Block partials of sum(M.H) and max |M x H| over material cells; H (AoS)
+// must have been computed by launch_hfield.  partial: [nblk][2] doubles.
+int launch_energy_h(const float *m, const float *h_aos, const uint8_t *matl, long n,
+                    double *partial, int nblk, hipStream_t s, Profiler *prof);
+// Stage kernel with the rk4_step drop-in's extra inputs/outputs (AoS):
+// h_given replaces the computed field; h_out/k_out/slope_out receive the field,
+// dM/dt and the float slope sum.  Null pointers disable each.
+struct StageExtra { const float *h_given; float *h_out; float *k_out; float *slope_out; int normalize; double *mn_out; };
+int launch_stage_ex(const Grid &g, const Material &mat, int stage, double dt, const float *phi,
+                    const float *m_in, float *m_out, double *mn, double *acc, const uint8_t *matl,
+                    const double *t_dev, const StageExtra &ex, hipStream_t s, Profiler *prof);
+// Block partials [nblk][5] of (sum Mx, My, Mz, |M|, count) over material cells
+int launch_mavg(const float *m, const uint8_t *matl, long n, double *partial, int nblk, hipStream_t s);
+int launch_soa_from_aos(const float *aos, float *soa32, double *soa64, long n, hipStream_t s);
+int launch_aos_from_soa(const float *soa32, float *aos, long n, hipStream_t s);
+int launch_add_time(double *t_dev, double dt, hipStream_t s);
+int launch_fill_f32(float *p, float v, long n, hipStream_t s);
+
+}  // namespace mms
+
+// --------------------------------------------------------------- context --
+struct mms_ctx {
+    mms_params p;
+    mms::Grid g;
+    mms::Material mat;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    // device state (SoA planes [3][n])
+    float *m32[3] = {nullptr, nullptr, nullptr};  // FP32 stage magnetisation slots
+    double *mn64 = nullptr;    // FP64 base M_n
+    double *mnew64 = nullptr;  // FP64 proposal M_{n+1} (adaptive time loop)
+    double *acc64 = nullptr;   // FP64 slope accumulator
+    float *rho = nullptr, *phi = nullptr;
+    float *phi_n = nullptr;    // potential at M_n (adaptive time loop)
+    uint8_t *matl = nullptr;
+    double *t_dev = nullptr;
+    float *h_aos = nullptr;    // field output / staging, AoS [n][3]
+    float *aux_aos = nullptr;  // staging, 3 AoS arrays
+    double *red = nullptr;     // reduction scratch [red_blocks][5]
+    int red_blocks = 0;
+    int cur = 0;               // slot of m32 holding the current state
+    double t_host = 0.0;
+    std::vector<unsigned char> host_matl;
+    mms::DirectPlan direct;
+    mms::FmmPlan fmm;
+    mms::Profiler prof;
+    // graph of one RK4 step
+    hipGraphExec_t graph = nullptr;
+    float graph_dt = 0.0f;
+    int potential(const float *rho_d, float *phi_d);
+    int one_step(float dt);
+};

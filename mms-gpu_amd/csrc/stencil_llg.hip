@@ -1,0 +1,401 @@
+// stencil_llg.hip -- HBM-bound per-cell kernels of the RK4 LLG step on gfx950:
+//   div_kernel     rho = div M                          (vector_functions.cpp:8-28)
+//   stage_kernel   H = grad(phi)/4pi + exchange + H_ext (vector_functions.cpp:31-103,
+//                  ode_functions.cpp:21-70), dM/dt (ode_functions.cpp:11-18) and the
+//                  RK4 stage combine (ode_functions.cpp:76-148) fused in one pass,
+//                  stage combination and |M| = Ms renormalisation in FP64.
+//   hfield_kernel  H only (for the Hfield entry point and diagnostics)
+//   energy_kernel  sum M.H and max |M x H| (ode_functions.cpp:234-240,268-278)
+//
+// Layout: M stage values are FP32 SoA planes [3][n] (x plane, y plane, z plane),
+// the RK4 base M_n and the slope accumulator are FP64 SoA [3][n].  Each thread
+// owns one (x,y) column and walks z, keeping the z-1/z/z+1 values in registers;
+// in-plane neighbours come from the caches (blocks are 64 x 4 columns, so the
+// y-neighbour rows of a block are resident in L1/L2).
+// Out-of-grid neighbours read as 0 (the reference's zero padding: smeared
+// surface charge, Dirichlet-zero exchange).
+#include "mms_internal.hpp"
+
+namespace mms {
+
+namespace {
+
+constexpr int BX = 64, BY = 4;
+
+__global__ __launch_bounds__(BX * BY) void div_kernel(const float *__restrict__ m, float *__restrict__ rho,
+                                                      int xd, int yd, int zd, float dx, float dy, float dz) {
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y;
+    if (x >= xd || y >= yd) return;
+    const long plane = (long)xd * yd, n = plane * zd;
+    const float *mx = m, *my = m + n, *mz = m + 2 * n;
+    const long c0 = (long)y * xd + x;
+    float zprev = 0.0f, zcur = mz[c0];
+    for (int z = 0; z < zd; z++) {
+        const long c = (long)z * plane + c0;
+        const float znext = (z != zd - 1) ? mz[c + plane] : 0.0f;
+        const float x2 = (x != xd - 1) ? mx[c + 1] : 0.0f;
+        const float x0 = (x != 0) ? mx[c - 1] : 0.0f;
+        const float y2 = (y != yd - 1) ? my[c + xd] : 0.0f;
+        const float y0 = (y != 0) ? my[c - xd] : 0.0f;
+        rho[c] = ((x2 - x0) / dx + (y2 - y0) / dy + (znext - zprev) / dz) * 0.5f;
+        zprev = zcur;
+        zcur = znext;
+    }
+}
+
+struct StageArgs {
+    const float *phi;      // potential (null when demag off or h_given)
+    const float *m_in;     // FP32 SoA stage magnetisation
+    float *m_out;          // FP32 SoA next stage magnetisation
+    double *mn;            // FP64 SoA base M_n (updated at stage 3)
+    double *acc;           // FP64 SoA slope accumulator
+    double *mn_out;        // FP64 SoA destination of M_{n+1} at stage 3 (null = mn)
+    const uint8_t *matl;   // material flags (null = all magnetic)
+    const double *t_dev;   // simulation time (device)
+    const float *h_given;  // AoS field to use instead of computing it (stage 0 of rk4_step)
+    float *h_out;          // optional AoS output of the field used
+    float *k_out;          // optional AoS output of this stage's dM/dt
+    float *slope_out;      // optional AoS output of the float slope sum
+    int xd, yd, zd;
+    float dx, dy, dz;
+    float inv4pi;          // (float)(1/(4 pi))
+    float exch_cm;         // 2 A / (mu0 Ms^2)
+    int demag, exchange, external;
+    float hx, hy, hz;      // constant external field (A/m), external == 2
+    float inv_mu0;         // (float)(1/mu0), external == 1
+    float alfa, gamma, Ms;
+    int stage;             // 0..3
+    double dt;
+    int normalize;
+};
+
+// External field of the reference ramp (ode_functions.cpp:61): the ramp is
+// evaluated in double and only the scalar multiplier is rounded to float.
+__device__ inline float ramp_hx(double t, float inv_mu0) {
+    double amp = (t < 1e-9) ? (-0.3 / 1e-9 * t + 0.3) : 0.0;
+    return (float)((double)inv_mu0 * amp);
+}
+
+__device__ inline void llg_rhs(float Mx, float My, float Mz, float Hx, float Hy, float Hz,
+                               float alfa, float gamma, float Ms, float &kx, float &ky, float &kz) {
+    // dM/dt = -gamma M x H - (alfa gamma / Ms) M x (M x H)   (ode_functions.cpp:11-18)
+    const float cx = My * Hz - Mz * Hy, cy = Mz * Hx - Mx * Hz, cz = Mx * Hy - My * Hx;
+    const float ddx = My * cz - Mz * cy, ddy = Mz * cx - Mx * cz, ddz = Mx * cy - My * cx;
+    const float a = alfa * gamma / Ms;
+    kx = cx * (-gamma) - ddx * a;
+    ky = cy * (-gamma) - ddy * a;
+    kz = cz * (-gamma) - ddz * a;
+}
+
+__global__ __launch_bounds__(BX * BY) void stage_kernel(StageArgs A) {
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y;
+    const int xd = A.xd, yd = A.yd, zd = A.zd;
+    if (x >= xd || y >= yd) return;
+    const long plane = (long)xd * yd, n = plane * zd;
+    const long c0 = (long)y * xd + x;
+    const float *mx = A.m_in, *my = A.m_in + n, *mz = A.m_in + 2 * n;
+    const double t = A.t_dev ? *A.t_dev : 0.0;
+    float hex = A.hx, hey = A.hy, hez = A.hz;
+    if (A.external == 1) { hex = ramp_hx(t, A.inv_mu0); hey = 0.0f; hez = 0.0f; }
+    const float dx2 = A.dx * A.dx, dy2 = A.dy * A.dy, dz2 = A.dz * A.dz;
+    const bool need_field = (A.h_given == nullptr);
+
+    // z-rotation registers
+    float pprev = 0.0f, pcur = 0.0f;
+    float mxp = 0.0f, myp = 0.0f, mzp = 0.0f;
+    float mxc = mx[c0], myc = my[c0], mzc = mz[c0];
+    if (need_field && A.demag) pcur = A.phi[c0];
+
+    for (int z = 0; z < zd; z++) {
+        const long c = (long)z * plane + c0;
+        const bool ztop = (z != zd - 1);
+        float mxn = 0.0f, myn = 0.0f, mzn = 0.0f, pnext = 0.0f;
+        if (ztop) { mxn = mx[c + plane]; myn = my[c + plane]; mzn = mz[c + plane]; }
+        float Hx, Hy, Hz;
+        if (need_field) {
+            Hx = 0.0f; Hy = 0.0f; Hz = 0.0f;
+            if (A.demag) {
+                if (ztop) pnext = A.phi[c + plane];
+                const float x2 = (x != xd - 1) ? A.phi[c + 1] : 0.0f;
+                const float x0 = (x != 0) ? A.phi[c - 1] : 0.0f;
+                const float y2 = (y != yd - 1) ? A.phi[c + xd] : 0.0f;
+                const float y0 = (y != 0) ? A.phi[c - xd] : 0.0f;
+                Hx = (x2 - x0) / (2.0f * A.dx) * A.inv4pi;
+                Hy = (y2 - y0) / (2.0f * A.dy) * A.inv4pi;
+                Hz = (pnext - pprev) / (2.0f * A.dz) * A.inv4pi;
+            }
+            if (A.exchange) {
+                const bool xr = (x != xd - 1), xl = (x != 0), yu = (y != yd - 1), yl = (y != 0);
+                float Lx, Ly, Lz;
+                {
+                    const float xa = xr ? mx[c + 1] : 0.0f, xb = xl ? mx[c - 1] : 0.0f;
+                    const float ya = yu ? mx[c + xd] : 0.0f, yb = yl ? mx[c - xd] : 0.0f;
+                    Lx = (xa + xb - 2.0f * mxc) / dx2 + (ya + yb - 2.0f * mxc) / dy2 + (mxn + mxp - 2.0f * mxc) / dz2;
+                }
+                {
+                    const float xa = xr ? my[c + 1] : 0.0f, xb = xl ? my[c - 1] : 0.0f;
+                    const float ya = yu ? my[c + xd] : 0.0f, yb = yl ? my[c - xd] : 0.0f;
+                    Ly = (xa + xb - 2.0f * myc) / dx2 + (ya + yb - 2.0f * myc) / dy2 + (myn + myp - 2.0f * myc) / dz2;
+                }
+                {
+                    const float xa = xr ? mz[c + 1] : 0.0f, xb = xl ? mz[c - 1] : 0.0f;
+                    const float ya = yu ? mz[c + xd] : 0.0f, yb = yl ? mz[c - xd] : 0.0f;
+                    Lz = (xa + xb - 2.0f * mzc) / dx2 + (ya + yb - 2.0f * mzc) / dy2 + (mzn + mzp - 2.0f * mzc) / dz2;
+                }
+                Hx += A.exch_cm * Lx; Hy += A.exch_cm * Ly; Hz += A.exch_cm * Lz;
+            }
+            if (A.external) { Hx += hex; Hy += hey; Hz += hez; }
+        } else {
+            Hx = A.h_given[3 * c]; Hy = A.h_given[3 * c + 1]; Hz = A.h_given[3 * c + 2];
+        }
+        if (A.h_out) { A.h_out[3 * c] = Hx; A.h_out[3 * c + 1] = Hy; A.h_out[3 * c + 2] = Hz; }
+
+        const bool magnetic = A.matl ? (A.matl[c] != 0) : true;
+        float *ox = A.m_out, *oy = A.m_out + n, *oz = A.m_out + 2 * n;
+        if (magnetic) {
+            float kx, ky, kz;
+            llg_rhs(mxc, myc, mzc, Hx, Hy, Hz, A.alfa, A.gamma, A.Ms, kx, ky, kz);
+            if (A.k_out) { A.k_out[3 * c] = kx; A.k_out[3 * c + 1] = ky; A.k_out[3 * c + 2] = kz; }
+            double *bx = A.mn, *by = A.mn + n, *bz = A.mn + 2 * n;
+            double *ax = A.acc, *ay = A.acc + n, *az = A.acc + 2 * n;
+            const double b0 = bx[c], b1 = by[c], b2 = bz[c];
+            double s0, s1, s2;
+            if (A.stage == 0) { s0 = kx; s1 = ky; s2 = kz; }
+            else { s0 = ax[c] + (A.stage == 3 ? 1.0 : 2.0) * kx;
+                   s1 = ay[c] + (A.stage == 3 ? 1.0 : 2.0) * ky;
+                   s2 = az[c] + (A.stage == 3 ? 1.0 : 2.0) * kz; }
+            if (A.slope_out) { A.slope_out[3 * c] = (float)s0; A.slope_out[3 * c + 1] = (float)s1; A.slope_out[3 * c + 2] = (float)s2; }
+            double n0, n1, n2;
+            if (A.stage < 3) {
+                const double h = (A.stage < 2 ? 0.5 : 1.0) * A.dt;
+                n0 = b0 + h * (double)kx; n1 = b1 + h * (double)ky; n2 = b2 + h * (double)kz;
+                ax[c] = s0; ay[c] = s1; az[c] = s2;
+            } else {
+                const double h = A.dt / 6.0;
+                n0 = b0 + h * s0; n1 = b1 + h * s1; n2 = b2 + h * s2;
+                if (A.normalize) {
+                    const double f = (double)A.Ms / sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+                    n0 *= f; n1 *= f; n2 *= f;
+                }
+                double *o64 = A.mn_out ? A.mn_out : A.mn;
+                o64[c] = n0; o64[n + c] = n1; o64[2 * n + c] = n2;
+            }
+            ox[c] = (float)n0; oy[c] = (float)n1; oz[c] = (float)n2;
+        } else {
+            ox[c] = mxc; oy[c] = myc; oz[c] = mzc;
+        }
+        pprev = pcur; pcur = pnext;
+        mxp = mxc; myp = myc; mzp = mzc;
+        mxc = mxn; myc = myn; mzc = mzn;
+    }
+}
+
+__global__ __launch_bounds__(BX * BY) void hfield_kernel(StageArgs A) {
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y;
+    const int xd = A.xd, yd = A.yd, zd = A.zd;
+    if (x >= xd || y >= yd) return;
+    const long plane = (long)xd * yd, n = plane * zd;
+    const long c0 = (long)y * xd + x;
+    const float *mx = A.m_in, *my = A.m_in + n, *mz = A.m_in + 2 * n;
+    const double t = A.t_dev ? *A.t_dev : 0.0;
+    float hex = A.hx, hey = A.hy, hez = A.hz;
+    if (A.external == 1) { hex = ramp_hx(t, A.inv_mu0); hey = 0.0f; hez = 0.0f; }
+    const float dx2 = A.dx * A.dx, dy2 = A.dy * A.dy, dz2 = A.dz * A.dz;
+    for (int z = 0; z < zd; z++) {
+        const long c = (long)z * plane + c0;
+        float Hx = 0.0f, Hy = 0.0f, Hz = 0.0f;
+        if (A.demag) {
+            const float x2 = (x != xd - 1) ? A.phi[c + 1] : 0.0f;
+            const float x0 = (x != 0) ? A.phi[c - 1] : 0.0f;
+            const float y2 = (y != yd - 1) ? A.phi[c + xd] : 0.0f;
+            const float y0 = (y != 0) ? A.phi[c - xd] : 0.0f;
+            const float z2 = (z != zd - 1) ? A.phi[c + plane] : 0.0f;
+            const float z0 = (z != 0) ? A.phi[c - plane] : 0.0f;
+            Hx = (x2 - x0) / (2.0f * A.dx) * A.inv4pi;
+            Hy = (y2 - y0) / (2.0f * A.dy) * A.inv4pi;
+            Hz = (z2 - z0) / (2.0f * A.dz) * A.inv4pi;
+        }
+        if (A.exchange) {
+            float L[3];
+            const float *comp[3] = {mx, my, mz};
+            for (int q = 0; q < 3; q++) {
+                const float *v = comp[q];
+                const float ctr = v[c];
+                const float xa = (x != xd - 1) ? v[c + 1] : 0.0f, xb = (x != 0) ? v[c - 1] : 0.0f;
+                const float ya = (y != yd - 1) ? v[c + xd] : 0.0f, yb = (y != 0) ? v[c - xd] : 0.0f;
+                const float za = (z != zd - 1) ? v[c + plane] : 0.0f, zb = (z != 0) ? v[c - plane] : 0.0f;
+                L[q] = (xa + xb - 2.0f * ctr) / dx2 + (ya + yb - 2.0f * ctr) / dy2 + (za + zb - 2.0f * ctr) / dz2;
+            }
+            Hx += A.exch_cm * L[0]; Hy += A.exch_cm * L[1]; Hz += A.exch_cm * L[2];
+        }
+        if (A.external) { Hx += hex; Hy += hey; Hz += hez; }
+        A.h_out[3 * c] = Hx; A.h_out[3 * c + 1] = Hy; A.h_out[3 * c + 2] = Hz;
+    }
+}
+
+// Block partials of E = sum M.H and T = max |M x H| over material cells.
+// H is read from h_aos (computed beforehand by hfield_kernel).
+__global__ __launch_bounds__(256) void energy_kernel(const float *__restrict__ m, const float *__restrict__ h_aos,
+                                                     const uint8_t *__restrict__ matl, long n,
+                                                     double *__restrict__ partial) {
+    __shared__ double se[256];
+    __shared__ double st[256];
+    double e = 0.0, tq = 0.0;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L) {
+        if (matl && !matl[i]) continue;
+        const double Mx = m[i], My = m[n + i], Mz = m[2 * n + i];
+        const double Hx = h_aos[3 * i], Hy = h_aos[3 * i + 1], Hz = h_aos[3 * i + 2];
+        e += Mx * Hx + My * Hy + Mz * Hz;
+        const double cx = My * Hz - Mz * Hy, cy = Mz * Hx - Mx * Hz, cz = Mx * Hy - My * Hx;
+        tq = fmax(tq, sqrt(cx * cx + cy * cy + cz * cz));
+    }
+    se[threadIdx.x] = e;
+    st[threadIdx.x] = tq;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            se[threadIdx.x] += se[threadIdx.x + s];
+            st[threadIdx.x] = fmax(st[threadIdx.x], st[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { partial[2 * blockIdx.x] = se[0]; partial[2 * blockIdx.x + 1] = st[0]; }
+}
+
+// Block partials of (sum Mx, sum My, sum Mz, sum |M|, count) over material
+// cells, for the time loop's log line (ode_functions.cpp:313-330).
+__global__ __launch_bounds__(256) void mavg_kernel(const float *__restrict__ m, const uint8_t *__restrict__ matl,
+                                                   long n, double *__restrict__ partial) {
+    __shared__ double red[5][256];
+    double a[5] = {0, 0, 0, 0, 0};
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L) {
+        if (matl && !matl[i]) continue;
+        const double x = m[i], y = m[n + i], z = m[2 * n + i];
+        a[0] += x; a[1] += y; a[2] += z; a[3] += sqrt(x * x + y * y + z * z); a[4] += 1.0;
+    }
+    for (int q = 0; q < 5; q++) red[q][threadIdx.x] = a[q];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int q = 0; q < 5; q++) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 5; q++) partial[5 * blockIdx.x + q] = red[q][0];
+}
+
+__global__ void soa_from_aos_kernel(const float *__restrict__ aos, float *__restrict__ s32, double *__restrict__ s64, long n) {
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L) {
+        const float a = aos[3 * i], b = aos[3 * i + 1], c = aos[3 * i + 2];
+        if (s32) { s32[i] = a; s32[n + i] = b; s32[2 * n + i] = c; }
+        if (s64) { s64[i] = a; s64[n + i] = b; s64[2 * n + i] = c; }
+    }
+}
+__global__ void aos_from_soa_kernel(const float *__restrict__ s32, float *__restrict__ aos, long n) {
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L) {
+        aos[3 * i] = s32[i]; aos[3 * i + 1] = s32[n + i]; aos[3 * i + 2] = s32[2 * n + i];
+    }
+}
+__global__ void add_time_kernel(double *t, double dt) { *t += dt; }
+__global__ void fill_f32_kernel(float *p, float v, long n) {
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L) p[i] = v;
+}
+
+inline int grid1d(long n) {
+    long b = (n + 255) / 256;
+    return (int)(b < 8192 ? (b < 1 ? 1 : b) : 8192);
+}
+
+StageArgs make_args(const Grid &g, const Material &mat) {
+    StageArgs A{};
+    A.xd = g.xd; A.yd = g.yd; A.zd = g.zd;
+    A.dx = g.dx; A.dy = g.dy; A.dz = g.dz;
+    A.inv4pi = (float)(1 / (4.0 * M_PI));
+    A.exch_cm = 2 * mat.Aexch / (mat.mu_0 * mat.Ms * mat.Ms);
+    A.demag = mat.demag; A.exchange = mat.exchange; A.external = mat.external;
+    A.hx = mat.hext[0]; A.hy = mat.hext[1]; A.hz = mat.hext[2];
+    A.inv_mu0 = 1 / mat.mu_0;
+    A.alfa = mat.alfa; A.gamma = mat.gamma; A.Ms = mat.Ms;
+    A.normalize = 1;
+    return A;
+}
+
+}  // namespace
+
+int launch_div(const Grid &g, const float *m, float *rho, hipStream_t s, Profiler *prof) {
+    KTimer kt(prof, K_DIV);
+    dim3 grid((g.xd + BX - 1) / BX, (g.yd + BY - 1) / BY), block(BX, BY);
+    hipLaunchKernelGGL(div_kernel, grid, block, 0, s, m, rho, g.xd, g.yd, g.zd, g.dx, g.dy, g.dz);
+    MMS_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_stage_ex(const Grid &g, const Material &mat, int stage, double dt, const float *phi,
+                    const float *m_in, float *m_out, double *mn, double *acc, const uint8_t *matl,
+                    const double *t_dev, const StageExtra &ex, hipStream_t s, Profiler *prof) {
+    KTimer kt(prof, K_STAGE);
+    StageArgs A = make_args(g, mat);
+    A.phi = phi; A.m_in = m_in; A.m_out = m_out; A.mn = mn; A.acc = acc; A.matl = matl;
+    A.t_dev = t_dev; A.stage = stage; A.dt = dt;
+    A.h_given = ex.h_given; A.h_out = ex.h_out; A.k_out = ex.k_out; A.slope_out = ex.slope_out;
+    A.normalize = ex.normalize; A.mn_out = ex.mn_out;
+    dim3 grid((g.xd + BX - 1) / BX, (g.yd + BY - 1) / BY), block(BX, BY);
+    hipLaunchKernelGGL(stage_kernel, grid, block, 0, s, A);
+    MMS_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_stage(const Grid &g, const Material &mat, int stage, double dt, const float *phi,
+                 const float *m_in, float *m_out, double *mn, double *acc, const uint8_t *matl,
+                 const double *t_dev, hipStream_t s, Profiler *prof) {
+    StageExtra ex{nullptr, nullptr, nullptr, nullptr, 1, nullptr};
+    return launch_stage_ex(g, mat, stage, dt, phi, m_in, m_out, mn, acc, matl, t_dev, ex, s, prof);
+}
+
+int launch_hfield(const Grid &g, const Material &mat, const float *phi, const float *m,
+                  const double *t_dev, float *H_aos, hipStream_t s, Profiler *prof) {
+    KTimer kt(prof, K_HFIELD);
+    StageArgs A = make_args(g, mat);
+    A.phi = phi; A.m_in = m; A.t_dev = t_dev; A.h_out = H_aos;
+    dim3 grid((g.xd + BX - 1) / BX, (g.yd + BY - 1) / BY), block(BX, BY);
+    hipLaunchKernelGGL(hfield_kernel, grid, block, 0, s, A);
+    MMS_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_energy_h(const float *m, const float *h_aos, const uint8_t *matl, long n,
+                    double *partial, int nblk, hipStream_t s, Profiler *prof) {
+    KTimer kt(prof, K_ENERGY);
+    hipLaunchKernelGGL(energy_kernel, dim3(nblk), dim3(256), 0, s, m, h_aos, matl, n, partial);
+    MMS_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_mavg(const float *m, const uint8_t *matl, long n, double *partial, int nblk, hipStream_t s) {
+    hipLaunchKernelGGL(mavg_kernel, dim3(nblk), dim3(256), 0, s, m, matl, n, partial);
+    MMS_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_soa_from_aos(const float *aos, float *soa32, double *soa64, long n, hipStream_t s) {
+    hipLaunchKernelGGL(soa_from_aos_kernel, dim3(grid1d(n)), dim3(256), 0, s, aos, soa32, soa64, n);
+    MMS_HIP(hipGetLastError());
+    return 0;
+}
+int launch_aos_from_soa(const float *soa32, float *aos, long n, hipStream_t s) {
+    hipLaunchKernelGGL(aos_from_soa_kernel, dim3(grid1d(n)), dim3(256), 0, s, soa32, aos, n);
+    MMS_HIP(hipGetLastError());
+    return 0;
+}
+int launch_add_time(double *t_dev, double dt, hipStream_t s) {
+    hipLaunchKernelGGL(add_time_kernel, dim3(1), dim3(1), 0, s, t_dev, dt);
+    MMS_HIP(hipGetLastError());
+    return 0;
+}
+int launch_fill_f32(float *p, float v, long n, hipStream_t s) {
+    hipLaunchKernelGGL(fill_f32_kernel, dim3(grid1d(n)), dim3(256), 0, s, p, v, n);
+    MMS_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace mms

@@ -1,0 +1,93 @@
+"""ctypes binding of libmmsgpu.so (the gfx950 HIP library).
+
+The library is built in-tree (``mms-gpu_amd/libmmsgpu.so``).  Loading fails
+loudly when it is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "libmmsgpu.so")
+
+_lib = None
+
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+I, F, D, VP = C.c_int, C.c_float, C.c_double, C.c_void_p
+
+
+class MmsParams(C.Structure):
+    """Mirror of ``mms_params`` in include/mms_gpu.h."""
+    _fields_ = [("xdim", I), ("ydim", I), ("zdim", I),
+                ("dx", F), ("dy", F), ("dz", F),
+                ("solver", I), ("P", I), ("m2l_precision", I), ("leaf_width", I),
+                ("mu_0", F), ("Ms", F), ("Aexch", F), ("alfa", F), ("gamma", F),
+                ("demag", I), ("exchange", I), ("external", I),
+                ("hext", F * 3)]
+
+
+# symbol -> (restype, argtypes); every symbol declared in include/mms_gpu.h
+SIGNATURES = {
+    "calc_potential_exact": (I, [f32p, I, I, I, F, F, F, f32p, I, I]),
+    "calc_potential_exact_gpu": (I, [f32p, I, I, I, F, F, F, f32p]),
+    "fmm_calc": (I, [f32p, f32p, I, I, I, F, F, F, I, I, I]),
+    "Hfield": (I, [f32p, f32p, VP, VP, F, I, I, I, F, F, F, I, F, F, F, I, I, I, I, I, I]),
+    "rk4_step": (I, [F, F, C.POINTER(F), u8p, f32p, f32p, f32p, VP, VP, VP, VP, I, I, I, F, F, F, I,
+                     F, F, F, F, F, I, I, I, I, I, I, I]),
+    "time_marching": (I, [u8p, f32p, F, F, I, I, I, F, F, F, I, F, F, F, F, F, I, I, I, I, I,
+                          C.c_char_p, I]),
+    "mms_create": (I, [C.POINTER(VP), C.POINTER(MmsParams), VP]),
+    "mms_destroy": (I, [VP]),
+    "mms_set_m": (I, [VP, f32p]),
+    "mms_get_m": (I, [VP, f32p]),
+    "mms_set_time": (I, [VP, D]),
+    "mms_steps": (I, [VP, I, F, I]),
+    "mms_sync": (I, [VP]),
+    "mms_hfield": (I, [VP, f32p]),
+    "mms_energy_torque": (I, [VP, C.POINTER(D), C.POINTER(D)]),
+    "mms_potential_dev": (I, [VP, VP, VP]),
+    "mms_potential": (I, [VP, f32p, f32p]),
+    "mms_potential_probes_f64": (I, [VP, f32p, i64p, I, f64p]),
+    "mms_profile_begin": (I, [VP]),
+    "mms_profile_end": (I, [VP]),
+    "mms_profile_count": (I, [VP]),
+    "mms_profile_name": (C.c_char_p, [VP, I]),
+    "mms_profile_ms": (D, [VP, I]),
+    "mms_profile_launches": (C.c_long, [VP, I]),
+    "mms_last_error": (C.c_char_p, []),
+    "mms_version": (C.c_char_p, []),
+}
+
+
+def lib():
+    """Load libmmsgpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                "(make -C mms-gpu_amd); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().mms_last_error().decode()
+
+
+class MmsError(RuntimeError):
+    pass
+
+
+def check(status, what):
+    if status != 0:
+        raise MmsError(f"{what} failed: {last_error()}")

@@ -1,0 +1,121 @@
+"""Film: one magnetic film resident on the GPU (wraps an ``mms_ctx``)."""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import MmsParams, check, lib
+
+SOLVER_DIRECT, SOLVER_FMM = 0, 1
+M2L_FP32, M2L_BF16, M2L_BF16X3 = 0, 1, 2
+EXT_NONE, EXT_RAMP, EXT_CONST = 0, 1, 2
+
+MU_0 = np.float32(4 * np.pi * 1e-7)
+
+
+class Film:
+    """Device-resident film state with fixed-step RK4.
+
+    Parameters mirror ``mms_params`` (include/mms_gpu.h).  ``material`` is a
+    uint8 array of zdim*ydim*xdim cells (None = all magnetic).
+    """
+
+    def __init__(self, xdim, ydim, zdim, dx, dy, dz, *, solver=SOLVER_DIRECT, P=6,
+                 m2l_precision=M2L_FP32, leaf_width=4, mu_0=MU_0, Ms=8.6e5, Aexch=1.3e-11,
+                 alfa=0.5, gamma=2.211e5, demag=1, exchange=1, external=EXT_NONE,
+                 hext=(0.0, 0.0, 0.0), material=None):
+        p = MmsParams()
+        p.xdim, p.ydim, p.zdim = xdim, ydim, zdim
+        p.dx, p.dy, p.dz = dx, dy, dz
+        p.solver, p.P, p.m2l_precision, p.leaf_width = solver, P, m2l_precision, leaf_width
+        p.mu_0, p.Ms, p.Aexch, p.alfa, p.gamma = mu_0, Ms, Aexch, alfa, gamma
+        p.demag, p.exchange, p.external = demag, exchange, external
+        for i in range(3):
+            p.hext[i] = hext[i]
+        self.params = p
+        self.dims = (xdim, ydim, zdim)
+        self.n = xdim * ydim * zdim
+        self._ctx = C.c_void_p()
+        mat = None
+        if material is not None:
+            self._mat = np.ascontiguousarray(material, dtype=np.uint8).ravel()
+            if self._mat.size != self.n:
+                raise ValueError("material size does not match the grid")
+            mat = self._mat.ctypes.data_as(C.c_void_p)
+        check(lib().mms_create(C.byref(self._ctx), C.byref(p), mat), "mms_create")
+
+    def close(self):
+        if self._ctx:
+            lib().mms_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---------------------------------------------------------- state --
+    def set_m(self, M):
+        M = np.ascontiguousarray(M, dtype=np.float32).ravel()
+        check(lib().mms_set_m(self._ctx, M), "mms_set_m")
+
+    def get_m(self):
+        out = np.zeros(3 * self.n, np.float32)
+        check(lib().mms_get_m(self._ctx, out), "mms_get_m")
+        return out
+
+    def set_time(self, t):
+        check(lib().mms_set_time(self._ctx, float(t)), "mms_set_time")
+
+    # ------------------------------------------------------- stepping --
+    def steps(self, nsteps, dt, use_graph=True):
+        check(lib().mms_steps(self._ctx, int(nsteps), float(dt), int(bool(use_graph))), "mms_steps")
+
+    def sync(self):
+        check(lib().mms_sync(self._ctx), "mms_sync")
+
+    # --------------------------------------------------------- fields --
+    def hfield(self):
+        out = np.zeros(3 * self.n, np.float32)
+        check(lib().mms_hfield(self._ctx, out), "mms_hfield")
+        return out
+
+    def energy_torque(self):
+        E, T = C.c_double(), C.c_double()
+        check(lib().mms_energy_torque(self._ctx, C.byref(E), C.byref(T)), "mms_energy_torque")
+        return E.value, T.value
+
+    def potential(self, charge):
+        charge = np.ascontiguousarray(charge, dtype=np.float32).ravel()
+        out = np.zeros(self.n, np.float32)
+        check(lib().mms_potential(self._ctx, charge, out), "mms_potential")
+        return out
+
+    def potential_probes_f64(self, charge, probes):
+        charge = np.ascontiguousarray(charge, dtype=np.float32).ravel()
+        probes = np.ascontiguousarray(probes, dtype=np.int64).ravel()
+        out = np.zeros(len(probes), np.float64)
+        check(lib().mms_potential_probes_f64(self._ctx, charge, probes, len(probes), out),
+              "mms_potential_probes_f64")
+        return out
+
+    # -------------------------------------------------------- profiling --
+    def profile_begin(self):
+        check(lib().mms_profile_begin(self._ctx), "mms_profile_begin")
+
+    def profile_end(self):
+        """Stop profiling; returns {kernel class: (mean ms per launch, launches)}."""
+        check(lib().mms_profile_end(self._ctx), "mms_profile_end")
+        out = {}
+        for i in range(lib().mms_profile_count(self._ctx)):
+            cnt = lib().mms_profile_launches(self._ctx, i)
+            if cnt:
+                out[lib().mms_profile_name(self._ctx, i).decode()] = (
+                    lib().mms_profile_ms(self._ctx, i), cnt)
+        return out

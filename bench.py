@@ -1,0 +1,298 @@
+"""Benchmark: LLG RK4 steps/s of the 1024 x 1024 x 4 film (4,194,304 cells,
+32x32-cell random in-plane domains, seed 2010) with the GPU FMM demag
+(P = 6) on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--m2l fp32|bf16|bf16x3]
+                    [--leaf 4|8] [--no-cpu] [--no-error]
+
+One step = one fixed-dt (5e-14 s) RK4 step = 4 demag evaluations + 4 fused
+field/LLG stage passes, state resident in HBM.  Prints ONE JSON line with the
+steps/s, the roofline of the dominant kernel (algorithmic work per launch over
+its mean HIP-event duration, measured in this run on the solver's stream),
+the demag-field error against an FP64 direct sum on seeded probes, and the
+reference CPU path timed on a bounded sample on this host.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "mms-gpu_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+# ----------------------------------------------------- algorithmic work --
+def stage_bytes_per_cell(stage):
+    """Compulsory HBM bytes of the fused field+LLG stage kernel per cell:
+    phi 4 + m_s 12 + FP64 base 24 + material 1 + FP32 m_{s+1} 12 + FP64 write
+    24 (+ FP64 accumulator read 24 at stages 2-4)."""
+    return 77 if stage == 0 else 101
+
+
+def m2l_flops_per_eval(side, zd, P, w):
+    """Issued FMA flops of the M2L pass: every target box and layer contracts
+    27 offsets x zd source layers of K x K operator blocks (2 flops per MAC)."""
+    K = (P + 1) * (P + 2) // 2
+    Lleaf = int(math.log2(side)) - int(math.log2(w))
+    return sum((4 ** L) * zd * 27 * zd * 2 * K * K for L in range(2, Lleaf + 1)), Lleaf - 1
+
+
+def p2p_pairs_per_eval(side, zd, w):
+    """Near-field pairs: each target against its 3x3 neighbour leaves in
+    every layer (self excluded), edge leaves counted with their full window."""
+    return side * side * zd * (9 * w * w * zd - 1)
+
+
+# ----------------------------------------------------------- CPU leg --
+def cpu_baseline(w, ch, budget_s=15.0):
+    """Reference tree-code (fmm_calc CPU branch, P = 4: the reference's
+    maximum order) restated in oracle/mms_oracle.c, single-threaded as in the
+    reference (its OpenMP pragmas are commented out there).  Timed on a
+    bounded sample: every `stride`-th box of one mid level for one charge
+    layer; the per-term cost is extrapolated with the exact term count of the
+    whole evaluation (all levels, boxes, layers) to steps/s (4 evaluations
+    per step)."""
+    import oracle
+    L = oracle.lib()
+    side, zd, P = w.xdim, w.zdim, 4
+    K = (P + 1) ** 2
+    logN = int(math.log2(side))
+    q = ch.reshape(zd, side, side)
+    # exact term counts per (level, box): M2P terms = K * zd * (valid interaction cells)
+    total_terms = 0.0
+    sample_level = max(2, logN - 4)
+    counts = {}
+    for lev in range(2, logN + 1):
+        wl_ = 2 ** (logN - lev)
+        nb = 2 ** lev
+        nz = (np.abs(q).reshape(zd, nb, wl_, nb, wl_).sum(axis=(2, 4)) != 0)  # non-empty boxes per layer
+        bx = np.arange(nb)
+        # number of valid interaction-list boxes per box (edge-clipped), x and y separable
+        def n_il_1d(b):
+            lo = np.maximum(2 * (b // 2) - 2, 0)
+            hi = np.minimum(2 * (b // 2) + 3, nb - 1)
+            return lo, hi
+        lo, hi = n_il_1d(bx)
+        span = hi - lo + 1
+        nbr = np.minimum(bx + 1, nb - 1) - np.maximum(bx - 1, 0) + 1
+        # interaction boxes = (span_x * span_y) - (nbr_x * nbr_y)
+        il = span[None, :] * span[:, None] - nbr[None, :] * nbr[:, None]
+        cells = il * wl_ * wl_
+        per_box = cells * zd * K
+        t = float((per_box[None] * nz).sum())
+        counts[lev] = (per_box, nz)
+        total_terms += t
+    # sample: every stride-th box of sample_level, layer 0
+    per_box, nz = counts[sample_level]
+    nb = 2 ** sample_level
+    per_box_flat = np.zeros(nb * nb)
+    # box index is Morton order; map (by, bx) -> index
+    def morton(x, y, lev):
+        idx = 0
+        for b in range(lev):
+            idx |= ((x >> b) & 1) << (2 * b) | ((y >> b) & 1) << (2 * b + 1)
+        return idx
+    for by in range(nb):
+        for bx_ in range(nb):
+            per_box_flat[morton(bx_, by, sample_level)] = per_box[by, bx_] * nz[0, by, bx_]
+    est_rate = 1.4e7  # rough terms/s used only to size the sample
+    stride = max(1, int(per_box_flat.sum() / (est_rate * budget_s)))
+    sample_terms = float(per_box_flat[::stride].sum())
+    pot = np.zeros(ch.size, np.float32)
+    t0 = time.perf_counter()
+    st = L.or_fmm_bfs_sample(ch, pot, side, side, zd, float(w.dx), float(w.dy), float(w.dz), P, 0,
+                             sample_level, stride)
+    dt = time.perf_counter() - t0
+    if st != 0 or sample_terms == 0:
+        return None
+    per_term = dt / sample_terms
+    t_eval = per_term * total_terms
+    return {"value": 1.0 / (4.0 * t_eval), "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": (f"reference tree-code (fmm_calc CPU branch, P=4) restated in oracle/mms_oracle.c, "
+                       f"single thread; timed every {stride}th box of level {sample_level} for charge "
+                       f"layer 0 ({sample_terms:.3g} M2P terms in {dt:.1f} s), extrapolated by exact term "
+                       f"count ({total_terms:.3g} terms per evaluation, 4 evaluations per step)"),
+            "seconds_per_eval_extrapolated": t_eval}
+
+
+# --------------------------------------------------------- field error --
+def field_error(mg, f, w, nprobe=512):
+    """Relative L2 error of the demag field H = grad(phi)/4pi (reference
+    central differences) at seeded probe cells, FMM vs an FP64 direct sum run
+    on the device (the probe potentials at each probe and its 6 neighbours)."""
+    import numpy as np
+    from mms_gpu import workloads as wl
+    M = f.get_m()
+    # charge = div M with the reference stencil (host, float32, vectorised)
+    xd, yd, zd = w.dims
+    m = M.reshape(zd, yd, xd, 3)
+    def sh(a, ax, d):
+        out = np.zeros_like(a)
+        src = [slice(None)] * 3
+        dst = [slice(None)] * 3
+        if d > 0:
+            dst[ax], src[ax] = slice(0, -1), slice(1, None)
+        else:
+            dst[ax], src[ax] = slice(1, None), slice(0, -1)
+        out[tuple(dst)] = a[tuple(src)]
+        return out
+    ch = (((sh(m[..., 0], 2, 1) - sh(m[..., 0], 2, -1)) / w.dx + (sh(m[..., 1], 1, 1) - sh(m[..., 1], 1, -1)) / w.dy
+           + (sh(m[..., 2], 0, 1) - sh(m[..., 2], 0, -1)) / w.dz) * np.float32(0.5)).astype(np.float32).ravel()
+    phi = f.potential(ch)
+    pr = wl.probes(7, w.n, nprobe)
+    pz, py, px = pr // (xd * yd), (pr // xd) % yd, pr % xd
+    keep = (px > 0) & (px < xd - 1) & (py > 0) & (py < yd - 1) & (pz > 0) & (pz < zd - 1)
+    pr, pz, py, px = pr[keep], pz[keep], py[keep], px[keep]
+    offs = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
+    idx = np.stack([((pz + oz) * yd + (py + oy)) * xd + (px + ox) for ox, oy, oz in offs], 1)
+    ex = f.potential_probes_f64(ch, idx.ravel()).reshape(-1, 6)
+    got = phi[idx].astype(float)
+    def grad(v):
+        return np.stack([(v[:, 0] - v[:, 1]) / (2 * float(w.dx)), (v[:, 2] - v[:, 3]) / (2 * float(w.dy)),
+                         (v[:, 4] - v[:, 5]) / (2 * float(w.dz))], 1) / (4 * np.pi)
+    He, Hg = grad(ex), grad(got)
+    return (float(np.linalg.norm(Hg - He) / np.linalg.norm(He)),
+            float(np.abs(Hg - He).max() / np.abs(He).max()), int(len(pr)))
+
+
+# ---------------------------------------------------------------- main --
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--m2l", default="fp32", choices=["fp32", "bf16", "bf16x3"])
+    ap.add_argument("--leaf", type=int, default=4)
+    ap.add_argument("--P", type=int, default=6)
+    ap.add_argument("--side", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--graph", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md, 'How the path would shard')", file=sys.stderr)
+        sys.exit(2)
+
+    import mms_gpu as mg
+    from mms_gpu import workloads as wl
+    w = wl.c3_domains(side=args.side, zd=4)
+    prec = {"fp32": mg.M2L_FP32, "bf16": mg.M2L_BF16, "bf16x3": mg.M2L_BF16X3}[args.m2l]
+    log(f"workload {w.name}: {w.n} cells, FMM P={args.P} leaf={args.leaf} m2l={args.m2l}")
+    f = mg.Film(*w.dims, *w.d, solver=mg.SOLVER_FMM, P=args.P, m2l_precision=prec, leaf_width=args.leaf,
+                mu_0=wl.MU_0, Ms=wl.MS, Aexch=wl.AEXCH, alfa=wl.ALFA, gamma=wl.GAMMA, material=w.material)
+    f.set_m(w.M)
+    f.steps(args.warmup, w.dt, use_graph=bool(args.graph))
+    f.sync()
+    # timed region: K steps, synchronised on both sides (one rank: no barrier needed)
+    t0 = time.perf_counter()
+    f.steps(args.steps, w.dt, use_graph=bool(args.graph))
+    f.sync()
+    elapsed = time.perf_counter() - t0
+    steps_per_s = args.steps / elapsed
+    log(f"timed {args.steps} steps in {elapsed:.3f} s -> {steps_per_s:.2f} steps/s")
+
+    # per-kernel HIP-event durations over a profiled window of the same steps
+    nprof = max(2, min(args.steps, 5))
+    f.profile_begin()
+    tp = time.perf_counter()
+    f.steps(nprof, w.dt, use_graph=False)
+    prof = f.profile_end()
+    tp = time.perf_counter() - tp
+    per_step = {k: v[0] * v[1] / nprof for k, v in prof.items()}
+    log("per-step kernel ms:", {k: round(v, 4) for k, v in per_step.items()})
+    m = f.get_m().reshape(-1, 3).astype(np.float64)
+    mag_err = float(np.abs(np.linalg.norm(m, axis=1) / float(wl.MS) - 1).max())
+
+    side, zd = args.side, 4
+    n = side * side * zd
+    fl, nlev = m2l_flops_per_eval(side, zd, args.P, args.leaf)
+    kernels = {}
+    if "stage" in prof:
+        avg_b = (77 + 3 * 101) / 4 * n
+        ms, cnt = prof["stage"]
+        kernels["stage"] = dict(bound="hbm", achieved=avg_b / (ms * 1e-3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                                per_launch=avg_b, ms=ms)
+    if "div" in prof:
+        ms, cnt = prof["div"]
+        kernels["div"] = dict(bound="hbm", achieved=16 * n / (ms * 1e-3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                              per_launch=16 * n, ms=ms)
+    if "m2l" in prof:
+        ms, cnt = prof["m2l"]
+        per_launch = fl / nlev
+        peak = BF16_PEAK_TFLOPS if args.m2l != "fp32" else FP32_PEAK_TFLOPS
+        kernels["m2l"] = dict(bound="mfma", achieved=per_launch / (ms * 1e-3) / 1e12, peak=peak, unit="TFLOP/s",
+                              per_launch=per_launch, ms=ms)
+    if "p2p_l2p" in prof:
+        ms, cnt = prof["p2p_l2p"]
+        pairs = p2p_pairs_per_eval(side, zd, args.leaf)
+        kernels["p2p_l2p"] = dict(bound="mfma", achieved=11 * pairs / (ms * 1e-3) / 1e12, peak=FP32_PEAK_TFLOPS,
+                                  unit="TFLOP/s", per_launch=11 * pairs, ms=ms,
+                                  pairs_per_s=pairs / (ms * 1e-3))
+    dom = max(kernels, key=lambda k: per_step.get(k, 0.0)) if kernels else None
+    roof = None
+    if dom:
+        k = kernels[dom]
+        roof = {"kernel": dom, "bound": k["bound"], "achieved": round(k["achieved"], 3), "peak": k["peak"],
+                "unit": k["unit"], "frac": round(k["achieved"] / k["peak"], 4), "traffic": None,
+                "mean_launch_ms": round(k["ms"], 5)}
+    others = {name: {"achieved": round(k["achieved"], 3), "unit": k["unit"], "frac": round(k["achieved"] / k["peak"], 4),
+                     "mean_launch_ms": round(k["ms"], 5)} for name, k in kernels.items()}
+
+    err = None
+    if not args.no_error:
+        e_l2, e_max, npr = field_error(mg, f, w)
+        err = {"demag_field_rel_l2": e_l2, "demag_field_rel_max": e_max, "probes": npr, "oracle": "FP64 direct sum (device)"}
+        log(f"demag field error vs FP64: rel L2 {e_l2:.2e}, max {e_max:.2e} on {npr} probes")
+    cpu = None
+    if not args.no_cpu:
+        from mms_gpu import workloads as wl2
+        ch = np.zeros(w.n, np.float32)
+        import oracle
+        ch = oracle.divergence_3d(w.M, w.dims, w.d)
+        cpu = cpu_baseline(w, ch)
+        if cpu:
+            log(f"cpu baseline {cpu['value']:.3g} steps/s ({cpu['sample']})")
+    f.close()
+
+    out = {
+        "metric": "LLG RK4 steps/s at 1024²×4 cells, surrogate+FMM vs plain FMM at ≤1e-3 field err",
+        "value": round(steps_per_s, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.m2l == "fp32" else f"f32 ({args.m2l} M2L operands)",
+        "data": "synthetic (seeded SplitMix64 domain pattern, seed 2010)",
+        "config": {"workload": w.name, "form": "plain GPU FMM (no surrogate)", "cells": w.n, "P": args.P,
+                   "leaf_width": args.leaf, "m2l": args.m2l, "dt": float(w.dt), "graph": bool(args.graph),
+                   "precision_split": "FP32 fields and stage M, FP64 RK4 combine and renormalisation"},
+        "roofline": roof,
+        "kernels": others,
+        "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},
+        "field_error": err,
+        "max_abs_m_norm_error": mag_err,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,176 @@
+"""GPU potential solvers (direct sum and FMM) against the oracle.
+
+Tolerances: direct sum within 1e-5 relative (L2 and max-norm) of the FP64
+direct sum; FMM within the order-P truncation bound of the reference's
+tree-code scheme, and within FP32 rounding (2e-6 relative) of the FP64 FMM
+oracle running the same algorithm at the same P.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+F = np.float32
+D5 = (F(5e-9), F(5e-9), F(5e-9))
+
+
+def _say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+def _rel(a, b):
+    return np.linalg.norm(np.asarray(a, float) - b) / np.linalg.norm(b)
+
+
+def test_direct_c1_against_fp64_and_reference_path(mg, orc):
+    from mms_gpu import workloads as wl
+    w = wl.c1_square()
+    ch = orc.divergence_3d(w.M, w.dims, w.d)
+    _say("direct C1", w.dims)
+    pot = np.zeros_like(ch)
+    assert mg.calc_potential_exact(ch, *w.dims, *w.d, pot, 1, 0) == 0, mg.last_error()
+    ex = orc.potential_exact_f64(ch, w.dims, w.d)
+    assert _rel(pot, ex) < 1e-5
+    assert np.abs(pot - ex).max() < 1e-5 * np.abs(ex).max()
+    refpath = orc.calc_potential_exact(ch, w.dims, w.d)   # the reference's FP32 CPU loop
+    assert _rel(pot, refpath) < 3e-6
+    pot2 = np.zeros_like(ch)
+    assert mg.calc_potential_exact_gpu(ch, *w.dims, *w.d, pot2) == 0
+    assert np.array_equal(pot, pot2)
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (2, 1, 1), (7, 5, 3), (33, 1, 2), (1, 40, 1), (17, 19, 4)])
+def test_direct_edge_shapes(mg, orc, dims):
+    _say("direct shape", dims)
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(n)
+    ch = (rng.normal(size=n) * 1e14).astype(F)
+    d = (F(5e-9), F(4e-9), F(3e-9))
+    pot = np.full(n, np.nan, F)
+    assert mg.calc_potential_exact(ch, *dims, *d, pot, 1, 0) == 0, mg.last_error()
+    ex = orc.potential_exact_f64(ch, dims, d)
+    if n == 1:
+        assert pot[0] == 0.0     # self term excluded
+    else:
+        assert np.abs(pot - ex).max() <= 1e-5 * np.abs(ex).max()
+    zero = np.zeros(n, F)
+    assert mg.calc_potential_exact(zero, *dims, *d, pot, 1, 0) == 0
+    assert np.all(pot == 0)
+
+
+def test_direct_two_charges_and_antisymmetry(mg):
+    dims = (6, 1, 1)
+    q = np.zeros(6, F)
+    q[0], q[5] = F(3e14), F(-1e14)
+    pot = np.zeros(6, F)
+    assert mg.calc_potential_exact(q, *dims, *D5, pot, 1, 0) == 0
+    dx = 5e-9
+    dV = dx ** 3
+    assert math.isclose(pot[2], dV * (3e14 / (2 * dx) - 1e14 / (3 * dx)), rel_tol=1e-6)
+    neg = np.zeros(6, F)
+    assert mg.calc_potential_exact(-q, *dims, *D5, neg, 1, 0) == 0
+    assert np.array_equal(neg, -pot)
+
+
+def _probe_bound(q, xd, zd, P, dx, probes):
+    """Order-P truncation bound of the reference tree-code at each probe:
+    sum over (level, source box, layer) whose interaction list holds the
+    probe's box of A/(r-a) (a/r)^(P+1), A = sum |q| dV of the box, a its
+    geometric radius (w-1)/sqrt(2) dx, r = |probe - box centre|."""
+    logN = int(round(math.log2(xd)))
+    q3 = np.abs(q.reshape(zd, xd, xd)).astype(float) * dx ** 3
+    pz, py, px = probes // (xd * xd), (probes // xd) % xd, probes % xd
+    bound = np.zeros(len(probes))
+    for L in range(2, logN + 1):
+        w = 2 ** (logN - L)
+        nb = 2 ** L
+        A = q3.reshape(zd, nb, w, nb, w).sum(axis=(2, 4))
+        a = (w - 1) / math.sqrt(2) * dx
+        tbx, tby = px // w, py // w
+        for dyb in range(-3, 4):
+            for dxb in range(-3, 4):
+                sbx, sby = tbx + dxb, tby + dyb
+                ok = (sbx >= 0) & (sbx < nb) & (sby >= 0) & (sby < nb)
+                ok &= (np.abs(sbx // 2 - tbx // 2) <= 1) & (np.abs(sby // 2 - tby // 2) <= 1)
+                ok &= (np.abs(dxb) > 1) | (np.abs(dyb) > 1)
+                if not ok.any():
+                    continue
+                cx, cy = w * sbx + (w - 1) / 2, w * sby + (w - 1) / 2
+                for zc in range(zd):
+                    r = np.sqrt(((px - cx) ** 2 + (py - cy) ** 2) * dx * dx + ((pz - zc) * dx) ** 2)
+                    Ab = np.where(ok, A[zc, np.clip(sby, 0, nb - 1), np.clip(sbx, 0, nb - 1)], 0.0)
+                    bound += np.where(ok, Ab / (r - a) * (a / r) ** (P + 1), 0.0)
+    return bound
+
+
+def test_fmm_c2_dots_against_fp64_probes(mg, orc):
+    """256x256x3 dots film at P=4: error vs FP64 on 4096 seeded probes, inside
+    the reference's order-4 truncation bound; the GPU FP64 probe kernel agrees
+    with the CPU oracle."""
+    from mms_gpu import workloads as wl
+    w = wl.c2_dots()
+    ch = orc.divergence_3d(w.M, w.dims, w.d)
+    _say("fmm C2 P=4", w.dims)
+    pot = np.zeros_like(ch)
+    assert mg.fmm_calc(ch, pot, *w.dims, *w.d, 4, 1, 0) == 0, mg.last_error()
+    probes = wl.probes(2010, w.n, 4096)
+    ex = orc.potential_probes_f64(ch, w.dims, w.d, probes)
+    err = np.abs(pot[probes] - ex)
+    _say("C2 probe rel L2", _rel(pot[probes], ex), "max", err.max() / np.abs(ex).max())
+    # the reference's own tree-code is NaN at P = 4 on 5 nm cells and has
+    # 4.0e-3 relative error at P = 3 on these probes; the FMM at P = 4 is at
+    # 1.5e-3 and stays within 2% of the reference's order-4 truncation bound
+    assert _rel(pot[probes], ex) < 2e-3
+    bound = _probe_bound(ch, 256, 3, 4, float(w.dx), probes)
+    slack = 2e-6 * np.abs(ex).max()
+    assert np.all(err <= bound + slack), float((err / (bound + slack)).max())
+    with mg.Film(*w.dims, *w.d) as f:
+        gpu64 = f.potential_probes_f64(ch, probes[:512])
+    assert np.abs(gpu64 - ex[:512]).max() <= 1e-10 * np.abs(ex).max()
+
+
+@pytest.mark.parametrize("P,wl_", [(4, 4), (6, 4), (6, 8), (8, 2)])
+def test_fmm_matches_fp64_fmm_oracle(mg, orc, P, wl_):
+    """Same algorithm, same P and leaf width in FP64: differences are FP32 rounding."""
+    from oracle import fmm_oracle as fo
+    rng = np.random.default_rng(P)
+    xd = yd = 64
+    zd = 3
+    q = (rng.normal(size=(zd, yd, xd)) * 1e14).astype(F)
+    _say("fmm vs oracle", P, wl_)
+    with mg.Film(xd, yd, zd, *D5, solver=mg.SOLVER_FMM, P=P, leaf_width=wl_) as f:
+        pot = f.potential(q.ravel())
+    ref = fo.FmmOracle(xd, yd, zd, P, w=wl_).potential_units(q.astype(float)).ravel() * float(D5[0]) ** 2
+    assert _rel(pot, ref) < 2e-6
+    ex = fo.direct_fft(q.astype(float)).ravel() * float(D5[0]) ** 2
+    _say("  vs exact", _rel(pot, ex))
+
+
+def test_fmm_ragged_anisotropic_and_convergence(mg, orc):
+    from oracle import fmm_oracle as fo
+    rng = np.random.default_rng(12)
+    xd, yd, zd = 100, 75, 2
+    d = (F(5e-9), F(5e-9), F(3e-9))
+    q = (rng.normal(size=(zd, yd, xd)) * 1e14).astype(F)
+    ex = orc.potential_exact_f64(q.ravel(), (xd, yd, zd), d)
+    errs = []
+    for P in (2, 4, 6, 8):
+        with mg.Film(xd, yd, zd, *d, solver=mg.SOLVER_FMM, P=P) as f:
+            errs.append(_rel(f.potential(q.ravel()), ex))
+    _say("ragged errs", errs)
+    assert errs[0] > errs[1] > errs[2] > errs[3]
+    assert errs[2] < 1e-4
+
+
+def test_fmm_linearity_and_sign(mg):
+    rng = np.random.default_rng(13)
+    xd, zd = 128, 4
+    a = (rng.normal(size=xd * xd * zd) * 1e14).astype(F)
+    b = (rng.normal(size=xd * xd * zd) * 1e14).astype(F)
+    with mg.Film(xd, xd, zd, *D5, solver=mg.SOLVER_FMM, P=6) as f:
+        pa, pb, pab, pn = f.potential(a), f.potential(b), f.potential(a + b), f.potential(-a)
+        p2 = f.potential(a * F(2))
+    assert np.array_equal(pn, -pa)
+    assert np.array_equal(p2, 2 * pa)
+    assert _rel(pab, pa.astype(float) + pb) < 5e-6   # FP32 rounding of a + b and of the sums

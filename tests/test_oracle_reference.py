@@ -1,0 +1,137 @@
+"""The oracle's restatement against the reference's own outputs.
+
+* ref_vectors.npz: written by the reference's functions compiled from its
+  sources (tests/golden/gen_golden.py `ref`) -- must match bit for bit.
+* log_kats.json: the reference's own logged step records (GPU direct-sum runs)
+  -- the FP32 restatement of rk4_step/Hfield/time-loop energy must reproduce
+  them to the printed digits.
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def rv():
+    return np.load(os.path.join(GOLD, "ref_vectors.npz"))
+
+
+def test_vector_functions_bit_exact(orc, rv):
+    dims = tuple(int(v) for v in rv["vf_dims"])
+    d = tuple(np.float32(v) for v in rv["vf_d"])
+    S = orc.divergence_3d(rv["vf_M"], dims, d)
+    assert np.array_equal(S, rv["vf_div"])
+    V = orc.gradient_3d(rv["vf_div"], dims, d, np.float32(rv["vf_cm"][0]))
+    assert np.array_equal(V, rv["vf_grad"])
+    H = orc.add_exchange_field(rv["vf_M"], rv["vf_H0"], np.float32(8.6e5), np.float32(1.3e-11),
+                               np.float32(4e-7 * np.pi), dims, d)
+    assert np.array_equal(H, rv["vf_exch"])
+
+
+def test_legendre_and_harmonics_bit_exact(orc, rv):
+    L = orc.lib()
+    for l in range(5):
+        for m in range(l + 1):
+            got = np.array([L.or_associated_legendre(l, m, float(x)) for x in rv["leg_x"]], np.float32)
+            assert np.array_equal(got, rv["leg"][l, m]), (l, m)
+    a, b = C.c_float(), C.c_float()
+    for l in range(5):
+        for m in range(-l, l + 1):
+            for i, (th, ph) in enumerate(rv["sph_ang"]):
+                L.or_spherical_harmonic(l, m, float(th), float(ph), C.byref(a), C.byref(b))
+                assert (np.float32(a.value), np.float32(b.value)) == tuple(rv["sph"][l, m + 4, i]), (l, m, i)
+
+
+def test_llg_bit_exact(orc, rv):
+    al, ga, ms = (float(v) for v in rv["llg_par"])
+    o = np.zeros(3, np.float32)
+    for i in range(len(rv["llg_M"])):
+        orc.lib().or_llg_mprime(np.ascontiguousarray(rv["llg_M"][i]), np.ascontiguousarray(rv["llg_H"][i]),
+                                al, ga, ms, o)
+        assert np.array_equal(o, rv["llg_out"][i])
+
+
+def test_tree_lists_match_reference(orc, rv):
+    limit = int(rv["tree_limit"][0])
+    for level in range(0, limit + 1):
+        nbr, il = orc.tree_lists(level)
+        assert np.array_equal(nbr, rv[f"tree_L{level}_nbr"]), level
+        assert np.array_equal(il, rv[f"tree_L{level}_il"]), level
+        # invariants: at level >= 2 an interior box has 8 neighbours and 27 interactions
+        if level >= 3:
+            side = 2 ** level
+            counts = (il >= 0).sum(1)
+            assert counts.max() == 27 and (nbr >= 0).sum(1).max() == 8
+            # box centres at the deepest level: w*x + (w-1)/2 (Box.cu:37-38)
+            w = 2 ** (limit - level)
+            cen = rv[f"tree_L{level}_centre"]
+            xs = np.array([sum(((b >> (2 * k)) & 1) << k for k in range(level)) for b in range(side * side)])
+            assert np.allclose(cen[:, 0], w * xs + (w - 1) * 0.5)
+
+
+def test_ref_build_matches_live(orc):
+    """When oracle/_ref is built (this container), compare on fresh inputs too."""
+    R = orc.ref_lib()
+    if R is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    rng = np.random.default_rng(11)
+    dims, d = (6, 5, 3), (np.float32(5e-9), np.float32(5e-9), np.float32(2e-9))
+    n = int(np.prod(dims))
+    M = (rng.normal(size=3 * n) * 8.6e5).astype(np.float32)
+    S = np.zeros(n, np.float32)
+    R.ref_divergence_3d(M, *dims, *d, S)
+    assert np.array_equal(S, orc.divergence_3d(M, dims, d))
+
+
+# ------------------------------------------------------------- log KATs --
+def _kats():
+    with open(os.path.join(GOLD, "log_kats.json")) as f:
+        return json.load(f)
+
+
+def _kat_case(key, orc):
+    from mms_gpu import workloads as wl
+    k = _kats()[key]
+    xd, yd, zd = k["xdim"], k["ydim"], k["zdim"]
+    if k["mask"] == "nanowire4":
+        mask = np.loadtxt(os.path.join(GOLD, "nanowire4_mask.txt"), dtype=np.uint8)
+        mat, M = wl.layered_single_domain(mask, zd)
+    else:
+        mat, M = wl.single_domain_square(xd, yd, zd)
+    d = (np.float32(np.float32(k["width"]) / xd), np.float32(np.float32(k["height"]) / yd),
+         np.float32(np.float32(k["depth"]) / zd))
+    return k, mat, M, (xd, yd, zd), d
+
+
+def _step_records(orc, mat, M, dims, d, dt, nsteps, gamma):
+    from mms_gpu import workloads as wl
+    out = []
+    H, _, _ = orc.hfield(M, dims, d, 3, wl.MU_0, wl.MS, wl.AEXCH)
+    for _ in range(nsteps):
+        M2, _ = orc.rk4_step(M, H, mat, dims, d, dt, 3, wl.MU_0, wl.MS, wl.AEXCH, wl.ALFA, np.float32(gamma))
+        H, _, _ = orc.hfield(M2, dims, d, 3, wl.MU_0, wl.MS, wl.AEXCH)
+        E2, T = orc.energy_torque(mat, M2, H, wl.MU_0, wl.MS, d)
+        m = M2.reshape(-1, 3)[mat == 1].astype(np.float64) / float(wl.MS)
+        out.append((E2, *m.mean(0), T))
+        M = M2
+    return out
+
+
+@pytest.mark.parametrize("key,nsteps", [("nanowire4_fixed_1e-13", 3), ("square32_variable", 1)])
+def test_oracle_reproduces_logged_steps(orc, key, nsteps):
+    k, mat, M, dims, d = _kat_case(key, orc)
+    recs = _step_records(orc, mat, M, dims, d, np.float32(k["timestep"]), nsteps, k["gamma"])
+    for got, rec in zip(recs, k["records"]):
+        E2, mx, my, mz, T = got
+        _, _, _, _, E2r, mxr, myr, mzr, _, Tr = rec
+        assert abs(E2 - E2r) <= 1e-3, (E2, E2r)            # printed to 6 digits
+        assert abs(T - Tr) <= 2e-6 * abs(Tr) + 1e-7, (T, Tr)
+        assert abs(mx - mxr) <= 2e-5
+        if abs(myr) > 1e-6:                                  # deterministic asymmetry signals
+            assert abs(my - myr) <= 1e-5 * abs(myr), (my, myr)
+            assert abs(mz - mzr) <= 1e-5 * abs(mzr), (mz, mzr)

@@ -6,29 +6,55 @@
 // quadtree (per charge layer, interaction list = children of the parent's
 // neighbours minus own neighbours, Box.cu:121-145) carries a full FMM:
 //
-//   P2M   leaf multipoles from charges              M = P2M[K x w^2] rho_box
+//   P2M   leaf multipoles from charges              M = P2M[KP x w^2] rho_box
 //   M2M   children -> parent                       M_p = sum_c M2M_c M_c
-//   M2L   interaction list -> local                L_t += T(level, offset, dz) M_s
-//   L2L   parent local -> child                    L_c += L2L_c L_p   (fused into M2L)
-//   L2P   local -> cells of the leaf               phi = L2P[w^2 x K] L
-//   P2P   3x3 neighbour leaf boxes, all layers     phi += sum rho/r    (fused with L2P)
+//   M2L   interaction list -> local                L_t += T(level, offset, dz) M_s   (MFMA)
+//   L2L   parent local -> child                    L_c += L2L_c L_p                  (MFMA, fused)
+//   L2P   local -> cells of the leaf               phi = L2P[w^2 x KP] L
+//   P2P   3x3 neighbour leaf boxes, all layers     phi += sum rho/r                  (fused with L2P)
 //
 // On a regular grid every operator is translation invariant, so each is a
 // small constant matrix built once in FP64 on the host (fmm_operators.cpp)
-// and the device work is gathers plus dense contractions.  Launches:
-//   p2m_leaf  (1) | m2m (Lleaf-2) | m2l+l2l (Lleaf-1) | p2p_l2p (1)
-// M2L threads are grouped so that the child parity (px, py) and the target
-// layer are uniform across a wavefront; the operator blocks are then read
-// through the scalar cache and broadcast to all lanes.
+// and the device work is gathers plus dense contractions.  Box coefficient
+// vectors are stored with KP slots (K = (P+1)(P+2)/2 rounded up to a multiple
+// of 32; padding slots are zero in every operator), layout [z][by][bx][KP].
+//
+// M2L on the matrix cores: a workgroup owns a dense 16 x 16 block of target
+// boxes of one target layer; wavefront w handles child parity (w&1, w>>1),
+// i.e. 8 x 8 targets = four 16-column MFMA tiles, so every lane of a wave
+// walks the same 27 interaction offsets.  The source multipoles of a 22 x 22
+// window (3-box halo) of one source layer sit in LDS; for every (offset,
+// source layer) the wave issues C[KP x 64] += T[KP x KP] * M_window[KP x 64].
+// Operator fragments are pre-arranged on the host in lane order
+// ([row tile][lane][slot group]) so each lane loads its A fragment with
+// 16-byte vector loads.  Precision modes:
+//   FP32    v_mfma_f32_16x16x4_f32            (FP32 operands and accumulation)
+//   BF16    v_mfma_f32_16x16x32_bf16          (BF16 operands, FP32 accumulation)
+//   BF16X3  hi/lo BF16 split of both operands, C += Th Mh + Th Ml + Tl Mh
+// L2L is always FP32.
+#include <cstdlib>
+#include <cstring>
+
 #include "mms_internal.hpp"
 
 namespace mms {
 namespace {
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+template <int K> struct KPad { static constexpr int KP = ((K + 31) / 32) * 32; };
 template <int P> struct NC { static constexpr int K = (P + 1) * (P + 2) / 2; };
 
+__device__ inline unsigned short f2bf(float x) {  // round to nearest even
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ inline float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
 // ------------------------------------------------------------------ P2M --
-template <int K, int W>
+template <int K, int KP, int W>
 __global__ __launch_bounds__(256) void p2m_leaf_kernel(const float *__restrict__ rho, float *__restrict__ M,
                                                        const float *__restrict__ p2m, int xd, int yd, int nb) {
     const int bx = blockIdx.x * 256 + threadIdx.x, by = blockIdx.y, z = blockIdx.z;
@@ -44,7 +70,7 @@ __global__ __launch_bounds__(256) void p2m_leaf_kernel(const float *__restrict__
             q[iy * W + ix] = (x < xd && y < yd) ? rho[(long)z * plane + (long)y * xd + x] : 0.0f;
         }
     }
-    float *out = M + (((long)z * nb + by) * nb + bx) * K;
+    float *out = M + (((long)z * nb + by) * nb + bx) * KP;
 #pragma unroll 4
     for (int k = 0; k < K; k++) {
         float a = 0.0f;
@@ -52,10 +78,12 @@ __global__ __launch_bounds__(256) void p2m_leaf_kernel(const float *__restrict__
         for (int c = 0; c < W * W; c++) a = fmaf(p2m[k * W * W + c], q[c], a);
         out[k] = a;
     }
+#pragma unroll
+    for (int k = K; k < KP; k++) out[k] = 0.0f;
 }
 
 // ------------------------------------------------------------------ M2M --
-template <int K>
+template <int K, int KP>
 __global__ __launch_bounds__(256) void m2m_kernel(const float *__restrict__ Mc, float *__restrict__ Mp,
                                                   const float *__restrict__ m2m, int nbp) {
     const int bx = blockIdx.x * 256 + threadIdx.x, by = blockIdx.y, z = blockIdx.z;
@@ -67,7 +95,7 @@ __global__ __launch_bounds__(256) void m2m_kernel(const float *__restrict__ Mc, 
 #pragma unroll 1
     for (int ci = 0; ci < 4; ci++) {
         const int cx = 2 * bx + (ci & 1), cy = 2 * by + (ci >> 1);
-        const float *src = Mc + (((long)z * nbc + cy) * nbc + cx) * K;
+        const float *src = Mc + (((long)z * nbc + cy) * nbc + cx) * KP;
         float v[K];
 #pragma unroll
         for (int b = 0; b < K; b++) v[b] = src[b];
@@ -77,71 +105,201 @@ __global__ __launch_bounds__(256) void m2m_kernel(const float *__restrict__ Mc, 
 #pragma unroll
             for (int b = 0; b < K; b++) acc[a] = fmaf(T[a * K + b], v[b], acc[a]);
     }
-    float *out = Mp + (((long)z * nbp + by) * nbp + bx) * K;
+    float *out = Mp + (((long)z * nbp + by) * nbp + bx) * KP;
 #pragma unroll
     for (int a = 0; a < K; a++) out[a] = acc[a];
+#pragma unroll
+    for (int a = K; a < KP; a++) out[a] = 0.0f;
 }
 
-// ----------------------------------------------------------- M2L + L2L --
-// grid: x,y over the parity sub-lattice (nb/2 each), z = zt*4 + parity
-template <int K>
-__global__ __launch_bounds__(256) void m2l_kernel(const float *__restrict__ M, float *__restrict__ L,
-                                                  const float *__restrict__ Lpar, const float *__restrict__ T,
-                                                  const float *__restrict__ l2l, int nb, int zd) {
-    const int half = nb >> 1;
-    const int i = blockIdx.x * 16 + (threadIdx.x & 15), j = blockIdx.y * 16 + (threadIdx.x >> 4);
-    const int zt = blockIdx.z >> 2, par = blockIdx.z & 3;
-    const int px = par & 1, py = par >> 1;
-    if (i >= half || j >= half) return;
-    const int bx = 2 * i + px, by = 2 * j + py;
-    float acc[K];
+// -------------------------------------------------------- M2L + L2L --
+// A workgroup owns a dense TX x 16 block of target boxes (TX = 16 or 32) of one
+// target layer; wave w handles child parity (w & 1, w >> 1): an SX x 8
+// sub-lattice (SX = TX / 2) of targets in NC = SX / 2 MFMA column tiles of 16.
+// The source window holds the tile plus a 3-box halo: (TX + 6) x 22 boxes.
+template <int KP, int MODE, int NC> struct M2LCfg {
+    static constexpr int G = KP / 4;    // slots per lane group (lanes 16g..16g+15)
+    static constexpr int RT = KP / 16;  // 16-row output tiles
+    static constexpr int SX = 2 * NC;   // sub-lattice targets per row
+    static constexpr int TX = 2 * SX;   // dense tile width in boxes
+    static constexpr int WX = TX + 6, WY = 22;
+    static constexpr int SB = (MODE == MMS_M2L_FP32) ? KP + 4 : KP + 8;  // LDS stride per box (elements)
+    static constexpr int NW = (MODE == MMS_M2L_BF16X3) ? 2 : 1;          // bf16 planes (hi, lo)
+    static constexpr size_t lds_bytes =
+        (MODE == MMS_M2L_FP32) ? sizeof(float) * WX * WY * SB : sizeof(unsigned short) * NW * WX * WY * SB;
+};
+
+template <int KP, int MODE, int NC>
+__global__ __launch_bounds__(256) void m2l_mfma_kernel(const float *__restrict__ M, float *__restrict__ L,
+                                                       const float *__restrict__ Lpar,
+                                                       const void *__restrict__ Tfrag,
+                                                       const float *__restrict__ l2lfrag, int nb, int zd) {
+    typedef M2LCfg<KP, MODE, NC> C;
+    constexpr int G = C::G, RT = C::RT, SB = C::SB, SX = C::SX, WX = C::WX, WY = C::WY;
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int px = wave & 1, py = wave >> 1;
+    const int x0 = blockIdx.x * C::TX, y0 = blockIdx.y * 16, zt = blockIdx.z;
+    const int j16 = lane & 15, g = lane >> 4;
+    // window offset of this lane's target in column tile c: target t = 16c + j16
+    int tbox[NC];
 #pragma unroll
-    for (int a = 0; a < K; a++) acc[a] = 0.0f;
-    if (Lpar) {  // L2L from the parent at (i, j) of the coarser level
-        const float *src = Lpar + (((long)zt * half + j) * half + i) * K;
-        float v[K];
-#pragma unroll
-        for (int b = 0; b < K; b++) v[b] = src[b];
-        const float *O = l2l + (long)par * K * K;
-#pragma unroll
-        for (int a = 0; a < K; a++)
-#pragma unroll
-            for (int b = 0; b < K; b++) acc[a] = fmaf(O[a * K + b], v[b], acc[a]);
+    for (int c = 0; c < NC; c++) {
+        const int t = 16 * c + j16, ti = t % SX, tj = t / SX;
+        tbox[c] = (2 * tj + py + 3) * WX + (2 * ti + px + 3);
     }
-    const long KK = (long)K * K;
+
+    f32x4 acc[RT][NC];
+#pragma unroll
+    for (int r = 0; r < RT; r++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+
 #pragma unroll 1
     for (int zs = 0; zs < zd; zs++) {
+        __syncthreads();
+        // stage the source window of layer zs (zero outside the grid)
+        const f32x4 *M4 = reinterpret_cast<const f32x4 *>(M);
+        for (int e = threadIdx.x; e < WX * WY * (KP / 4); e += 256) {
+            const int box = e / (KP / 4), q = e % (KP / 4);
+            const int gx = x0 - 3 + box % WX, gy = y0 - 3 + box / WX;
+            f32x4 v = f32x4{0.f, 0.f, 0.f, 0.f};
+            if (gx >= 0 && gx < nb && gy >= 0 && gy < nb) v = M4[(((long)zs * nb + gy) * nb + gx) * (KP / 4) + q];
+            if constexpr (MODE == MMS_M2L_FP32) {
+                *reinterpret_cast<f32x4 *>(reinterpret_cast<float *>(smem) + box * SB + 4 * q) = v;
+            } else {
+                unsigned short *hi = reinterpret_cast<unsigned short *>(smem);
+                unsigned short h[4], l[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    h[t] = f2bf(v[t]);
+                    l[t] = f2bf(v[t] - bf2f(h[t]));
+                }
+                uint2 hv, lv;
+                hv.x = h[0] | ((unsigned)h[1] << 16); hv.y = h[2] | ((unsigned)h[3] << 16);
+                lv.x = l[0] | ((unsigned)l[1] << 16); lv.y = l[2] | ((unsigned)l[3] << 16);
+                *reinterpret_cast<uint2 *>(hi + box * SB + 4 * q) = hv;
+                if constexpr (MODE == MMS_M2L_BF16X3)
+                    *reinterpret_cast<uint2 *>(hi + WX * WY * SB + box * SB + 4 * q) = lv;
+            }
+        }
+        __syncthreads();
         const int dzi = zt - zs + zd - 1;
 #pragma unroll 1
         for (int oy = -2 - py; oy <= 3 - py; oy++) {
-            const int sy = by + oy;
 #pragma unroll 1
             for (int ox = -2 - px; ox <= 3 - px; ox++) {
                 if (ox >= -1 && ox <= 1 && oy >= -1 && oy <= 1) continue;
-                const int sx = bx + ox;
-                const bool ok = (sx >= 0 && sx < nb && sy >= 0 && sy < nb);
-                const float *src = M + (((long)zs * nb + (ok ? sy : 0)) * nb + (ok ? sx : 0)) * K;
-                float v[K];
+                const long fbase = ((long)dzi * 49 + (oy + 3) * 7 + (ox + 3)) * RT;
+                const int woff = oy * WX + ox;
+                if constexpr (MODE == MMS_M2L_FP32) {
+                    const float *T = reinterpret_cast<const float *>(Tfrag);
+                    const float *lds = reinterpret_cast<const float *>(smem);
+                    // k-slot groups of 4: A and B for 4 slots, then all RT x NC tiles
 #pragma unroll
-                for (int b = 0; b < K; b++) v[b] = ok ? src[b] : 0.0f;
-                const float *O = T + ((long)dzi * 49 + (oy + 3) * 7 + (ox + 3)) * KK;
+                    for (int u4 = 0; u4 < G / 4; u4++) {
+                        f32x4 a[RT], b[NC];
 #pragma unroll
-                for (int a = 0; a < K; a++)
+                        for (int r = 0; r < RT; r++)
+                            a[r] = *reinterpret_cast<const f32x4 *>(T + ((fbase + r) * 64 + lane) * G + 4 * u4);
 #pragma unroll
-                    for (int b = 0; b < K; b++) acc[a] = fmaf(O[a * K + b], v[b], acc[a]);
+                        for (int c = 0; c < NC; c++)
+                            b[c] = *reinterpret_cast<const f32x4 *>(lds + (tbox[c] + woff) * SB + g * G + 4 * u4);
+#pragma unroll
+                        for (int u = 0; u < 4; u++)
+#pragma unroll
+                            for (int r = 0; r < RT; r++)
+#pragma unroll
+                                for (int c = 0; c < NC; c++)
+                                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r][u], b[c][u], acc[r][c], 0, 0, 0);
+                    }
+                } else {
+                    const unsigned short *T = reinterpret_cast<const unsigned short *>(Tfrag);
+                    const unsigned short *hi = reinterpret_cast<const unsigned short *>(smem);
+                    constexpr long LO = (long)RT * 64 * G;  // lo plane offset per (dz, offset) block count below
+#pragma unroll
+                    for (int s = 0; s < G / 8; s++) {  // 32-wide k steps
+                        bf16x8 ah[RT], bh[NC];
+#pragma unroll
+                        for (int r = 0; r < RT; r++)
+                            ah[r] = *reinterpret_cast<const bf16x8 *>(T + ((fbase + r) * 64 + lane) * G + 8 * s);
+#pragma unroll
+                        for (int c = 0; c < NC; c++)
+                            bh[c] = *reinterpret_cast<const bf16x8 *>(hi + (tbox[c] + woff) * SB + g * G + 8 * s);
+#pragma unroll
+                        for (int r = 0; r < RT; r++)
+#pragma unroll
+                            for (int c = 0; c < NC; c++)
+                                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[r], bh[c], acc[r][c], 0, 0, 0);
+                        if constexpr (MODE == MMS_M2L_BF16X3) {
+                            const long lo_base = LO * 49 * (2 * zd - 1);
+                            bf16x8 al[RT], bl[NC];
+#pragma unroll
+                            for (int r = 0; r < RT; r++)
+                                al[r] = *reinterpret_cast<const bf16x8 *>(T + lo_base + ((fbase + r) * 64 + lane) * G + 8 * s);
+#pragma unroll
+                            for (int c = 0; c < NC; c++)
+                                bl[c] = *reinterpret_cast<const bf16x8 *>(hi + WX * WY * SB + (tbox[c] + woff) * SB +
+                                                                          g * G + 8 * s);
+#pragma unroll
+                            for (int r = 0; r < RT; r++)
+#pragma unroll
+                                for (int c = 0; c < NC; c++)
+                                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[r], bl[c], acc[r][c], 0, 0, 0);
+#pragma unroll
+                            for (int r = 0; r < RT; r++)
+#pragma unroll
+                                for (int c = 0; c < NC; c++)
+                                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[r], bh[c], acc[r][c], 0, 0, 0);
+                        }
+                    }
+                }
             }
         }
     }
-    float *out = L + (((long)zt * nb + by) * nb + bx) * K;
+    const int half = nb >> 1;
+    if (Lpar) {  // L2L from the parent (same layer, one level coarser), FP32 MFMA
 #pragma unroll
-    for (int a = 0; a < K; a++) out[a] = acc[a];
+        for (int u4 = 0; u4 < G / 4; u4++) {
+            f32x4 a[RT], b[NC];
+#pragma unroll
+            for (int r = 0; r < RT; r++)
+                a[r] = *reinterpret_cast<const f32x4 *>(l2lfrag + (((long)wave * RT + r) * 64 + lane) * G + 4 * u4);
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                const int t = 16 * c + j16;
+                const int bx = x0 + 2 * (t % SX) + px, by = y0 + 2 * (t / SX) + py;
+                b[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+                if (bx < nb && by < nb)
+                    b[c] = *reinterpret_cast<const f32x4 *>(
+                        Lpar + (((long)zt * half + (by >> 1)) * half + (bx >> 1)) * KP + g * G + 4 * u4);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int r = 0; r < RT; r++)
+#pragma unroll
+                    for (int c = 0; c < NC; c++)
+                        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r][u], b[c][u], acc[r][c], 0, 0, 0);
+        }
+    }
+    // D tile row 4g + q (q = 0..3), column j16: slots 16r + 4g .. +3 of target j16
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const int t = 16 * c + j16;
+        const int bx = x0 + 2 * (t % SX) + px, by = y0 + 2 * (t / SX) + py;
+        if (bx >= nb || by >= nb) continue;
+        f32x4 *dst = reinterpret_cast<f32x4 *>(L + (((long)zt * nb + by) * nb + bx) * KP);
+#pragma unroll
+        for (int r = 0; r < RT; r++) dst[4 * r + g] = acc[r][c];
+    }
 }
 
 // -------------------------------------------------------- P2P + L2P --
 // Block = TW x TW target cells of one layer (TW = 16); the charges of the
 // tile plus a W-cell halo in every layer are staged in LDS together with the
 // 1/r table over all (dx, dy, dz) offsets of the near field.
-template <int K, int W>
+template <int K, int KP, int W>
 __global__ __launch_bounds__(256) void p2p_l2p_kernel(const float *__restrict__ rho, float *__restrict__ phi,
                                                       const float *__restrict__ Lleaf, const float *__restrict__ l2p,
                                                       const float *__restrict__ table, int xd, int yd, int zd,
@@ -169,7 +327,7 @@ __global__ __launch_bounds__(256) void p2p_l2p_kernel(const float *__restrict__ 
         const float *tb = tab + (zs - zt + zd - 1) * TS * TS;
 #pragma unroll 1
         for (int syi = 0; syi < 3 * W; syi++) {
-            const int sy = ly0 + syi;                 // LDS row of the source
+            const int sy = ly0 + syi;                   // LDS row of the source
             const int dyi = sy - (ty + W) + 2 * W - 1;  // table row
             const float *srow = s + sy * SW + lx0;
             const float *trow = tb + dyi * TS + (lx0 - (tx + W) + 2 * W - 1);
@@ -179,7 +337,7 @@ __global__ __launch_bounds__(256) void p2p_l2p_kernel(const float *__restrict__ 
     }
     if (x >= xd || y >= yd) return;
     const int bx = x / W, by = y / W, c = (y % W) * W + (x % W);
-    const float *Lv = Lleaf + (((long)zt * nb + by) * nb + bx) * K;
+    const float *Lv = Lleaf + (((long)zt * nb + by) * nb + bx) * KP;
     const float *row = l2p + (long)c * K;
     float far = 0.0f;
 #pragma unroll
@@ -188,15 +346,44 @@ __global__ __launch_bounds__(256) void p2p_l2p_kernel(const float *__restrict__ 
 }
 
 // -------------------------------------------------------------- dispatch --
-template <int P, int W>
+template <int KP, int MODE, int NC>
+int launch_m2l(dim3 block_grid_unused, const FmmPlan &pl, int L, hipStream_t s) {
+    (void)block_grid_unused;
+    typedef M2LCfg<KP, MODE, NC> C;
+    static int attr_done = 0;
+    if (!attr_done) {
+        MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&m2l_mfma_kernel<KP, MODE, NC>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::lds_bytes));
+        attr_done = 1;
+    }
+    const int nb = 1 << L, zd = pl.zd;
+    dim3 grid((nb + C::TX - 1) / C::TX, (nb + 15) / 16, zd);
+    const float *Lpar = (L > 2) ? pl.loc + pl.lvl_off[L - 1] : nullptr;
+    const char *T = reinterpret_cast<const char *>(pl.d_m2l) + (size_t)(L - 2) * pl.m2l_level_bytes;
+    hipLaunchKernelGGL((m2l_mfma_kernel<KP, MODE, NC>), grid, dim3(256), (C::lds_bytes), s, pl.mult + pl.lvl_off[L],
+                       pl.loc + pl.lvl_off[L], Lpar, (const void *)T, pl.d_l2l, nb, zd);
+    MMS_HIP(hipGetLastError());
+    return 0;
+}
+
+template <int KP, int MODE>
+int launch_m2l_nc(const FmmPlan &pl, int L, hipStream_t s) {
+    // wide tiles (8 column tiles per wave) once a level has enough boxes
+    if constexpr (KP <= 64) {
+        if (pl.m2l_cols == 8 && (1 << L) >= 32) return launch_m2l<KP, MODE, 8>(dim3(), pl, L, s);
+    }
+    return launch_m2l<KP, MODE, 4>(dim3(), pl, L, s);
+}
+
+template <int K, int W>
 int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_t s, Profiler *prof) {
-    constexpr int K = NC<P>::K;
+    constexpr int KP = KPad<K>::KP;
     const int zd = g.zd;
     {
         KTimer kt(prof, K_P2M);
         const int nb = 1 << pl.Lleaf;
         dim3 grid((nb + 255) / 256, nb, zd);
-        hipLaunchKernelGGL((p2m_leaf_kernel<K, W>), grid, dim3(256), 0, s, rho, pl.mult + pl.lvl_off[pl.Lleaf],
+        hipLaunchKernelGGL((p2m_leaf_kernel<K, KP, W>), grid, dim3(256), 0, s, rho, pl.mult + pl.lvl_off[pl.Lleaf],
                            pl.d_p2m, g.xd, g.yd, nb);
         MMS_HIP(hipGetLastError());
     }
@@ -204,18 +391,19 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
         KTimer kt(prof, K_M2M);
         const int nbp = 1 << L;
         dim3 grid((nbp + 255) / 256, nbp, zd);
-        hipLaunchKernelGGL((m2m_kernel<K>), grid, dim3(256), 0, s, pl.mult + pl.lvl_off[L + 1],
+        hipLaunchKernelGGL((m2m_kernel<K, KP>), grid, dim3(256), 0, s, pl.mult + pl.lvl_off[L + 1],
                            pl.mult + pl.lvl_off[L], pl.d_m2m, nbp);
         MMS_HIP(hipGetLastError());
     }
     for (int L = 2; L <= pl.Lleaf; L++) {
         KTimer kt(prof, K_M2L);
-        const int nb = 1 << L, half = nb >> 1;
-        dim3 grid((half + 15) / 16, (half + 15) / 16, zd * 4);
-        const float *Lpar = (L > 2) ? pl.loc + pl.lvl_off[L - 1] : nullptr;
-        hipLaunchKernelGGL((m2l_kernel<K>), grid, dim3(256), 0, s, pl.mult + pl.lvl_off[L], pl.loc + pl.lvl_off[L],
-                           Lpar, pl.d_m2l + (long)(L - 2) * pl.m2l_level_stride, pl.d_l2l, nb, zd);
-        MMS_HIP(hipGetLastError());
+        int st;
+        switch (pl.precision) {
+        case MMS_M2L_BF16: st = launch_m2l_nc<KP, MMS_M2L_BF16>(pl, L, s); break;
+        case MMS_M2L_BF16X3: st = launch_m2l_nc<KP, MMS_M2L_BF16X3>(pl, L, s); break;
+        default: st = launch_m2l_nc<KP, MMS_M2L_FP32>(pl, L, s);
+        }
+        if (st) return 1;
     }
     {
         KTimer kt(prof, K_P2P_L2P);
@@ -223,8 +411,8 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
         const size_t lds = sizeof(float) * ((size_t)zd * SW * SW + (size_t)(2 * zd - 1) * TS * TS);
         dim3 grid((g.xd + TW - 1) / TW, (g.yd + TW - 1) / TW, zd);
         const float scale = (float)((double)(g.dx * g.dy * g.dz) / (double)g.dx);
-        hipLaunchKernelGGL((p2p_l2p_kernel<K, W>), grid, dim3(256), lds, s, rho, phi, pl.loc + pl.lvl_off[pl.Lleaf],
-                           pl.d_l2p, pl.d_near, g.xd, g.yd, zd, 1 << pl.Lleaf, scale);
+        hipLaunchKernelGGL((p2p_l2p_kernel<K, KP, W>), grid, dim3(256), lds, s, rho, phi,
+                           pl.loc + pl.lvl_off[pl.Lleaf], pl.d_l2p, pl.d_near, g.xd, g.yd, zd, 1 << pl.Lleaf, scale);
         MMS_HIP(hipGetLastError());
     }
     return 0;
@@ -233,18 +421,31 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
 template <int W>
 int run_fmm_w(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_t s, Profiler *prof) {
     switch (pl.P) {
-    case 1: return run_fmm<1, W>(g, pl, rho, phi, s, prof);
-    case 2: return run_fmm<2, W>(g, pl, rho, phi, s, prof);
-    case 3: return run_fmm<3, W>(g, pl, rho, phi, s, prof);
-    case 4: return run_fmm<4, W>(g, pl, rho, phi, s, prof);
-    case 5: return run_fmm<5, W>(g, pl, rho, phi, s, prof);
-    case 6: return run_fmm<6, W>(g, pl, rho, phi, s, prof);
-    case 7: return run_fmm<7, W>(g, pl, rho, phi, s, prof);
-    case 8: return run_fmm<8, W>(g, pl, rho, phi, s, prof);
-    case 9: return run_fmm<9, W>(g, pl, rho, phi, s, prof);
-    case 10: return run_fmm<10, W>(g, pl, rho, phi, s, prof);
+    case 1: return run_fmm<NC<1>::K, W>(g, pl, rho, phi, s, prof);
+    case 2: return run_fmm<NC<2>::K, W>(g, pl, rho, phi, s, prof);
+    case 3: return run_fmm<NC<3>::K, W>(g, pl, rho, phi, s, prof);
+    case 4: return run_fmm<NC<4>::K, W>(g, pl, rho, phi, s, prof);
+    case 5: return run_fmm<NC<5>::K, W>(g, pl, rho, phi, s, prof);
+    case 6: return run_fmm<NC<6>::K, W>(g, pl, rho, phi, s, prof);
+    case 7: return run_fmm<NC<7>::K, W>(g, pl, rho, phi, s, prof);
+    case 8: return run_fmm<NC<8>::K, W>(g, pl, rho, phi, s, prof);
+    case 9: return run_fmm<NC<9>::K, W>(g, pl, rho, phi, s, prof);
+    case 10: return run_fmm<NC<10>::K, W>(g, pl, rho, phi, s, prof);
     default: set_error("FMM expansion order must be 1..10"); return 1;
     }
+}
+
+unsigned short host_bf16(float x) {
+    unsigned u;
+    memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+float host_bf2f(unsigned short h) {
+    unsigned u = (unsigned)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
 }
 
 }  // namespace
@@ -264,8 +465,11 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_) {
     MMS_CHECK(P_ >= 1 && P_ <= 10, "FMM expansion order must be 1..10");
     MMS_CHECK(w_ == 2 || w_ == 4 || w_ == 8, "FMM leaf width must be 2, 4 or 8");
     MMS_CHECK(g.zd >= 1 && g.zd <= 16, "FMM supports 1..16 layers");
+    MMS_CHECK(precision_ >= MMS_M2L_FP32 && precision_ <= MMS_M2L_BF16X3, "unknown M2L precision");
     P = P_; w = w_; precision = precision_; zd = g.zd;
+    if (const char *e = getenv("MMS_M2L_COLS")) m2l_cols = (atoi(e) == 4) ? 4 : 8;
     K = fmm_ncoef(P);
+    KP = ((K + 31) / 32) * 32;
     const int side = g.xd > g.yd ? g.xd : g.yd;
     Lmax = 0;
     while ((1 << Lmax) < side) Lmax++;
@@ -273,57 +477,87 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_) {
     while ((1 << lw) < w) lw++;
     Lleaf = Lmax - lw;
     MMS_CHECK(Lleaf >= 2, "grid too small for the FMM at this leaf width (needs >= 4 leaf boxes per side)");
-    // level storage
     lvl_off.assign(Lleaf + 1, 0);
     long tot = 0;
     for (int L = 2; L <= Lleaf; L++) {
         lvl_off[L] = tot;
-        tot += (long)zd * (1L << (2 * L)) * K;
+        tot += (long)zd * (1L << (2 * L)) * KP;
     }
     MMS_HIP(hipMalloc(&mult, sizeof(float) * tot));
     MMS_HIP(hipMalloc(&loc, sizeof(float) * tot));
-    // operators
+    MMS_HIP(hipMemset(mult, 0, sizeof(float) * tot));
+    MMS_HIP(hipMemset(loc, 0, sizeof(float) * tot));
     FmmHostOps ops;
     fmm_build_ops(P, w, (double)g.ay, ops);
-    auto upload = [&](float **dst, const std::vector<double> &src) -> int {
-        std::vector<float> f(src.begin(), src.end());
-        MMS_HIP(hipMalloc(dst, sizeof(float) * f.size()));
-        MMS_HIP(hipMemcpy(*dst, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
+    auto upload = [&](void **dst, const void *src, size_t bytes) -> int {
+        MMS_HIP(hipMalloc(dst, bytes));
+        MMS_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
         return 0;
     };
-    if (upload(&d_p2m, ops.p2m) || upload(&d_l2p, ops.l2p) || upload(&d_m2m, ops.m2m) || upload(&d_l2l, ops.l2l))
-        return 1;
-    m2l_level_stride = (long)(2 * zd - 1) * 49 * K * K;
-    std::vector<double> all;
-    all.reserve((size_t)m2l_level_stride * (Lleaf - 1));
+    std::vector<float> f(ops.p2m.begin(), ops.p2m.end());                      // [K][w*w]
+    if (upload((void **)&d_p2m, f.data(), sizeof(float) * f.size())) return 1;
+    f.assign(ops.l2p.begin(), ops.l2p.end());                                  // [w*w][K]
+    if (upload((void **)&d_l2p, f.data(), sizeof(float) * f.size())) return 1;
+    f.assign(ops.m2m.begin(), ops.m2m.end());                                  // [4][K][K]
+    if (upload((void **)&d_m2m, f.data(), sizeof(float) * f.size())) return 1;
+    // MFMA A-fragments: frag[blk][rt][lane][u] = Op[16 rt + lane % 16][G (lane / 16) + u]
+    const int G = KP / 4, RT = KP / 16;
+    auto frag = [&](const double *op, std::vector<float> &out, size_t off) {
+        for (int rt = 0; rt < RT; rt++)
+            for (int lane = 0; lane < 64; lane++)
+                for (int u = 0; u < G; u++) {
+                    const int row = 16 * rt + lane % 16, col = G * (lane / 16) + u;
+                    out[off + ((size_t)rt * 64 + lane) * G + u] =
+                        (row < K && col < K) ? (float)op[(size_t)row * K + col] : 0.0f;
+                }
+    };
+    const size_t blk = (size_t)RT * 64 * G;  // floats per KP x KP fragment block (= KP*KP)
+    std::vector<float> l2lf(4 * blk);
+    for (int ci = 0; ci < 4; ci++) frag(&ops.l2l[(size_t)ci * K * K], l2lf, ci * blk);
+    if (upload((void **)&d_l2l, l2lf.data(), sizeof(float) * l2lf.size())) return 1;
+    // M2L fragments per level: [dz][49][blk]
+    const size_t nblk = (size_t)(2 * zd - 1) * 49;
+    m2l_level_bytes = (precision == MMS_M2L_FP32) ? nblk * blk * sizeof(float)
+                                                  : nblk * blk * sizeof(unsigned short) * (precision == MMS_M2L_BF16X3 ? 2 : 1);
+    std::vector<char> all(m2l_level_bytes * (Lleaf - 1));
+    std::vector<float> lv(nblk * blk);
     for (int L = 2; L <= Lleaf; L++) {
         std::vector<double> T = ops.m2l_level(L, Lmax, zd, (double)g.ay, (double)g.az);
-        all.insert(all.end(), T.begin(), T.end());
+        for (size_t b = 0; b < nblk; b++) frag(&T[b * K * K], lv, b * blk);
+        char *dst = all.data() + (size_t)(L - 2) * m2l_level_bytes;
+        if (precision == MMS_M2L_FP32) {
+            memcpy(dst, lv.data(), m2l_level_bytes);
+        } else {
+            unsigned short *h = reinterpret_cast<unsigned short *>(dst);
+            for (size_t i = 0; i < lv.size(); i++) {
+                h[i] = host_bf16(lv[i]);
+                if (precision == MMS_M2L_BF16X3) h[lv.size() + i] = host_bf16(lv[i] - host_bf2f(h[i]));
+            }
+        }
     }
-    if (upload(&d_m2l, all)) return 1;
+    if (upload((void **)&d_m2l, all.data(), all.size())) return 1;
     // near-field 1/r table over offsets dx, dy in [-(2w-1), 2w-1], dz in [-(zd-1), zd-1]
     near_side = 4 * w - 1;
-    std::vector<double> tab((size_t)(2 * zd - 1) * near_side * near_side);
+    std::vector<float> tab((size_t)(2 * zd - 1) * near_side * near_side);
     for (int dz = -(zd - 1); dz <= zd - 1; dz++)
         for (int dy = -(2 * w - 1); dy <= 2 * w - 1; dy++)
             for (int dx = -(2 * w - 1); dx <= 2 * w - 1; dx++) {
                 const double ry = dy * (double)g.ay, rz = dz * (double)g.az;
                 const double r2 = (double)dx * dx + ry * ry + rz * rz;
                 tab[((size_t)(dz + zd - 1) * near_side + (dy + 2 * w - 1)) * near_side + (dx + 2 * w - 1)] =
-                    r2 > 0 ? 1.0 / std::sqrt(r2) : 0.0;
+                    r2 > 0 ? (float)(1.0 / std::sqrt(r2)) : 0.0f;
             }
-    if (upload(&d_near, tab)) return 1;
+    if (upload((void **)&d_near, tab.data(), sizeof(float) * tab.size())) return 1;
     return 0;
 }
 
 void FmmPlan::release() {
-    float **bufs[] = {&mult, &loc, &d_p2m, &d_l2p, &d_m2m, &d_l2l, &d_m2l, &d_near};
-    for (float **b : bufs) {
-        if (*b) (void)hipFree(*b);
-        *b = nullptr;
-    }
-    if (d_m2l_bf) (void)hipFree(d_m2l_bf);
-    d_m2l_bf = nullptr;
+    void *bufs[] = {mult, loc, d_p2m, d_l2p, d_m2m, d_l2l, d_m2l, d_near};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    mult = loc = d_p2m = d_l2p = d_m2m = d_l2l = d_near = nullptr;
+    d_m2l = nullptr;
+    m2l_level_bytes = 0;
 }
 
 }  // namespace mms

@@ -89,21 +89,22 @@ int direct_probes_f64(const Grid &g, const float *rho, const int64_t *probes, in
 // stored as K = (P+1)(P+2)/2 reals per box and layer.
 struct FmmOps;  // host FP64 operators (fmm_operators.cpp)
 struct FmmPlan {
-    int P = 0, K = 0, w = 4, Lmax = 0, Lleaf = 0, zd = 0;
+    int P = 0, K = 0, KP = 0, w = 4, Lmax = 0, Lleaf = 0, zd = 0;
     int precision = MMS_M2L_FP32;
-    // device buffers
-    float *mult = nullptr;   // all levels, [level][z][by][bx][K]
+    int m2l_cols = 8;        // MFMA column tiles (16 targets each) per wave in M2L: 4 or 8
+    // device buffers; box vectors have KP slots (K rounded up to 32, zero padded)
+    float *mult = nullptr;   // all levels, [level][z][by][bx][KP]
     float *loc = nullptr;    // all levels, same layout
     std::vector<long> lvl_off;  // float offset of each level in mult/loc
     float *d_p2m = nullptr;  // [K][w*w]
     float *d_l2p = nullptr;  // [w*w][K]
     float *d_m2m = nullptr;  // [4][K][K]
-    float *d_l2l = nullptr;  // [4][K][K]
-    float *d_m2l = nullptr;  // [level][dz][49 offsets][K][K] (level 2..Lleaf)
-    uint16_t *d_m2l_bf = nullptr;  // bf16 hi (and lo) copies for MFMA paths
+    float *d_l2l = nullptr;  // MFMA fragments [4 child positions][KP/16][64][KP/4]
+    void *d_m2l = nullptr;   // MFMA fragments per level [2zd-1][49 offsets][KP/16][64][KP/4]
+                             // (float, or bf16 hi [+ bf16 lo] planes)
+    size_t m2l_level_bytes = 0;
     float *d_near = nullptr; // near-field kernel table 1/r over offsets
     int near_side = 0;       // 4w-1
-    long m2l_level_stride = 0;
     int init(const Grid &g, int P, int w, int precision);
     void release();
 };

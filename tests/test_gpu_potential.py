@@ -174,3 +174,21 @@ def test_fmm_linearity_and_sign(mg):
     assert np.array_equal(pn, -pa)
     assert np.array_equal(p2, 2 * pa)
     assert _rel(pab, pa.astype(float) + pb) < 5e-6   # FP32 rounding of a + b and of the sums
+
+
+@pytest.mark.parametrize("mode,tol", [("fp32", 2e-6), ("bf16x3", 2e-5), ("bf16", 3e-3)])
+def test_fmm_m2l_precision_modes(mg, mode, tol):
+    """M2L on the matrix cores in each operand precision vs the FP64 FMM
+    oracle (same P, leaf width, tree): FP32 to rounding; hi/lo BF16 split
+    to ~2^-16 of the far field; plain BF16 to ~2^-8 of the far field."""
+    from oracle import fmm_oracle as fo
+    rng = np.random.default_rng(21)
+    xd, zd, P = 128, 4, 6
+    q = (rng.normal(size=(zd, xd, xd)) * 1e14).astype(F)
+    prec = {"fp32": mg.M2L_FP32, "bf16": mg.M2L_BF16, "bf16x3": mg.M2L_BF16X3}[mode]
+    with mg.Film(xd, xd, zd, *D5, solver=mg.SOLVER_FMM, P=P, m2l_precision=prec) as f:
+        pot = f.potential(q.ravel())
+    ref = fo.FmmOracle(xd, xd, zd, P, w=4).potential_units(q.astype(float)).ravel() * float(D5[0]) ** 2
+    err = _rel(pot, ref)
+    _say(f"m2l {mode}: rel err vs FP64 FMM {err:.2e}")
+    assert err < tol

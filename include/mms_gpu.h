@@ -16,7 +16,7 @@
  *     function-level statics sized by the first call, so any sequence of shapes
  *     is accepted;
  *   - fmm_calc accepts any xdim, ydim (zero-padded to a power-of-two square)
- *     and expansion orders P = 1..12 (the reference asserts P <= 4).
+ *     and expansion orders P = 1..9 (the reference asserts P <= 4).
  *
  * Part 2 is the resident API: an explicit context that owns the state on the
  * device, runs fixed-step RK4 without host round trips, and selects the
@@ -133,6 +133,8 @@ typedef struct {
     int demag, exchange;        /* 0/1 switches                                */
     int external;               /* enum mms_external                           */
     float hext[3];              /* A/m, for MMS_EXT_CONST                      */
+    int m2l_cols;               /* M2L targets per wavefront in 16-column MFMA tiles (4 or 8; 0 = 4) */
+    int direct_split;           /* direct sum: source-chunk multiplier (1, 2 or 4; 0 = 1) */
 } mms_params;
 
 /* Create a context owning device state for one film.  material: one byte per
@@ -172,6 +174,15 @@ int mms_potential(mms_ctx *ctx, const float *charge, float *potential);
 int mms_potential_probes_f64(mms_ctx *ctx, const float *charge, const int64_t *probes,
                              int nprobe, double *out);
 
+/* Autotuner: time the solver's launch shapes on the context's current
+ * charge (call after mms_set_m) -- FMM leaf width in {4, 8} x M2L column tiles
+ * in {4, 8}; direct sum source-chunk multiplier in {1, 2, 4} -- `reps` solves
+ * each, keep the fastest.  Accuracy settings (P, M2L precision) are not
+ * changed.  report (may be NULL) receives one line per candidate. */
+int mms_autotune(mms_ctx *ctx, int reps, char *report, int report_len);
+/* Current launch shape: leaf width, M2L column tiles, direct split. */
+int mms_get_shape(mms_ctx *ctx, int *leaf_width, int *m2l_cols, int *direct_split);
+
 /* Timing: mean device time (ms) per launch of each kernel class over the last
  * mms_profile_end() window; names are returned through mms_profile_name(). */
 int mms_profile_begin(mms_ctx *ctx);
@@ -180,6 +191,41 @@ int mms_profile_count(mms_ctx *ctx);
 const char *mms_profile_name(mms_ctx *ctx, int i);
 double mms_profile_ms(mms_ctx *ctx, int i);
 long mms_profile_launches(mms_ctx *ctx, int i);
+
+/* ------------------------------------------------------------------------ */
+/* Part 3: many independent films on one GPU (no counterpart in the          */
+/* reference, which runs one film per process)                              */
+/* ------------------------------------------------------------------------ */
+
+typedef struct mms_batch mms_batch;
+typedef struct {
+    int xdim, ydim;   /* film extent (zdim, cell size and material constants are common) */
+    float hext[3];    /* constant external field of this film, A/m                        */
+} mms_film_spec;
+
+/* Create a batch.  `common` gives zdim, dx/dy/dz, P, m2l_precision,
+ * m2l_cols, leaf_width, mu_0, Ms, Aexch, alfa, gamma, demag, exchange;
+ * solver = MMS_SOLVER_FMM lets films of >= 4096 cells use the FMM (padded to
+ * power-of-two size classes), smaller films use the direct sum; material is
+ * the films' flags packed in film order (NULL = all magnetic).  Films do not
+ * interact. */
+int mms_batch_create(mms_batch **b, const mms_params *common, int nfilm, const mms_film_spec *films,
+                     const unsigned char *material);
+int mms_batch_destroy(mms_batch *b);
+/* Packed host AoS M of every film in film order (film f's cells are
+ * contiguous, index z*ydim*xdim + y*xdim + x within the film). */
+int mms_batch_set_m(mms_batch *b, const float *M);
+int mms_batch_get_m(mms_batch *b, float *M);
+/* nsteps fixed-dt RK4 steps of every film; one launch per kernel class per
+ * stage over all films; use_graph != 0 replays a captured step. */
+int mms_batch_steps(mms_batch *b, int nsteps, float dt, int use_graph);
+int mms_batch_sync(mms_batch *b);
+int mms_batch_info(mms_batch *b, int *n_direct, int *n_fmm, int *n_classes, long *cells, int *launches_per_step);
+/* Autotuner: per FMM size class, leaf width x M2L column tiles on the current state. */
+int mms_batch_autotune(mms_batch *b, int reps, char *report, int report_len);
+int mms_batch_profile_begin(mms_batch *b);
+/* mean ms and launch count per kernel class (arrays of mms_profile_count entries) */
+int mms_batch_profile_end(mms_batch *b, double *ms_per_class, long *launches_per_class);
 
 /* Message for the last failing call on this host thread ("" if none). */
 const char *mms_last_error(void);

@@ -172,8 +172,12 @@ extern "C" int mms_create(mms_ctx **out, const mms_params *p, const unsigned cha
     c->prof.stream = c->stream;
     if (ctx_alloc(c.get(), material)) { ctx_free(c.get()); return 1; }
     int st = 0;
-    if (c->p.solver == MMS_SOLVER_FMM) st = c->fmm.init(c->g, c->p.P, c->p.leaf_width, c->p.m2l_precision);
-    else st = c->direct.init(c->g);
+    if (c->p.m2l_cols == 0) c->p.m2l_cols = 4;
+    if (c->p.direct_split == 0) c->p.direct_split = 1;
+    if (c->p.solver == MMS_SOLVER_FMM)
+        st = c->fmm.init(c->g, c->p.P, c->p.leaf_width, c->p.m2l_precision, c->p.m2l_cols);
+    else
+        st = c->direct.init(c->g, c->p.direct_split);
     if (st) { ctx_free(c.get()); return 1; }
     *out = c.release();
     return 0;
@@ -332,6 +336,84 @@ extern "C" int mms_potential_probes_f64(mms_ctx *c, const float *charge, const i
     return st;
 }
 
+// ------------------------------------------------------------ autotuner --
+static int time_solve(mms_ctx *c, int reps, double *ms) {
+    hipEvent_t a, b;
+    MMS_HIP(hipEventCreate(&a));
+    MMS_HIP(hipEventCreate(&b));
+    int st = c->potential(c->rho, c->phi);  // warm-up
+    if (!st && hipEventRecord(a, c->stream) != hipSuccess) st = 1;
+    for (int i = 0; i < reps && !st; i++) st = c->potential(c->rho, c->phi);
+    if (!st && hipEventRecord(b, c->stream) != hipSuccess) st = 1;
+    float t = 0.0f;
+    if (!st && (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&t, a, b) != hipSuccess)) st = 1;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms = t / (reps > 0 ? reps : 1);
+    return st;
+}
+
+extern "C" int mms_autotune(mms_ctx *c, int reps, char *report, int report_len) {
+    MMS_CHECK(c, "null context");
+    if (reps < 1) reps = 3;
+    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    std::string rep;
+    // charge of the current state
+    if (launch_div(c->g, c->m32[c->cur], c->rho, c->stream, nullptr)) return 1;
+    char line[160];
+    if (c->p.solver == MMS_SOLVER_FMM) {
+        int best_w = c->p.leaf_width, best_c = c->p.m2l_cols;
+        double best = 1e30;
+        const int side = c->g.xd > c->g.yd ? c->g.xd : c->g.yd;
+        int S = 1;
+        while (S < side) S <<= 1;
+        for (int w : {4, 8}) {
+            if (S / w < 4) continue;
+            for (int cols : {4, 8}) {
+                if (c->fmm.init(c->g, c->p.P, w, c->p.m2l_precision, cols)) return 1;
+                double ms = 0;
+                if (time_solve(c, reps, &ms)) return 1;
+                snprintf(line, sizeof line, "fmm leaf=%d m2l_cols=%d: %.4f ms/solve\n", w, cols, ms);
+                rep += line;
+                if (ms < best) { best = ms; best_w = w; best_c = cols; }
+            }
+        }
+        c->p.leaf_width = best_w;
+        c->p.m2l_cols = best_c;
+        if (c->fmm.init(c->g, c->p.P, best_w, c->p.m2l_precision, best_c)) return 1;
+        snprintf(line, sizeof line, "chosen: leaf=%d m2l_cols=%d\n", best_w, best_c);
+        rep += line;
+    } else {
+        int best_s = c->p.direct_split;
+        double best = 1e30;
+        for (int sp : {1, 2, 4}) {
+            if (c->direct.init(c->g, sp)) return 1;
+            double ms = 0;
+            if (time_solve(c, reps, &ms)) return 1;
+            snprintf(line, sizeof line, "direct split=%d: %.4f ms/solve\n", sp, ms);
+            rep += line;
+            if (ms < best) { best = ms; best_s = sp; }
+        }
+        c->p.direct_split = best_s;
+        if (c->direct.init(c->g, best_s)) return 1;
+        snprintf(line, sizeof line, "chosen: direct split=%d\n", best_s);
+        rep += line;
+    }
+    if (report && report_len > 0) {
+        strncpy(report, rep.c_str(), report_len - 1);
+        report[report_len - 1] = 0;
+    }
+    return 0;
+}
+
+extern "C" int mms_get_shape(mms_ctx *c, int *leaf_width, int *m2l_cols, int *direct_split) {
+    MMS_CHECK(c, "null context");
+    if (leaf_width) *leaf_width = c->p.solver == MMS_SOLVER_FMM ? c->fmm.w : 0;
+    if (m2l_cols) *m2l_cols = c->p.solver == MMS_SOLVER_FMM ? c->fmm.m2l_cols : 0;
+    if (direct_split) *direct_split = c->p.solver == MMS_SOLVER_DIRECT ? c->direct.split : 0;
+    return 0;
+}
+
 extern "C" int mms_profile_begin(mms_ctx *c) {
     MMS_CHECK(c, "null context");
     c->prof.flush();
@@ -465,7 +547,7 @@ extern "C" int calc_potential_exact_gpu(const float *charge, int xdim, int ydim,
 extern "C" int fmm_calc(const float *charge, float *potential, int xdim, int ydim, int zdim, float dx, float dy,
                         float dz, int P, int use_gpu, int verbosity) {
     (void)use_gpu;
-    if (P < 1 || P > 10) { set_error("fmm_calc: P must be in 1..10"); return EXIT_FAILURE; }
+    if (P < 1 || P > 9) { set_error("fmm_calc: P must be in 1..9"); return EXIT_FAILURE; }
     std::lock_guard<std::mutex> lk(g_cache_mu);
     timeval t1, t2;
     gettimeofday(&t1, nullptr);
@@ -495,7 +577,7 @@ extern "C" int Hfield(const float *M, float *H, float *charge, float *potential,
                       int zdim, float dx, float dy, float dz, int P, float mu_0, float Ms, float Aexch, int demag,
                       int exchange, int external, int use_fmm, int use_gpu, int verbosity) {
     (void)use_gpu; (void)verbosity;
-    if (use_fmm && (P < 1 || P > 10)) { set_error("Hfield: P must be in 1..10"); return EXIT_FAILURE; }
+    if (use_fmm && (P < 1 || P > 9)) { set_error("Hfield: P must be in 1..9"); return EXIT_FAILURE; }
     std::lock_guard<std::mutex> lk(g_cache_mu);
     mms_params p = field_params(xdim, ydim, zdim, dx, dy, dz, P, mu_0, Ms, Aexch, demag, exchange, external, use_fmm);
     mms_ctx *c = cached_ctx(p, nullptr);
@@ -522,7 +604,7 @@ extern "C" int rk4_step(float t, float dt, float *t2, const unsigned char *mater
                         float alfa, float gamma, int demag, int exchange, int external, int use_fmm, int normalize,
                         int use_gpu, int verbosity) {
     (void)use_gpu; (void)verbosity;
-    if (use_fmm && (P < 1 || P > 10)) { set_error("rk4_step: P must be in 1..10"); return EXIT_FAILURE; }
+    if (use_fmm && (P < 1 || P > 9)) { set_error("rk4_step: P must be in 1..9"); return EXIT_FAILURE; }
     std::lock_guard<std::mutex> lk(g_cache_mu);
     mms_params p = field_params(xdim, ydim, zdim, dx, dy, dz, P, mu_0, Ms, Aexch, demag, exchange, external, use_fmm);
     p.alfa = alfa; p.gamma = gamma;
@@ -601,7 +683,7 @@ extern "C" int time_marching(unsigned char *material, float *M, float finaltime,
                              float alfa, float gamma, int demag, int exchange, int external, int use_fmm,
                              int use_gpu, const char *sim_name, int verbosity) {
     (void)use_gpu;
-    if (use_fmm && (P < 1 || P > 10)) { set_error("time_marching: P must be in 1..10"); return EXIT_FAILURE; }
+    if (use_fmm && (P < 1 || P > 9)) { set_error("time_marching: P must be in 1..9"); return EXIT_FAILURE; }
     mms_params p = field_params(xdim, ydim, zdim, dx, dy, dz, P, mu_0, Ms, Aexch, demag, exchange, external, use_fmm);
     p.alfa = alfa; p.gamma = gamma;
     mms_ctx *c = nullptr;

@@ -98,14 +98,73 @@ __global__ __launch_bounds__(256) void probes_f64_kernel(const float *__restrict
     if (threadIdx.x == 0) out[blockIdx.x] = red[0] * dx * dy * dz;
 }
 
+// Batched small films: block b takes work item items[b] = (film, target block)
+// and sums every source of its film (LDS tiles, FP32 per tile, FP64 across
+// tiles), writing phi directly.
+__global__ __launch_bounds__(TB) void direct_batch_kernel(const float *__restrict__ rho, float *__restrict__ phi,
+                                                          const int2 *__restrict__ items,
+                                                          const BatchFilm *__restrict__ films, float ay, float az,
+                                                          double scale) {
+    __shared__ float4 src[TILE];
+    const int2 it = items[blockIdx.x];
+    const BatchFilm f = films[it.x];
+    const int xd = f.xd, yd = f.yd;
+    const long plane = (long)xd * yd, n = plane * f.zd;
+    rho += f.off;
+    phi += f.off;
+    const long t = (long)it.y * TB + threadIdx.x;
+    float tx = 0.f, ty = 0.f, tz = 0.f;
+    if (t < n) {
+        tx = (float)(t % xd);
+        ty = (float)((t / xd) % yd) * ay;
+        tz = (float)(t / plane) * az;
+    }
+    double acc = 0.0;
+    for (long base = 0; base < n; base += TILE) {
+        const long j = base + threadIdx.x;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (j < n) {
+            v.x = (float)(j % xd);
+            v.y = (float)((j / xd) % yd) * ay;
+            v.z = (float)(j / plane) * az;
+            v.w = rho[j];
+        }
+        __syncthreads();
+        src[threadIdx.x] = v;
+        __syncthreads();
+        const int cnt = (int)((n - base) < TILE ? (n - base) : TILE);
+        float s = 0.f;
+#pragma unroll 8
+        for (int k = 0; k < cnt; k++) {
+            const float4 q = src[k];
+            const float ddx = tx - q.x, ddy = ty - q.y, ddz = tz - q.z;
+            const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            const float inv = (r2 > 0.f) ? rsqrtf(r2) : 0.f;
+            s = fmaf(q.w, inv, s);
+        }
+        acc += (double)s;
+    }
+    if (t < n) phi[t] = (float)(acc * scale);
+}
+
 }  // namespace
 
-int DirectPlan::init(const Grid &g) {
+int direct_batch_potential(const float *rho, float *phi, const int2 *items, int nitems, const BatchFilm *films,
+                           float ay, float az, double scale, hipStream_t s, Profiler *prof) {
+    KTimer kt(prof, K_DIRECT);
+    if (nitems <= 0) return 0;
+    hipLaunchKernelGGL(direct_batch_kernel, dim3(nitems), dim3(TB), 0, s, rho, phi, items, films, ay, az, scale);
+    MMS_HIP(hipGetLastError());
+    return 0;
+}
+
+int DirectPlan::init(const Grid &g, int split_) {
     release();
+    split = (split_ == 2 || split_ == 4) ? split_ : 1;
     const long n = g.n();
     const long tblocks = (n + TB - 1) / TB;
     // aim for >= 1024 blocks in total (4 per CU) without chunks below one tile
-    long want = (1024 + tblocks - 1) / tblocks;
+    long want = (1024L * split + tblocks - 1) / tblocks;
     long maxchunks = (n + TILE - 1) / TILE;
     if (want > maxchunks) want = maxchunks;
     if (want < 1) want = 1;

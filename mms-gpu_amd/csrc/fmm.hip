@@ -56,8 +56,15 @@ __device__ inline float bf2f(unsigned short h) { return __uint_as_float((unsigne
 // ------------------------------------------------------------------ P2M --
 template <int K, int KP, int W>
 __global__ __launch_bounds__(256) void p2m_leaf_kernel(const float *__restrict__ rho, float *__restrict__ M,
-                                                       const float *__restrict__ p2m, int xd, int yd, int nb) {
-    const int bx = blockIdx.x * 256 + threadIdx.x, by = blockIdx.y, z = blockIdx.z;
+                                                       const float *__restrict__ p2m, int xd, int yd, int nb,
+                                                       FilmSet fs) {
+    const int bx = blockIdx.x * 256 + threadIdx.x, by = blockIdx.y;
+    const int f = blockIdx.z / fs.zd, z = blockIdx.z % fs.zd;
+    if (fs.desc) {
+        const FilmDesc d = fs.desc[f];
+        xd = d.xd; yd = d.yd; rho += d.off;
+    }
+    M += (long)f * fs.stride;
     if (bx >= nb) return;
     const long plane = (long)xd * yd;
     float q[W * W];
@@ -83,33 +90,60 @@ __global__ __launch_bounds__(256) void p2m_leaf_kernel(const float *__restrict__
 }
 
 // ------------------------------------------------------------------ M2M --
-template <int K, int KP>
-__global__ __launch_bounds__(256) void m2m_kernel(const float *__restrict__ Mc, float *__restrict__ Mp,
-                                                  const float *__restrict__ m2m, int nbp) {
-    const int bx = blockIdx.x * 256 + threadIdx.x, by = blockIdx.y, z = blockIdx.z;
-    if (bx >= nbp) return;
+// One wave per 64 parent boxes of one layer (4 MFMA column tiles of 16):
+// C[KP x 64] = sum over the 4 child positions of M2M_c[KP x KP] * M_child[KP x 64],
+// FP32 MFMA, operator fragments in lane order (see M2L below).
+template <int KP>
+__global__ __launch_bounds__(256) void m2m_mfma_kernel(const float *__restrict__ Mc, float *__restrict__ Mp,
+                                                       const float *__restrict__ m2mfrag, int nbp, FilmSet fs) {
+    constexpr int G = KP / 4, RT = KP / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j16 = lane & 15, g = lane >> 4;
+    const int by = blockIdx.y, z = blockIdx.z % fs.zd;
+    Mc += (long)(blockIdx.z / fs.zd) * fs.stride;
+    Mp += (long)(blockIdx.z / fs.zd) * fs.stride;
+    const int bx0 = (blockIdx.x * 4 + wave) * 64;
+    if (bx0 >= nbp) return;
     const int nbc = 2 * nbp;
-    float acc[K];
+    f32x4 acc[RT][4];
 #pragma unroll
-    for (int a = 0; a < K; a++) acc[a] = 0.0f;
+    for (int r = 0; r < RT; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 1
     for (int ci = 0; ci < 4; ci++) {
-        const int cx = 2 * bx + (ci & 1), cy = 2 * by + (ci >> 1);
-        const float *src = Mc + (((long)z * nbc + cy) * nbc + cx) * KP;
-        float v[K];
+        const int cyy = 2 * by + (ci >> 1);
 #pragma unroll
-        for (int b = 0; b < K; b++) v[b] = src[b];
-        const float *T = m2m + (long)ci * K * K;
+        for (int u4 = 0; u4 < G / 4; u4++) {
+            f32x4 a[RT], b[4];
 #pragma unroll
-        for (int a = 0; a < K; a++)
+            for (int r = 0; r < RT; r++)
+                a[r] = *reinterpret_cast<const f32x4 *>(m2mfrag + (((long)ci * RT + r) * 64 + lane) * G + 4 * u4);
 #pragma unroll
-            for (int b = 0; b < K; b++) acc[a] = fmaf(T[a * K + b], v[b], acc[a]);
+            for (int c = 0; c < 4; c++) {
+                const int bx = bx0 + 16 * c + j16;
+                b[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+                if (bx < nbp)
+                    b[c] = *reinterpret_cast<const f32x4 *>(Mc + (((long)z * nbc + cyy) * nbc + 2 * bx + (ci & 1)) * KP +
+                                                            g * G + 4 * u4);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int r = 0; r < RT; r++)
+#pragma unroll
+                    for (int c = 0; c < 4; c++)
+                        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r][u], b[c][u], acc[r][c], 0, 0, 0);
+        }
     }
-    float *out = Mp + (((long)z * nbp + by) * nbp + bx) * KP;
 #pragma unroll
-    for (int a = 0; a < K; a++) out[a] = acc[a];
+    for (int c = 0; c < 4; c++) {
+        const int bx = bx0 + 16 * c + j16;
+        if (bx >= nbp) continue;
+        f32x4 *dst = reinterpret_cast<f32x4 *>(Mp + (((long)z * nbp + by) * nbp + bx) * KP);
 #pragma unroll
-    for (int a = K; a < KP; a++) out[a] = 0.0f;
+        for (int r = 0; r < RT; r++) dst[4 * r + g] = acc[r][c];
+    }
 }
 
 // -------------------------------------------------------- M2L + L2L --
@@ -133,13 +167,20 @@ template <int KP, int MODE, int NC>
 __global__ __launch_bounds__(256) void m2l_mfma_kernel(const float *__restrict__ M, float *__restrict__ L,
                                                        const float *__restrict__ Lpar,
                                                        const void *__restrict__ Tfrag,
-                                                       const float *__restrict__ l2lfrag, int nb, int zd) {
+                                                       const float *__restrict__ l2lfrag, int nb, int zd,
+                                                       long film_stride) {
     typedef M2LCfg<KP, MODE, NC> C;
+    {
+        const long fo = (long)(blockIdx.z / zd) * film_stride;
+        M += fo;
+        L += fo;
+        if (Lpar) Lpar += fo;
+    }
     constexpr int G = C::G, RT = C::RT, SB = C::SB, SX = C::SX, WX = C::WX, WY = C::WY;
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int px = wave & 1, py = wave >> 1;
-    const int x0 = blockIdx.x * C::TX, y0 = blockIdx.y * 16, zt = blockIdx.z;
+    const int x0 = blockIdx.x * C::TX, y0 = blockIdx.y * 16, zt = blockIdx.z % zd;
     const int j16 = lane & 15, g = lane >> 4;
     // window offset of this lane's target in column tile c: target t = 16c + j16
     int tbox[NC];
@@ -303,13 +344,22 @@ template <int K, int KP, int W>
 __global__ __launch_bounds__(256) void p2p_l2p_kernel(const float *__restrict__ rho, float *__restrict__ phi,
                                                       const float *__restrict__ Lleaf, const float *__restrict__ l2p,
                                                       const float *__restrict__ table, int xd, int yd, int zd,
-                                                      int nb, float scale) {
+                                                      int nb, float scale, FilmSet fs) {
     constexpr int TW = 16, SW = TW + 2 * W, TS = 4 * W - 1;
     extern __shared__ float lds[];
     float *src = lds;                      // [zd][SW][SW]
     float *tab = lds + zd * SW * SW;       // [2zd-1][TS][TS]
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TW, zt = blockIdx.z;
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TW, zt = blockIdx.z % zd;
+    {
+        const int f = blockIdx.z / zd;
+        if (fs.desc) {
+            const FilmDesc d = fs.desc[f];
+            xd = d.xd; yd = d.yd; rho += d.off; phi += d.off;
+            if (x0 >= xd || y0 >= yd) return;  // block-uniform
+        }
+        Lleaf += (long)f * fs.stride;
+    }
     const long plane = (long)xd * yd;
     for (int e = threadIdx.x; e < zd * SW * SW; e += 256) {
         const int zs = e / (SW * SW), r = e % (SW * SW), sy = r / SW, sx = r % SW;
@@ -345,6 +395,103 @@ __global__ __launch_bounds__(256) void p2p_l2p_kernel(const float *__restrict__ 
     phi[(long)zt * plane + (long)y * xd + x] = (near + far) * scale;
 }
 
+// -------------------------------------------- P2P + L2P on the matrix cores --
+// Leaf width W (4 or 8): a leaf box has W*W cells = RT 16-row MFMA tiles.  A
+// block owns 4 x 4 leaf boxes (4W x 4W cells, one 16-column tile) and all
+// layers; wave zt (looping when zd > 4) computes for its target layer
+//   phi[cell i of box j] = sum_k G[i][k] q_j[k] + sum_s L2P[i][s] L_j[s]
+// where k runs over the 3W x 3W x zd source window of box j (its 3 x 3
+// neighbour leaves in every layer) and G[i][k] = 1/r is read from the LDS
+// offset table (self entry 0).  FP32 MFMA 16x16x4 with k = (zs, sy, sx) and
+// sx in groups of 4 matching the 4 lane groups.
+template <int K, int KP, int W>
+__global__ __launch_bounds__(256) void p2p_l2p_mfma_kernel(const float *__restrict__ rho, float *__restrict__ phi,
+                                                           const float *__restrict__ Lleaf,
+                                                           const float *__restrict__ l2pfrag,
+                                                           const float *__restrict__ table, int xd, int yd, int zd,
+                                                           int nb, float scale, FilmSet fs) {
+    constexpr int TW = 4 * W, SW = TW + 2 * W, TS = 4 * W - 1;
+    if (fs.desc) {
+        const FilmDesc d = fs.desc[blockIdx.z];
+        xd = d.xd; yd = d.yd; rho += d.off; phi += d.off;
+        if ((int)blockIdx.x * TW >= xd || (int)blockIdx.y * TW >= yd) return;  // block-uniform
+    }
+    Lleaf += (long)blockIdx.z * fs.stride;
+    constexpr int G = KP / 4, RT = W * W / 16, NSG = 3 * W / 4;
+    extern __shared__ float lds[];
+    float *src = lds;                      // [zd][SW][SW]
+    float *tab = lds + zd * SW * SW;       // [2zd-1][TS][TS]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j16 = lane & 15, g = lane >> 4;
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TW;
+    const long plane = (long)xd * yd;
+    for (int e = threadIdx.x; e < zd * SW * SW; e += 256) {
+        const int zs = e / (SW * SW), r = e % (SW * SW), sy = r / SW, sx = r % SW;
+        const int gx = x0 - W + sx, gy = y0 - W + sy;
+        src[e] = (gx >= 0 && gx < xd && gy >= 0 && gy < yd) ? rho[(long)zs * plane + (long)gy * xd + gx] : 0.0f;
+    }
+    for (int e = threadIdx.x; e < (2 * zd - 1) * TS * TS; e += 256) tab[e] = table[e];
+    __syncthreads();
+    // B column j16: box (jx, jy) of the block; A row j16 of tile r: cell 16 r + j16
+    const int jx = j16 & 3, jy = j16 >> 2;
+    int toff[RT];
+#pragma unroll
+    for (int r = 0; r < RT; r++) {
+        const int c = 16 * r + j16, iy = c / W, ix = c % W;
+        toff[r] = (W - 1 - iy) * TS + (W - 1 - ix);
+    }
+    for (int zt = wave; zt < zd; zt += 4) {
+        f32x4 acc[RT];
+#pragma unroll
+        for (int r = 0; r < RT; r++) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+        for (int zs = 0; zs < zd; zs++) {
+            const float *tb = tab + (zs - zt + zd - 1) * TS * TS;
+            const float *sb = src + zs * SW * SW + (W * jy) * SW + (W * jx);
+#pragma unroll 2
+            for (int sy = 0; sy < 3 * W; sy++) {
+#pragma unroll
+                for (int sg = 0; sg < NSG; sg++) {
+                    const int sx = 4 * sg + g;  // this lane group's source column
+                    const float b = sb[sy * SW + sx];
+#pragma unroll
+                    for (int r = 0; r < RT; r++)
+                        acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(tb[toff[r] + sy * TS + sx], b, acc[r], 0, 0, 0);
+                }
+            }
+        }
+        // far field: L2P[W*W x KP] * L[KP x 16 boxes]
+        const int bxg = blockIdx.x * 4 + jx, byg = blockIdx.y * 4 + jy;
+        const bool boxok = (bxg < nb && byg < nb);
+        const float *Lv = Lleaf + (((long)zt * nb + (boxok ? byg : 0)) * nb + (boxok ? bxg : 0)) * KP + g * G;
+#pragma unroll
+        for (int u4 = 0; u4 < G / 4; u4++) {
+            f32x4 b = boxok ? *reinterpret_cast<const f32x4 *>(Lv + 4 * u4) : f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int r = 0; r < RT; r++) {
+                const f32x4 a = *reinterpret_cast<const f32x4 *>(l2pfrag + ((long)r * 64 + lane) * G + 4 * u4);
+#pragma unroll
+                for (int u = 0; u < 4; u++) acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], acc[r], 0, 0, 0);
+            }
+        }
+        // D row 4g + q of tile r = cell 16 r + 4g + q of box j16: 4 consecutive x cells
+#pragma unroll
+        for (int r = 0; r < RT; r++) {
+            const int c = 16 * r + 4 * g;
+            const int y = y0 + W * jy + c / W, xb = x0 + W * jx + c % W;
+            if (y >= yd) continue;
+            float *dst = phi + (long)zt * plane + (long)y * xd + xb;
+            if (xb + 3 < xd && (((long)zt * plane + (long)y * xd + xb) & 3) == 0) {
+                *reinterpret_cast<f32x4 *>(dst) = acc[r] * scale;
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    if (xb + q < xd) dst[q] = acc[r][q] * scale;
+            }
+        }
+    }
+}
+
 // -------------------------------------------------------------- dispatch --
 template <int KP, int MODE, int NC>
 int launch_m2l(dim3 block_grid_unused, const FmmPlan &pl, int L, hipStream_t s) {
@@ -357,11 +504,11 @@ int launch_m2l(dim3 block_grid_unused, const FmmPlan &pl, int L, hipStream_t s) 
         attr_done = 1;
     }
     const int nb = 1 << L, zd = pl.zd;
-    dim3 grid((nb + C::TX - 1) / C::TX, (nb + 15) / 16, zd);
+    dim3 grid((nb + C::TX - 1) / C::TX, (nb + 15) / 16, zd * pl.nfilm);
     const float *Lpar = (L > 2) ? pl.loc + pl.lvl_off[L - 1] : nullptr;
     const char *T = reinterpret_cast<const char *>(pl.d_m2l) + (size_t)(L - 2) * pl.m2l_level_bytes;
     hipLaunchKernelGGL((m2l_mfma_kernel<KP, MODE, NC>), grid, dim3(256), (C::lds_bytes), s, pl.mult + pl.lvl_off[L],
-                       pl.loc + pl.lvl_off[L], Lpar, (const void *)T, pl.d_l2l, nb, zd);
+                       pl.loc + pl.lvl_off[L], Lpar, (const void *)T, pl.d_l2l, nb, zd, pl.film_stride);
     MMS_HIP(hipGetLastError());
     return 0;
 }
@@ -369,9 +516,11 @@ int launch_m2l(dim3 block_grid_unused, const FmmPlan &pl, int L, hipStream_t s) 
 template <int KP, int MODE>
 int launch_m2l_nc(const FmmPlan &pl, int L, hipStream_t s) {
     // wide tiles (8 column tiles per wave) once a level has enough boxes
-    if constexpr (KP <= 64) {
+    // wide tiles (8 column tiles per wave) when the source window fits in LDS
+    if constexpr (M2LCfg<KP, MODE, 8>::lds_bytes <= 160 * 1024) {
         if (pl.m2l_cols == 8 && (1 << L) >= 32) return launch_m2l<KP, MODE, 8>(dim3(), pl, L, s);
     }
+    static_assert(M2LCfg<KP, MODE, 4>::lds_bytes <= 160 * 1024, "M2L source window exceeds LDS");
     return launch_m2l<KP, MODE, 4>(dim3(), pl, L, s);
 }
 
@@ -382,17 +531,17 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
     {
         KTimer kt(prof, K_P2M);
         const int nb = 1 << pl.Lleaf;
-        dim3 grid((nb + 255) / 256, nb, zd);
+        dim3 grid((nb + 255) / 256, nb, zd * pl.nfilm);
         hipLaunchKernelGGL((p2m_leaf_kernel<K, KP, W>), grid, dim3(256), 0, s, rho, pl.mult + pl.lvl_off[pl.Lleaf],
-                           pl.d_p2m, g.xd, g.yd, nb);
+                           pl.d_p2m, g.xd, g.yd, nb, pl.films());
         MMS_HIP(hipGetLastError());
     }
     for (int L = pl.Lleaf - 1; L >= 2; L--) {
         KTimer kt(prof, K_M2M);
         const int nbp = 1 << L;
-        dim3 grid((nbp + 255) / 256, nbp, zd);
-        hipLaunchKernelGGL((m2m_kernel<K, KP>), grid, dim3(256), 0, s, pl.mult + pl.lvl_off[L + 1],
-                           pl.mult + pl.lvl_off[L], pl.d_m2m, nbp);
+        dim3 grid((nbp + 255) / 256, nbp, zd * pl.nfilm);
+        hipLaunchKernelGGL((m2m_mfma_kernel<KP>), grid, dim3(256), 0, s, pl.mult + pl.lvl_off[L + 1],
+                           pl.mult + pl.lvl_off[L], pl.d_m2m, nbp, pl.films());
         MMS_HIP(hipGetLastError());
     }
     for (int L = 2; L <= pl.Lleaf; L++) {
@@ -407,12 +556,29 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
     }
     {
         KTimer kt(prof, K_P2P_L2P);
-        constexpr int TW = 16, SW = TW + 2 * W, TS = 4 * W - 1;
-        const size_t lds = sizeof(float) * ((size_t)zd * SW * SW + (size_t)(2 * zd - 1) * TS * TS);
-        dim3 grid((g.xd + TW - 1) / TW, (g.yd + TW - 1) / TW, zd);
         const float scale = (float)((double)(g.dx * g.dy * g.dz) / (double)g.dx);
-        hipLaunchKernelGGL((p2p_l2p_kernel<K, KP, W>), grid, dim3(256), lds, s, rho, phi,
-                           pl.loc + pl.lvl_off[pl.Lleaf], pl.d_l2p, pl.d_near, g.xd, g.yd, zd, 1 << pl.Lleaf, scale);
+        if constexpr (W == 4 || W == 8) {
+            constexpr int TW = 4 * W, SW = TW + 2 * W, TS = 4 * W - 1;
+            const size_t lds = sizeof(float) * ((size_t)zd * SW * SW + (size_t)(2 * zd - 1) * TS * TS);
+            static int attr = 0;
+            if (!attr) {
+                MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&p2p_l2p_mfma_kernel<K, KP, W>),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                attr = 1;
+            }
+            MMS_CHECK(lds <= 160 * 1024, "FMM near field: too many layers for the LDS tile");
+            dim3 grid((g.xd + TW - 1) / TW, (g.yd + TW - 1) / TW, pl.nfilm);
+            hipLaunchKernelGGL((p2p_l2p_mfma_kernel<K, KP, W>), grid, dim3(256), lds, s, rho, phi,
+                               pl.loc + pl.lvl_off[pl.Lleaf], pl.d_l2pf, pl.d_near, g.xd, g.yd, zd, 1 << pl.Lleaf,
+                               scale, pl.films());
+        } else {
+            constexpr int TW = 16, SW = TW + 2 * W, TS = 4 * W - 1;
+            const size_t lds = sizeof(float) * ((size_t)zd * SW * SW + (size_t)(2 * zd - 1) * TS * TS);
+            dim3 grid((g.xd + TW - 1) / TW, (g.yd + TW - 1) / TW, zd * pl.nfilm);
+            hipLaunchKernelGGL((p2p_l2p_kernel<K, KP, W>), grid, dim3(256), lds, s, rho, phi,
+                               pl.loc + pl.lvl_off[pl.Lleaf], pl.d_l2p, pl.d_near, g.xd, g.yd, zd, 1 << pl.Lleaf,
+                               scale, pl.films());
+        }
         MMS_HIP(hipGetLastError());
     }
     return 0;
@@ -430,8 +596,7 @@ int run_fmm_w(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStrea
     case 7: return run_fmm<NC<7>::K, W>(g, pl, rho, phi, s, prof);
     case 8: return run_fmm<NC<8>::K, W>(g, pl, rho, phi, s, prof);
     case 9: return run_fmm<NC<9>::K, W>(g, pl, rho, phi, s, prof);
-    case 10: return run_fmm<NC<10>::K, W>(g, pl, rho, phi, s, prof);
-    default: set_error("FMM expansion order must be 1..10"); return 1;
+    default: set_error("FMM expansion order must be 1..9"); return 1;
     }
 }
 
@@ -460,14 +625,23 @@ int fmm_potential(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipS
     }
 }
 
-int FmmPlan::init(const Grid &g, int P_, int w_, int precision_) {
+int FmmPlan::init_batch(const Grid &g, int P_, int w_, int precision_, int cols_, const std::vector<FilmDesc> &fl) {
+    MMS_CHECK(!fl.empty(), "empty film batch");
+    if (init(g, P_, w_, precision_, cols_, (int)fl.size())) return 1;
+    MMS_HIP(hipMalloc(&d_films, sizeof(FilmDesc) * fl.size()));
+    MMS_HIP(hipMemcpy(d_films, fl.data(), sizeof(FilmDesc) * fl.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int nfilm_) {
     release();
-    MMS_CHECK(P_ >= 1 && P_ <= 10, "FMM expansion order must be 1..10");
+    nfilm = nfilm_ > 0 ? nfilm_ : 1;
+    MMS_CHECK(P_ >= 1 && P_ <= 9, "FMM expansion order must be 1..9");
     MMS_CHECK(w_ == 2 || w_ == 4 || w_ == 8, "FMM leaf width must be 2, 4 or 8");
     MMS_CHECK(g.zd >= 1 && g.zd <= 16, "FMM supports 1..16 layers");
     MMS_CHECK(precision_ >= MMS_M2L_FP32 && precision_ <= MMS_M2L_BF16X3, "unknown M2L precision");
     P = P_; w = w_; precision = precision_; zd = g.zd;
-    if (const char *e = getenv("MMS_M2L_COLS")) m2l_cols = (atoi(e) == 4) ? 4 : 8;
+    m2l_cols = (cols_ == 8) ? 8 : 4;
     K = fmm_ncoef(P);
     KP = ((K + 31) / 32) * 32;
     const int side = g.xd > g.yd ? g.xd : g.yd;
@@ -483,10 +657,11 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_) {
         lvl_off[L] = tot;
         tot += (long)zd * (1L << (2 * L)) * KP;
     }
-    MMS_HIP(hipMalloc(&mult, sizeof(float) * tot));
-    MMS_HIP(hipMalloc(&loc, sizeof(float) * tot));
-    MMS_HIP(hipMemset(mult, 0, sizeof(float) * tot));
-    MMS_HIP(hipMemset(loc, 0, sizeof(float) * tot));
+    film_stride = tot;
+    MMS_HIP(hipMalloc(&mult, sizeof(float) * tot * nfilm));
+    MMS_HIP(hipMalloc(&loc, sizeof(float) * tot * nfilm));
+    MMS_HIP(hipMemset(mult, 0, sizeof(float) * tot * nfilm));
+    MMS_HIP(hipMemset(loc, 0, sizeof(float) * tot * nfilm));
     FmmHostOps ops;
     fmm_build_ops(P, w, (double)g.ay, ops);
     auto upload = [&](void **dst, const void *src, size_t bytes) -> int {
@@ -498,8 +673,6 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_) {
     if (upload((void **)&d_p2m, f.data(), sizeof(float) * f.size())) return 1;
     f.assign(ops.l2p.begin(), ops.l2p.end());                                  // [w*w][K]
     if (upload((void **)&d_l2p, f.data(), sizeof(float) * f.size())) return 1;
-    f.assign(ops.m2m.begin(), ops.m2m.end());                                  // [4][K][K]
-    if (upload((void **)&d_m2m, f.data(), sizeof(float) * f.size())) return 1;
     // MFMA A-fragments: frag[blk][rt][lane][u] = Op[16 rt + lane % 16][G (lane / 16) + u]
     const int G = KP / 4, RT = KP / 16;
     auto frag = [&](const double *op, std::vector<float> &out, size_t off) {
@@ -512,9 +685,24 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_) {
                 }
     };
     const size_t blk = (size_t)RT * 64 * G;  // floats per KP x KP fragment block (= KP*KP)
-    std::vector<float> l2lf(4 * blk);
-    for (int ci = 0; ci < 4; ci++) frag(&ops.l2l[(size_t)ci * K * K], l2lf, ci * blk);
+    std::vector<float> l2lf(4 * blk), m2mf(4 * blk);
+    for (int ci = 0; ci < 4; ci++) {
+        frag(&ops.l2l[(size_t)ci * K * K], l2lf, ci * blk);
+        frag(&ops.m2m[(size_t)ci * K * K], m2mf, ci * blk);
+    }
     if (upload((void **)&d_l2l, l2lf.data(), sizeof(float) * l2lf.size())) return 1;
+    if (upload((void **)&d_m2m, m2mf.data(), sizeof(float) * m2mf.size())) return 1;
+    if (w == 4 || w == 8) {  // L2P fragments [w*w/16 row tiles][64 lanes][G]: L2P[16 r + lane % 16][G (lane / 16) + u]
+        const int rtw = w * w / 16;
+        std::vector<float> l2pf((size_t)rtw * 64 * G, 0.0f);
+        for (int r = 0; r < rtw; r++)
+            for (int lane = 0; lane < 64; lane++)
+                for (int u = 0; u < G; u++) {
+                    const int row = 16 * r + lane % 16, col = G * (lane / 16) + u;
+                    if (col < K) l2pf[((size_t)r * 64 + lane) * G + u] = (float)ops.l2p[(size_t)row * K + col];
+                }
+        if (upload((void **)&d_l2pf, l2pf.data(), sizeof(float) * l2pf.size())) return 1;
+    }
     // M2L fragments per level: [dz][49][blk]
     const size_t nblk = (size_t)(2 * zd - 1) * 49;
     m2l_level_bytes = (precision == MMS_M2L_FP32) ? nblk * blk * sizeof(float)
@@ -552,12 +740,15 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_) {
 }
 
 void FmmPlan::release() {
-    void *bufs[] = {mult, loc, d_p2m, d_l2p, d_m2m, d_l2l, d_m2l, d_near};
+    void *bufs[] = {mult, loc, d_p2m, d_l2p, d_l2pf, d_m2m, d_l2l, d_m2l, d_near};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    mult = loc = d_p2m = d_l2p = d_m2m = d_l2l = d_near = nullptr;
+    mult = loc = d_p2m = d_l2p = d_l2pf = d_m2m = d_l2l = d_near = nullptr;
     d_m2l = nullptr;
     m2l_level_bytes = 0;
+    if (d_films) (void)hipFree(d_films);
+    d_films = nullptr;
+    nfilm = 1;
 }
 
 }  // namespace mms

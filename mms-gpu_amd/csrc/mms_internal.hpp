@@ -74,7 +74,8 @@ struct DirectPlan {
     int nchunk = 0;      // source chunks
     long chunk = 0;      // sources per chunk
     double *part = nullptr;  // [nchunk][n] partial sums
-    int init(const Grid &g);
+    int split = 1;           // multiplier on the number of source chunks
+    int init(const Grid &g, int split = 1);
     void release();
 };
 int direct_potential(const Grid &g, DirectPlan &plan, const float *rho, float *phi,
@@ -87,25 +88,34 @@ int direct_probes_f64(const Grid &g, const float *rho, const int64_t *probes, in
 // leaf boxes of w x w cells at level Lleaf; expansions are truncated at order
 // P and keep the (n+m even) coefficients only (in-plane sources and targets),
 // stored as K = (P+1)(P+2)/2 reals per box and layer.
-struct FmmOps;  // host FP64 operators (fmm_operators.cpp)
+// A batch of films sharing one padded FMM size class: film f has xd x yd cells
+// (<= the plan's grid) starting at cell `off` of the packed rho/phi arrays.
+struct FilmDesc { int xd, yd; long off; };
+struct FilmSet { const FilmDesc *desc; long stride; int zd; };  // desc null: single film
 struct FmmPlan {
     int P = 0, K = 0, KP = 0, w = 4, Lmax = 0, Lleaf = 0, zd = 0;
     int precision = MMS_M2L_FP32;
-    int m2l_cols = 8;        // MFMA column tiles (16 targets each) per wave in M2L: 4 or 8
+    int m2l_cols = 4;        // MFMA column tiles (16 targets each) per wave in M2L: 4 or 8
     // device buffers; box vectors have KP slots (K rounded up to 32, zero padded)
     float *mult = nullptr;   // all levels, [level][z][by][bx][KP]
     float *loc = nullptr;    // all levels, same layout
     std::vector<long> lvl_off;  // float offset of each level in mult/loc
     float *d_p2m = nullptr;  // [K][w*w]
     float *d_l2p = nullptr;  // [w*w][K]
-    float *d_m2m = nullptr;  // [4][K][K]
+    float *d_l2pf = nullptr; // leaf width 4: L2P as MFMA fragments [64][KP/4]
+    float *d_m2m = nullptr;  // MFMA fragments [4 child positions][KP/16][64][KP/4]
     float *d_l2l = nullptr;  // MFMA fragments [4 child positions][KP/16][64][KP/4]
     void *d_m2l = nullptr;   // MFMA fragments per level [2zd-1][49 offsets][KP/16][64][KP/4]
                              // (float, or bf16 hi [+ bf16 lo] planes)
     size_t m2l_level_bytes = 0;
     float *d_near = nullptr; // near-field kernel table 1/r over offsets
     int near_side = 0;       // 4w-1
-    int init(const Grid &g, int P, int w, int precision);
+    int nfilm = 1;           // films in the batch (1 = single film)
+    long film_stride = 0;    // floats per film in mult/loc
+    FilmDesc *d_films = nullptr;
+    FilmSet films() const { return FilmSet{d_films, film_stride, zd}; }
+    int init(const Grid &g, int P, int w, int precision, int cols = 8, int nfilm = 1);
+    int init_batch(const Grid &g, int P, int w, int precision, int cols, const std::vector<FilmDesc> &films);
     void release();
 };
 int fmm_potential(const Grid &g, FmmPlan &plan, const float *rho, float *phi,
@@ -125,6 +135,8 @@ int fmm_build_ops(int P, int w, double ay, FmmHostOps &ops);
 int fmm_ncoef(int P);
 
 // -------------------------------------------------------- stencil / LLG --
+// Per-film record of a batch for the stencil/stage kernels (16-byte aligned).
+struct BatchFilm { int xd, yd, zd, pad; long off; float hx, hy, hz, pad2; };
 struct Material {
     float mu_0, Ms, Aexch, alfa, gamma;
     int demag, exchange, external;
@@ -152,6 +164,16 @@ int launch_stage_ex(const Grid &g, const Material &mat, int stage, double dt, co
 // Block partials [nblk][5] of (sum Mx, My, Mz, |M|, count) over material cells
 int launch_mavg(const float *m, const uint8_t *matl, long n, double *partial, int nblk, hipStream_t s);
 int launch_soa_from_aos(const float *aos, float *soa32, double *soa64, long n, hipStream_t s);
+// Batched films (tile list of (film, tile x, tile y) over 64 x 4 column tiles;
+// component planes pstride cells apart; per-film constant external field)
+constexpr int STENCIL_TILE_X = 64, STENCIL_TILE_Y = 4;
+int launch_div_batch(const float *m, float *rho, const int4 *tiles, int ntiles, const BatchFilm *films, long pstride,
+                     float dx, float dy, float dz, int zd, hipStream_t s, Profiler *prof);
+int launch_stage_batch(const Grid &g, const Material &mat, int stage, double dt, const float *phi, const float *m_in,
+                       float *m_out, double *mn, double *acc, const uint8_t *matl, const int4 *tiles, int ntiles,
+                       const BatchFilm *films, long pstride, hipStream_t s, Profiler *prof);
+int direct_batch_potential(const float *rho, float *phi, const int2 *items, int nitems, const BatchFilm *films,
+                           float ay, float az, double scale, hipStream_t s, Profiler *prof);
 int launch_aos_from_soa(const float *soa32, float *aos, long n, hipStream_t s);
 int launch_add_time(double *t_dev, double dt, hipStream_t s);
 int launch_fill_f32(float *p, float v, long n, hipStream_t s);

@@ -22,14 +22,16 @@ namespace {
 
 constexpr int BX = 64, BY = 4;
 
-__global__ __launch_bounds__(BX * BY) void div_kernel(const float *__restrict__ m, float *__restrict__ rho,
-                                                      int xd, int yd, int zd, float dx, float dy, float dz) {
-    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y;
+template <int ZD>
+__device__ __forceinline__ void div_column(const float *__restrict__ m, float *__restrict__ rho, int x, int y, int xd,
+                                           int yd, int zd_rt, long n, float dx, float dy, float dz) {
+    const int zd = ZD ? ZD : zd_rt;
     if (x >= xd || y >= yd) return;
-    const long plane = (long)xd * yd, n = plane * zd;
+    const long plane = (long)xd * yd;
     const float *mx = m, *my = m + n, *mz = m + 2 * n;
     const long c0 = (long)y * xd + x;
     float zprev = 0.0f, zcur = mz[c0];
+#pragma unroll
     for (int z = 0; z < zd; z++) {
         const long c = (long)z * plane + c0;
         const float znext = (z != zd - 1) ? mz[c + plane] : 0.0f;
@@ -42,6 +44,25 @@ __global__ __launch_bounds__(BX * BY) void div_kernel(const float *__restrict__ 
         zcur = znext;
     }
 }
+
+template <int ZD>
+__global__ __launch_bounds__(BX * BY) void div_kernel(const float *__restrict__ m, float *__restrict__ rho, int xd,
+                                                      int yd, int zd, float dx, float dy, float dz) {
+    div_column<ZD>(m, rho, blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y, xd, yd, zd,
+                   (long)xd * yd * zd, dx, dy, dz);
+}
+
+template <int ZD>
+__global__ __launch_bounds__(BX * BY) void div_batch_kernel(const float *__restrict__ m, float *__restrict__ rho,
+                                                            const int4 *__restrict__ tiles,
+                                                            const BatchFilm *__restrict__ films, long pstride,
+                                                            float dx, float dy, float dz) {
+    const int4 t = tiles[blockIdx.x];
+    const BatchFilm f = films[t.x];
+    div_column<ZD>(m + f.off, rho + f.off, t.y * BX + threadIdx.x, t.z * BY + threadIdx.y, f.xd, f.yd, f.zd, pstride,
+                   dx, dy, dz);
+}
+
 
 struct StageArgs {
     const float *phi;      // potential (null when demag off or h_given)
@@ -87,15 +108,25 @@ __device__ inline void llg_rhs(float Mx, float My, float Mz, float Hx, float Hy,
     kz = cz * (-gamma) - ddz * a;
 }
 
-__global__ __launch_bounds__(BX * BY) void stage_kernel(StageArgs A) {
-    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y;
-    const int xd = A.xd, yd = A.yd, zd = A.zd;
+// One (x, y) column of one film: film extent xd x yd x zd, its first cell at
+// `off` of the arrays, component planes `n` cells apart.
+template <int ZD>
+__device__ __forceinline__ void stage_column(const StageArgs &A, int x, int y, int xd, int yd, int zd_rt, long off,
+                                             long n, float hx, float hy, float hz) {
+    const int zd = ZD ? ZD : zd_rt;
     if (x >= xd || y >= yd) return;
-    const long plane = (long)xd * yd, n = plane * zd;
+    const long plane = (long)xd * yd;
+    const float *m_in_ = A.m_in + off;
+    float *m_out_ = A.m_out + off;
+    double *mn_ = A.mn + off;
+    double *acc_ = A.acc + off;
+    double *mn_out_ = A.mn_out ? A.mn_out + off : nullptr;
+    const float *phi_ = A.phi ? A.phi + off : nullptr;
+    const uint8_t *matl_ = A.matl ? A.matl + off : nullptr;
     const long c0 = (long)y * xd + x;
-    const float *mx = A.m_in, *my = A.m_in + n, *mz = A.m_in + 2 * n;
+    const float *mx = m_in_, *my = m_in_ + n, *mz = m_in_ + 2 * n;
     const double t = A.t_dev ? *A.t_dev : 0.0;
-    float hex = A.hx, hey = A.hy, hez = A.hz;
+    float hex = hx, hey = hy, hez = hz;
     if (A.external == 1) { hex = ramp_hx(t, A.inv_mu0); hey = 0.0f; hez = 0.0f; }
     const float dx2 = A.dx * A.dx, dy2 = A.dy * A.dy, dz2 = A.dz * A.dz;
     const bool need_field = (A.h_given == nullptr);
@@ -104,8 +135,9 @@ __global__ __launch_bounds__(BX * BY) void stage_kernel(StageArgs A) {
     float pprev = 0.0f, pcur = 0.0f;
     float mxp = 0.0f, myp = 0.0f, mzp = 0.0f;
     float mxc = mx[c0], myc = my[c0], mzc = mz[c0];
-    if (need_field && A.demag) pcur = A.phi[c0];
+    if (need_field && A.demag) pcur = phi_[c0];
 
+#pragma unroll
     for (int z = 0; z < zd; z++) {
         const long c = (long)z * plane + c0;
         const bool ztop = (z != zd - 1);
@@ -115,11 +147,11 @@ __global__ __launch_bounds__(BX * BY) void stage_kernel(StageArgs A) {
         if (need_field) {
             Hx = 0.0f; Hy = 0.0f; Hz = 0.0f;
             if (A.demag) {
-                if (ztop) pnext = A.phi[c + plane];
-                const float x2 = (x != xd - 1) ? A.phi[c + 1] : 0.0f;
-                const float x0 = (x != 0) ? A.phi[c - 1] : 0.0f;
-                const float y2 = (y != yd - 1) ? A.phi[c + xd] : 0.0f;
-                const float y0 = (y != 0) ? A.phi[c - xd] : 0.0f;
+                if (ztop) pnext = phi_[c + plane];
+                const float x2 = (x != xd - 1) ? phi_[c + 1] : 0.0f;
+                const float x0 = (x != 0) ? phi_[c - 1] : 0.0f;
+                const float y2 = (y != yd - 1) ? phi_[c + xd] : 0.0f;
+                const float y0 = (y != 0) ? phi_[c - xd] : 0.0f;
                 Hx = (x2 - x0) / (2.0f * A.dx) * A.inv4pi;
                 Hy = (y2 - y0) / (2.0f * A.dy) * A.inv4pi;
                 Hz = (pnext - pprev) / (2.0f * A.dz) * A.inv4pi;
@@ -150,14 +182,14 @@ __global__ __launch_bounds__(BX * BY) void stage_kernel(StageArgs A) {
         }
         if (A.h_out) { A.h_out[3 * c] = Hx; A.h_out[3 * c + 1] = Hy; A.h_out[3 * c + 2] = Hz; }
 
-        const bool magnetic = A.matl ? (A.matl[c] != 0) : true;
-        float *ox = A.m_out, *oy = A.m_out + n, *oz = A.m_out + 2 * n;
+        const bool magnetic = matl_ ? (matl_[c] != 0) : true;
+        float *ox = m_out_, *oy = m_out_ + n, *oz = m_out_ + 2 * n;
         if (magnetic) {
             float kx, ky, kz;
             llg_rhs(mxc, myc, mzc, Hx, Hy, Hz, A.alfa, A.gamma, A.Ms, kx, ky, kz);
             if (A.k_out) { A.k_out[3 * c] = kx; A.k_out[3 * c + 1] = ky; A.k_out[3 * c + 2] = kz; }
-            double *bx = A.mn, *by = A.mn + n, *bz = A.mn + 2 * n;
-            double *ax = A.acc, *ay = A.acc + n, *az = A.acc + 2 * n;
+            double *bx = mn_, *by = mn_ + n, *bz = mn_ + 2 * n;
+            double *ax = acc_, *ay = acc_ + n, *az = acc_ + 2 * n;
             const double b0 = bx[c], b1 = by[c], b2 = bz[c];
             double s0, s1, s2;
             if (A.stage == 0) { s0 = kx; s1 = ky; s2 = kz; }
@@ -177,7 +209,7 @@ __global__ __launch_bounds__(BX * BY) void stage_kernel(StageArgs A) {
                     const double f = (double)A.Ms / sqrt(n0 * n0 + n1 * n1 + n2 * n2);
                     n0 *= f; n1 *= f; n2 *= f;
                 }
-                double *o64 = A.mn_out ? A.mn_out : A.mn;
+                double *o64 = mn_out_ ? mn_out_ : mn_;
                 o64[c] = n0; o64[n + c] = n1; o64[2 * n + c] = n2;
             }
             ox[c] = (float)n0; oy[c] = (float)n1; oz[c] = (float)n2;
@@ -188,6 +220,23 @@ __global__ __launch_bounds__(BX * BY) void stage_kernel(StageArgs A) {
         mxp = mxc; myp = myc; mzp = mzc;
         mxc = mxn; myc = myn; mzc = mzn;
     }
+}
+
+template <int ZD>
+__global__ __launch_bounds__(BX * BY) void stage_kernel(StageArgs A) {
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y;
+    stage_column<ZD>(A, x, y, A.xd, A.yd, A.zd, 0, (long)A.xd * A.yd * A.zd, A.hx, A.hy, A.hz);
+}
+
+// Batched films: block b handles tile tiles[b] = (film, tile x, tile y) of a
+// film whose extent, cell offset and external field come from films[].
+template <int ZD>
+__global__ __launch_bounds__(BX * BY) void stage_batch_kernel(StageArgs A, const int4 *__restrict__ tiles,
+                                                              const BatchFilm *__restrict__ films, long pstride) {
+    const int4 t = tiles[blockIdx.x];
+    const BatchFilm f = films[t.x];
+    stage_column<ZD>(A, t.y * BX + threadIdx.x, t.z * BY + threadIdx.y, f.xd, f.yd, f.zd, f.off, pstride, f.hx, f.hy,
+                     f.hz);
 }
 
 __global__ __launch_bounds__(BX * BY) void hfield_kernel(StageArgs A) {
@@ -325,7 +374,11 @@ StageArgs make_args(const Grid &g, const Material &mat) {
 int launch_div(const Grid &g, const float *m, float *rho, hipStream_t s, Profiler *prof) {
     KTimer kt(prof, K_DIV);
     dim3 grid((g.xd + BX - 1) / BX, (g.yd + BY - 1) / BY), block(BX, BY);
-    hipLaunchKernelGGL(div_kernel, grid, block, 0, s, m, rho, g.xd, g.yd, g.zd, g.dx, g.dy, g.dz);
+    switch (g.zd) {
+    case 3: hipLaunchKernelGGL(div_kernel<3>, grid, block, 0, s, m, rho, g.xd, g.yd, g.zd, g.dx, g.dy, g.dz); break;
+    case 4: hipLaunchKernelGGL(div_kernel<4>, grid, block, 0, s, m, rho, g.xd, g.yd, g.zd, g.dx, g.dy, g.dz); break;
+    default: hipLaunchKernelGGL(div_kernel<0>, grid, block, 0, s, m, rho, g.xd, g.yd, g.zd, g.dx, g.dy, g.dz);
+    }
     MMS_HIP(hipGetLastError());
     return 0;
 }
@@ -340,7 +393,11 @@ int launch_stage_ex(const Grid &g, const Material &mat, int stage, double dt, co
     A.h_given = ex.h_given; A.h_out = ex.h_out; A.k_out = ex.k_out; A.slope_out = ex.slope_out;
     A.normalize = ex.normalize; A.mn_out = ex.mn_out;
     dim3 grid((g.xd + BX - 1) / BX, (g.yd + BY - 1) / BY), block(BX, BY);
-    hipLaunchKernelGGL(stage_kernel, grid, block, 0, s, A);
+    switch (g.zd) {
+    case 3: hipLaunchKernelGGL(stage_kernel<3>, grid, block, 0, s, A); break;
+    case 4: hipLaunchKernelGGL(stage_kernel<4>, grid, block, 0, s, A); break;
+    default: hipLaunchKernelGGL(stage_kernel<0>, grid, block, 0, s, A);
+    }
     MMS_HIP(hipGetLastError());
     return 0;
 }
@@ -359,6 +416,34 @@ int launch_hfield(const Grid &g, const Material &mat, const float *phi, const fl
     A.phi = phi; A.m_in = m; A.t_dev = t_dev; A.h_out = H_aos;
     dim3 grid((g.xd + BX - 1) / BX, (g.yd + BY - 1) / BY), block(BX, BY);
     hipLaunchKernelGGL(hfield_kernel, grid, block, 0, s, A);
+    MMS_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_div_batch(const float *m, float *rho, const int4 *tiles, int ntiles, const BatchFilm *films, long pstride,
+                     float dx, float dy, float dz, int zd, hipStream_t s, Profiler *prof) {
+    KTimer kt(prof, K_DIV);
+    if (ntiles <= 0) return 0;
+    if (zd == 3)
+        hipLaunchKernelGGL(div_batch_kernel<3>, dim3(ntiles), dim3(BX, BY), 0, s, m, rho, tiles, films, pstride, dx, dy, dz);
+    else
+        hipLaunchKernelGGL(div_batch_kernel<0>, dim3(ntiles), dim3(BX, BY), 0, s, m, rho, tiles, films, pstride, dx, dy, dz);
+    MMS_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_stage_batch(const Grid &g, const Material &mat, int stage, double dt, const float *phi, const float *m_in,
+                       float *m_out, double *mn, double *acc, const uint8_t *matl, const int4 *tiles, int ntiles,
+                       const BatchFilm *films, long pstride, hipStream_t s, Profiler *prof) {
+    KTimer kt(prof, K_STAGE);
+    if (ntiles <= 0) return 0;
+    StageArgs A = make_args(g, mat);
+    A.phi = phi; A.m_in = m_in; A.m_out = m_out; A.mn = mn; A.acc = acc; A.matl = matl;
+    A.t_dev = nullptr; A.stage = stage; A.dt = dt; A.external = 2;
+    if (g.zd == 3)
+        hipLaunchKernelGGL(stage_batch_kernel<3>, dim3(ntiles), dim3(BX, BY), 0, s, A, tiles, films, pstride);
+    else
+        hipLaunchKernelGGL(stage_batch_kernel<0>, dim3(ntiles), dim3(BX, BY), 0, s, A, tiles, films, pstride);
     MMS_HIP(hipGetLastError());
     return 0;
 }

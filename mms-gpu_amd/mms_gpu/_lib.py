@@ -20,6 +20,11 @@ i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 I, F, D, VP = C.c_int, C.c_float, C.c_double, C.c_void_p
 
 
+class FilmSpec(C.Structure):
+    """Mirror of ``mms_film_spec``."""
+    _fields_ = [("xdim", I), ("ydim", I), ("hext", F * 3)]
+
+
 class MmsParams(C.Structure):
     """Mirror of ``mms_params`` in include/mms_gpu.h."""
     _fields_ = [("xdim", I), ("ydim", I), ("zdim", I),
@@ -27,7 +32,7 @@ class MmsParams(C.Structure):
                 ("solver", I), ("P", I), ("m2l_precision", I), ("leaf_width", I),
                 ("mu_0", F), ("Ms", F), ("Aexch", F), ("alfa", F), ("gamma", F),
                 ("demag", I), ("exchange", I), ("external", I),
-                ("hext", F * 3)]
+                ("hext", F * 3), ("m2l_cols", I), ("direct_split", I)]
 
 
 # symbol -> (restype, argtypes); every symbol declared in include/mms_gpu.h
@@ -52,12 +57,24 @@ SIGNATURES = {
     "mms_potential_dev": (I, [VP, VP, VP]),
     "mms_potential": (I, [VP, f32p, f32p]),
     "mms_potential_probes_f64": (I, [VP, f32p, i64p, I, f64p]),
+    "mms_autotune": (I, [VP, I, C.c_char_p, I]),
+    "mms_get_shape": (I, [VP, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
     "mms_profile_begin": (I, [VP]),
     "mms_profile_end": (I, [VP]),
     "mms_profile_count": (I, [VP]),
     "mms_profile_name": (C.c_char_p, [VP, I]),
     "mms_profile_ms": (D, [VP, I]),
     "mms_profile_launches": (C.c_long, [VP, I]),
+    "mms_batch_create": (I, [C.POINTER(VP), C.POINTER(MmsParams), I, VP, VP]),
+    "mms_batch_destroy": (I, [VP]),
+    "mms_batch_set_m": (I, [VP, f32p]),
+    "mms_batch_get_m": (I, [VP, f32p]),
+    "mms_batch_steps": (I, [VP, I, F, I]),
+    "mms_batch_sync": (I, [VP]),
+    "mms_batch_info": (I, [VP, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(C.c_long), C.POINTER(I)]),
+    "mms_batch_autotune": (I, [VP, I, C.c_char_p, I]),
+    "mms_batch_profile_begin": (I, [VP]),
+    "mms_batch_profile_end": (I, [VP, f64p, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")]),
     "mms_last_error": (C.c_char_p, []),
     "mms_version": (C.c_char_p, []),
 }

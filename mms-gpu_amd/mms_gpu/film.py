@@ -22,7 +22,7 @@ class Film:
     def __init__(self, xdim, ydim, zdim, dx, dy, dz, *, solver=SOLVER_DIRECT, P=6,
                  m2l_precision=M2L_FP32, leaf_width=4, mu_0=MU_0, Ms=8.6e5, Aexch=1.3e-11,
                  alfa=0.5, gamma=2.211e5, demag=1, exchange=1, external=EXT_NONE,
-                 hext=(0.0, 0.0, 0.0), material=None):
+                 hext=(0.0, 0.0, 0.0), material=None, m2l_cols=4, direct_split=1):
         p = MmsParams()
         p.xdim, p.ydim, p.zdim = xdim, ydim, zdim
         p.dx, p.dy, p.dz = dx, dy, dz
@@ -31,6 +31,7 @@ class Film:
         p.demag, p.exchange, p.external = demag, exchange, external
         for i in range(3):
             p.hext[i] = hext[i]
+        p.m2l_cols, p.direct_split = m2l_cols, direct_split
         self.params = p
         self.dims = (xdim, ydim, zdim)
         self.n = xdim * ydim * zdim
@@ -104,6 +105,19 @@ class Film:
         check(lib().mms_potential_probes_f64(self._ctx, charge, probes, len(probes), out),
               "mms_potential_probes_f64")
         return out
+
+    # ------------------------------------------------------- autotuner --
+    def autotune(self, reps=3):
+        """Time the solver's launch shapes on the current state's charge and
+        keep the fastest; returns the per-candidate report."""
+        buf = C.create_string_buffer(4096)
+        check(lib().mms_autotune(self._ctx, int(reps), buf, 4096), "mms_autotune")
+        return buf.value.decode()
+
+    def shape(self):
+        w, c, s = C.c_int(), C.c_int(), C.c_int()
+        check(lib().mms_get_shape(self._ctx, C.byref(w), C.byref(c), C.byref(s)), "mms_get_shape")
+        return {"leaf_width": w.value, "m2l_cols": c.value, "direct_split": s.value}
 
     # -------------------------------------------------------- profiling --
     def profile_begin(self):

@@ -191,3 +191,43 @@ def layered_single_domain(mask2, zdim):
                   np.cos(np.float64(theta))], np.float32)
     M[mat == 1] = (MS * v).astype(np.float32)
     return mat.ravel(), M.ravel()
+
+
+# ----------------------------------------------------------- film sweep --
+def film_mask(kind, side):
+    """In-plane mask of one sweep film: 'disc' (cells within side/2 of the
+    centre), 'square' (every cell) or 'U' (square minus a notch of width
+    side/3 and depth side/2 from the top edge)."""
+    y, x = np.mgrid[0:side, 0:side]
+    if kind == "disc":
+        c = side / 2.0
+        return (((x + 0.5 - c) ** 2 + (y + 0.5 - c) ** 2) <= c * c).astype(np.uint8)
+    m = np.ones((side, side), np.uint8)
+    if kind == "U":
+        w = max(1, side // 3)
+        x0 = (side - w) // 2
+        m[side - side // 2:, x0:x0 + w] = 0
+    return m
+
+
+def c5_films(seed=SEED, nfilm=512, smin=16, smax=96, zd=3):
+    """Hysteresis-style sweep: nfilm independent films, side ~ U{smin..smax},
+    masks uniform over {disc, square, U}, random unit m on magnetic cells,
+    film f in a uniform field (-50 + 100 f/(nfilm-1)) mT / mu0 along x;
+    direct sum below 4096 cells, FMM P=4 above; dt 1e-13 x 2000."""
+    u = uniform(seed, 6, 2 * nfilm).reshape(nfilm, 2)
+    films = []
+    kinds = ("disc", "square", "U")
+    for f in range(nfilm):
+        side = smin + int(u[f, 0] * (smax - smin + 1))
+        kind = kinds[int(u[f, 1] * 3)]
+        mask = film_mask(kind, side)
+        mat = np.broadcast_to(mask, (zd, side, side)).ravel().copy()
+        n = side * side * zd
+        m = random_unit(seed, 1000 + f, n)
+        M = _finish(mat, m)
+        hx = float(np.float32((-50 + 100 * f / max(1, nfilm - 1)) * 1e-3) / MU_0)
+        solver = "direct" if n < 4096 else "fmm"
+        films.append(Workload(f"film{f}_{kind}_{side}x{side}x{zd}", side, side, zd, mat, M, dt=1e-13,
+                              steps=2000, hext=(hx, 0.0, 0.0), solver=solver, P=4 if solver == "fmm" else 0))
+    return films

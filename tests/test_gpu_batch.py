@@ -1,0 +1,72 @@
+"""Batched many-film stepping against the same films stepped one by one."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+F = np.float32
+
+
+def _say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+def _films():
+    from mms_gpu import workloads as wl
+    out = []
+    for k, (side, kind) in enumerate([(20, "disc"), (36, "U"), (40, "square"), (50, "U"), (70, "disc"),
+                                       (90, "square")]):
+        mask = wl.film_mask(kind, side)
+        mat = np.broadcast_to(mask, (3, side, side)).ravel().copy()
+        M = wl._finish(mat, wl.random_unit(77, 100 + k, side * side * 3))
+        hx = float(F((-50 + 20 * k) * 1e-3) / wl.MU_0)
+        out.append(wl.Workload(f"f{k}", side, side, 3, mat, M, dt=1e-13, steps=10, hext=(hx, 0.0, 0.0),
+                               solver="direct" if side * side * 3 < 4096 else "fmm", P=4))
+    return out
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_matches_single_films(mg, use_graph):
+    from mms_gpu import workloads as wl
+    films = _films()
+    mat = np.concatenate([f.material for f in films])
+    M0 = np.concatenate([f.M for f in films])
+    kw = dict(mu_0=wl.MU_0, Ms=wl.MS, Aexch=wl.AEXCH, alfa=wl.ALFA, gamma=wl.GAMMA)
+    _say("batch of", len(films), "films, graph", use_graph)
+    with mg.FilmBatch(films, 3, *films[0].d, solver=mg.SOLVER_FMM, P=4, material=mat, **kw) as b:
+        info = b.info()
+        _say(info)
+        assert info["direct_films"] == 2 and info["fmm_films"] == 4 and info["fmm_size_classes"] == 2
+        b.set_m(M0)
+        b.steps(10, F(1e-13), use_graph=use_graph)
+        Mb = b.get_m()
+        offsets = b.offsets
+    worst = 0.0
+    for i, f in enumerate(films):
+        solver = mg.SOLVER_DIRECT if f.solver == "direct" else mg.SOLVER_FMM
+        with mg.Film(*f.dims, *f.d, solver=solver, P=4, external=mg.EXT_CONST, hext=f.hext,
+                     material=f.material, **kw) as s:
+            s.set_m(f.M)
+            s.steps(10, F(1e-13), use_graph=False)
+            Ms = s.get_m()
+        d = np.abs(Mb[3 * offsets[i]:3 * offsets[i + 1]] - Ms).max() / float(wl.MS)
+        worst = max(worst, d)
+        _say(f"  film {f.name} {f.dims} {f.solver}: max |dm| {d:.2e}")
+    assert worst <= 1e-5
+
+
+def test_batch_autotune_and_sweep_shape(mg):
+    """The 512-film sweep builds, autotunes and steps; magnetisation stays unit."""
+    from mms_gpu import workloads as wl
+    films = wl.c5_films(nfilm=64)
+    mat = np.concatenate([f.material for f in films])
+    M0 = np.concatenate([f.M for f in films])
+    with mg.FilmBatch(films, 3, *films[0].d, solver=mg.SOLVER_FMM, P=4, material=mat, mu_0=wl.MU_0, Ms=wl.MS,
+                      Aexch=wl.AEXCH, alfa=wl.ALFA, gamma=wl.GAMMA) as b:
+        b.set_m(M0)
+        rep = b.autotune(reps=2)
+        _say(rep)
+        b.steps(5, F(1e-13))
+        M = b.get_m().reshape(-1, 3)
+    mag = np.linalg.norm(M[mat == 1].astype(float), axis=1) / float(wl.MS)
+    assert np.abs(mag - 1).max() < 1e-6
+    assert np.all(M[mat == 0] == 0)

@@ -192,3 +192,28 @@ def test_fmm_m2l_precision_modes(mg, mode, tol):
     err = _rel(pot, ref)
     _say(f"m2l {mode}: rel err vs FP64 FMM {err:.2e}")
     assert err < tol
+
+
+@pytest.mark.parametrize("solver", ["fmm", "direct"])
+def test_autotune_keeps_results(mg, solver):
+    """The autotuner only changes launch shapes: results before and after agree
+    to FP32 rounding of the reordered sums (FMM: same P, different leaf width
+    changes the truncation error slightly, bounded by the P=6 accuracy)."""
+    from mms_gpu import workloads as wl
+    side = 128 if solver == "fmm" else 24
+    w = wl.c3_domains(side=side, zd=4, domain=16)
+    kw = dict(solver=mg.SOLVER_FMM, P=6) if solver == "fmm" else dict(solver=mg.SOLVER_DIRECT)
+    with mg.Film(*w.dims, *w.d, material=w.material, **kw) as f:
+        f.set_m(w.M)
+        before = f.hfield()
+        rep = f.autotune(reps=2)
+        _say(rep)
+        shape = f.shape()
+        after = f.hfield()
+    assert "chosen" in rep
+    if solver == "fmm":
+        assert shape["leaf_width"] in (4, 8) and shape["m2l_cols"] in (4, 8)
+        assert np.abs(after - before).max() <= 1e-3 * np.abs(before).max()
+    else:
+        assert shape["direct_split"] in (1, 2, 4)
+        assert np.abs(after - before).max() <= 1e-6 * np.abs(before).max()

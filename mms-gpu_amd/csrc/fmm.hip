@@ -151,6 +151,16 @@ __global__ __launch_bounds__(256) void m2m_mfma_kernel(const float *__restrict__
 // target layer; wave w handles child parity (w & 1, w >> 1): an SX x 8
 // sub-lattice (SX = TX / 2) of targets in NC = SX / 2 MFMA column tiles of 16.
 // The source window holds the tile plus a 3-box halo: (TX + 6) x 22 boxes.
+// Interaction-list offsets (ox + 3) | (oy + 3) << 4 of child parity p =
+// px + 2 py: the 6 x 6 block [-2-px, 3-px] x [-2-py, 3-py] of source boxes
+// minus the 3 x 3 neighbourhood (children of the parent's neighbours minus
+// own neighbours, Box.cu:121-145).
+__constant__ unsigned char c_m2l_offs[4][27] = {
+    {17, 18, 19, 20, 21, 22, 33, 37, 38, 49, 53, 54, 65, 69, 70, 81, 82, 83, 84, 85, 86, 97, 98, 99, 100, 101, 102},
+    {16, 17, 18, 19, 20, 21, 32, 33, 37, 48, 49, 53, 64, 65, 69, 80, 81, 82, 83, 84, 85, 96, 97, 98, 99, 100, 101},
+    {1, 2, 3, 4, 5, 6, 17, 18, 19, 20, 21, 22, 33, 37, 38, 49, 53, 54, 65, 69, 70, 81, 82, 83, 84, 85, 86},
+    {0, 1, 2, 3, 4, 5, 16, 17, 18, 19, 20, 21, 32, 33, 37, 48, 49, 53, 64, 65, 69, 80, 81, 82, 83, 84, 85}};
+
 template <int KP, int MODE, int NC> struct M2LCfg {
     static constexpr int G = KP / 4;    // slots per lane group (lanes 16g..16g+15)
     static constexpr int RT = KP / 16;  // 16-row output tiles
@@ -226,75 +236,101 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(const float *__restrict__
         }
         __syncthreads();
         const int dzi = zt - zs + zd - 1;
+        const unsigned char *offs = c_m2l_offs[__builtin_amdgcn_readfirstlane(wave)];
+        if constexpr (MODE == MMS_M2L_FP32) {
+            const float *T = reinterpret_cast<const float *>(Tfrag);
+            const float *lds = reinterpret_cast<const float *>(smem);
+            f32x4 acur[RT][G / 4], anxt[RT][G / 4];
+            auto loadA = [&](f32x4 (&a)[RT][G / 4], int code) {
+                const long fbase = ((long)dzi * 49 + (code >> 4) * 7 + (code & 15)) * RT;
+#pragma unroll
+                for (int r = 0; r < RT; r++)
+#pragma unroll
+                    for (int u4 = 0; u4 < G / 4; u4++)
+                        a[r][u4] = *reinterpret_cast<const f32x4 *>(T + ((fbase + r) * 64 + lane) * G + 4 * u4);
+            };
+            loadA(acur, offs[0]);
 #pragma unroll 1
-        for (int oy = -2 - py; oy <= 3 - py; oy++) {
-#pragma unroll 1
-            for (int ox = -2 - px; ox <= 3 - px; ox++) {
-                if (ox >= -1 && ox <= 1 && oy >= -1 && oy <= 1) continue;
-                const long fbase = ((long)dzi * 49 + (oy + 3) * 7 + (ox + 3)) * RT;
-                const int woff = oy * WX + ox;
-                if constexpr (MODE == MMS_M2L_FP32) {
-                    const float *T = reinterpret_cast<const float *>(Tfrag);
-                    const float *lds = reinterpret_cast<const float *>(smem);
-                    // k-slot groups of 4: A and B for 4 slots, then all RT x NC tiles
+            for (int oi = 0; oi < 27; oi++) {
+                const int code = offs[oi];
+                if (oi < 26) loadA(anxt, offs[oi + 1]);  // prefetch the next offset's operator
+                const int woff = ((code >> 4) - 3) * WX + ((code & 15) - 3);
 #pragma unroll
-                    for (int u4 = 0; u4 < G / 4; u4++) {
-                        f32x4 a[RT], b[NC];
+                for (int u4 = 0; u4 < G / 4; u4++) {
+                    f32x4 b[NC];
 #pragma unroll
-                        for (int r = 0; r < RT; r++)
-                            a[r] = *reinterpret_cast<const f32x4 *>(T + ((fbase + r) * 64 + lane) * G + 4 * u4);
+                    for (int c = 0; c < NC; c++)
+                        b[c] = *reinterpret_cast<const f32x4 *>(lds + (tbox[c] + woff) * SB + g * G + 4 * u4);
 #pragma unroll
-                        for (int c = 0; c < NC; c++)
-                            b[c] = *reinterpret_cast<const f32x4 *>(lds + (tbox[c] + woff) * SB + g * G + 4 * u4);
-#pragma unroll
-                        for (int u = 0; u < 4; u++)
-#pragma unroll
-                            for (int r = 0; r < RT; r++)
-#pragma unroll
-                                for (int c = 0; c < NC; c++)
-                                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r][u], b[c][u], acc[r][c], 0, 0, 0);
-                    }
-                } else {
-                    const unsigned short *T = reinterpret_cast<const unsigned short *>(Tfrag);
-                    const unsigned short *hi = reinterpret_cast<const unsigned short *>(smem);
-                    constexpr long LO = (long)RT * 64 * G;  // lo plane offset per (dz, offset) block count below
-#pragma unroll
-                    for (int s = 0; s < G / 8; s++) {  // 32-wide k steps
-                        bf16x8 ah[RT], bh[NC];
-#pragma unroll
-                        for (int r = 0; r < RT; r++)
-                            ah[r] = *reinterpret_cast<const bf16x8 *>(T + ((fbase + r) * 64 + lane) * G + 8 * s);
-#pragma unroll
-                        for (int c = 0; c < NC; c++)
-                            bh[c] = *reinterpret_cast<const bf16x8 *>(hi + (tbox[c] + woff) * SB + g * G + 8 * s);
+                    for (int u = 0; u < 4; u++)
 #pragma unroll
                         for (int r = 0; r < RT; r++)
 #pragma unroll
                             for (int c = 0; c < NC; c++)
-                                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[r], bh[c], acc[r][c], 0, 0, 0);
-                        if constexpr (MODE == MMS_M2L_BF16X3) {
-                            const long lo_base = LO * 49 * (2 * zd - 1);
-                            bf16x8 al[RT], bl[NC];
+                                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(acur[r][u4][u], b[c][u], acc[r][c], 0, 0, 0);
+                }
 #pragma unroll
-                            for (int r = 0; r < RT; r++)
-                                al[r] = *reinterpret_cast<const bf16x8 *>(T + lo_base + ((fbase + r) * 64 + lane) * G + 8 * s);
+                for (int r = 0; r < RT; r++)
+#pragma unroll
+                    for (int u4 = 0; u4 < G / 4; u4++) acur[r][u4] = anxt[r][u4];
+            }
+        } else {
+            const unsigned short *T = reinterpret_cast<const unsigned short *>(Tfrag);
+            const unsigned short *hi = reinterpret_cast<const unsigned short *>(smem);
+            constexpr int NK = G / 8;  // 32-wide k steps
+            constexpr int NP = (MODE == MMS_M2L_BF16X3) ? 2 : 1;
+            const long lo_base = (long)RT * 64 * G * 49 * (2 * zd - 1);
+            bf16x8 acur[NP][RT][NK], anxt[NP][RT][NK];
+            auto loadA = [&](bf16x8 (&a)[NP][RT][NK], int code) {
+                const long fbase = ((long)dzi * 49 + (code >> 4) * 7 + (code & 15)) * RT;
+#pragma unroll
+                for (int q = 0; q < NP; q++)
+#pragma unroll
+                    for (int r = 0; r < RT; r++)
+#pragma unroll
+                        for (int sk = 0; sk < NK; sk++)
+                            a[q][r][sk] = *reinterpret_cast<const bf16x8 *>(T + q * lo_base + ((fbase + r) * 64 + lane) * G + 8 * sk);
+            };
+            loadA(acur, offs[0]);
+#pragma unroll 1
+            for (int oi = 0; oi < 27; oi++) {
+                const int code = offs[oi];
+                if (oi < 26) loadA(anxt, offs[oi + 1]);
+                const int woff = ((code >> 4) - 3) * WX + ((code & 15) - 3);
+#pragma unroll
+                for (int sk = 0; sk < NK; sk++) {
+                    bf16x8 bh[NC];
+#pragma unroll
+                    for (int c = 0; c < NC; c++)
+                        bh[c] = *reinterpret_cast<const bf16x8 *>(hi + (tbox[c] + woff) * SB + g * G + 8 * sk);
+#pragma unroll
+                    for (int r = 0; r < RT; r++)
+#pragma unroll
+                        for (int c = 0; c < NC; c++)
+                            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[0][r][sk], bh[c], acc[r][c], 0, 0, 0);
+                    if constexpr (MODE == MMS_M2L_BF16X3) {
+                        bf16x8 bl[NC];
+#pragma unroll
+                        for (int c = 0; c < NC; c++)
+                            bl[c] = *reinterpret_cast<const bf16x8 *>(hi + WX * WY * SB + (tbox[c] + woff) * SB + g * G + 8 * sk);
+#pragma unroll
+                        for (int r = 0; r < RT; r++)
 #pragma unroll
                             for (int c = 0; c < NC; c++)
-                                bl[c] = *reinterpret_cast<const bf16x8 *>(hi + WX * WY * SB + (tbox[c] + woff) * SB +
-                                                                          g * G + 8 * s);
+                                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[0][r][sk], bl[c], acc[r][c], 0, 0, 0);
 #pragma unroll
-                            for (int r = 0; r < RT; r++)
+                        for (int r = 0; r < RT; r++)
 #pragma unroll
-                                for (int c = 0; c < NC; c++)
-                                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[r], bl[c], acc[r][c], 0, 0, 0);
-#pragma unroll
-                            for (int r = 0; r < RT; r++)
-#pragma unroll
-                                for (int c = 0; c < NC; c++)
-                                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[r], bh[c], acc[r][c], 0, 0, 0);
-                        }
+                            for (int c = 0; c < NC; c++)
+                                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[NP - 1][r][sk], bh[c], acc[r][c], 0, 0, 0);
                     }
                 }
+#pragma unroll
+                for (int q = 0; q < NP; q++)
+#pragma unroll
+                    for (int r = 0; r < RT; r++)
+#pragma unroll
+                        for (int sk = 0; sk < NK; sk++) acur[q][r][sk] = anxt[q][r][sk];
             }
         }
     }

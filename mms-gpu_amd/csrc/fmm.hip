@@ -34,6 +34,7 @@
 // L2L is always FP32.
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "mms_internal.hpp"
 
@@ -173,24 +174,39 @@ template <int KP, int MODE, int NC> struct M2LCfg {
         (MODE == MMS_M2L_FP32) ? sizeof(float) * WX * WY * SB : sizeof(unsigned short) * NW * WX * WY * SB;
 };
 
+// Levels of one M2L launch: the flat tile index blockIdx.x runs over the
+// levels' tiles back to back.
+struct M2LLevels {
+    static constexpr int kMax = 16;
+    int n;                   // levels in the launch
+    int tile0[kMax + 1];     // first flat tile of level i; tile0[n] = total
+    int ntx[kMax];           // tiles along x
+    int nb[kMax];            // boxes per side
+    long off[kMax];          // float offset of the level in mult / loc / partial sums (per film)
+    long toff[kMax];         // byte offset of the level's operator fragments
+};
+
+// One workgroup = one dense tile of target boxes of one level, one target
+// layer zt and one source layer zs; the result is the zs partial of the
+// local expansions (summed over zs, with L2L from the parent, by
+// l2l_sum_kernel).
 template <int KP, int MODE, int NC>
-__global__ __launch_bounds__(256) void m2l_mfma_kernel(const float *__restrict__ M, float *__restrict__ L,
-                                                       const float *__restrict__ Lpar,
-                                                       const void *__restrict__ Tfrag,
-                                                       const float *__restrict__ l2lfrag, int nb, int zd,
-                                                       long film_stride) {
+__global__ __launch_bounds__(256) void m2l_mfma_kernel(const float *__restrict__ Mall, float *__restrict__ Part,
+                                                       const char *__restrict__ Tall, M2LLevels lv, int zd,
+                                                       long film_stride, long part_stride) {
     typedef M2LCfg<KP, MODE, NC> C;
-    {
-        const long fo = (long)(blockIdx.z / zd) * film_stride;
-        M += fo;
-        L += fo;
-        if (Lpar) Lpar += fo;
-    }
+    int li = 0;
+    while (li + 1 < lv.n && (int)blockIdx.x >= lv.tile0[li + 1]) li++;
+    const int tl = blockIdx.x - lv.tile0[li], nb = lv.nb[li];
+    const int zz = blockIdx.z, film = zz / (zd * zd), zt = (zz / zd) % zd, zs = zz % zd;
+    const float *M = Mall + film * film_stride + lv.off[li];
+    float *Lz = Part + zs * part_stride + film * film_stride + lv.off[li];
+    const char *Tfrag = Tall + lv.toff[li];
     constexpr int G = C::G, RT = C::RT, SB = C::SB, SX = C::SX, WX = C::WX, WY = C::WY;
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int px = wave & 1, py = wave >> 1;
-    const int x0 = blockIdx.x * C::TX, y0 = blockIdx.y * 16, zt = blockIdx.z % zd;
+    const int x0 = (tl % lv.ntx[li]) * C::TX, y0 = (tl / lv.ntx[li]) * 16;
     const int j16 = lane & 15, g = lane >> 4;
     // window offset of this lane's target in column tile c: target t = 16c + j16
     int tbox[NC];
@@ -206,10 +222,35 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(const float *__restrict__
 #pragma unroll
         for (int c = 0; c < NC; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-#pragma unroll 1
-    for (int zs = 0; zs < zd; zs++) {
-        __syncthreads();
-        // stage the source window of layer zs (zero outside the grid)
+    const int dzi = zt - zs + zd - 1;
+    const unsigned char *offs = c_m2l_offs[__builtin_amdgcn_readfirstlane(wave)];
+    // operator fragments of the first three offsets, in flight while the window is staged
+    constexpr int NP = (MODE == MMS_M2L_BF16X3) ? 2 : 1;
+    constexpr int NK = (MODE == MMS_M2L_FP32) ? G / 4 : G / 8;  // 16-byte A loads per row tile and plane
+    typedef typename std::conditional<MODE == MMS_M2L_FP32, f32x4, bf16x8>::type AV;
+    AV abuf[3][NP][RT][NK];
+    const long lo_base = (long)RT * 64 * G * 49 * (2 * zd - 1);  // bf16 lo plane offset (elements)
+    auto loadA = [&](AV (&a)[NP][RT][NK], int code) {
+        const long fbase = ((long)dzi * 49 + (code >> 4) * 7 + (code & 15)) * RT;
+#pragma unroll
+        for (int q = 0; q < NP; q++)
+#pragma unroll
+            for (int r = 0; r < RT; r++)
+#pragma unroll
+                for (int sk = 0; sk < NK; sk++) {
+                    if constexpr (MODE == MMS_M2L_FP32)
+                        a[q][r][sk] = *reinterpret_cast<const f32x4 *>(
+                            reinterpret_cast<const float *>(Tfrag) + ((fbase + r) * 64 + lane) * G + 4 * sk);
+                    else
+                        a[q][r][sk] = *reinterpret_cast<const bf16x8 *>(
+                            reinterpret_cast<const unsigned short *>(Tfrag) + q * lo_base + ((fbase + r) * 64 + lane) * G + 8 * sk);
+                }
+    };
+#pragma unroll
+    for (int k = 0; k < 3; k++) loadA(abuf[k], offs[k]);
+
+    // stage the source window of layer zs (zero outside the grid)
+    {
         const f32x4 *M4 = reinterpret_cast<const f32x4 *>(M);
         for (int e = threadIdx.x; e < WX * WY * (KP / 4); e += 256) {
             const int box = e / (KP / 4), q = e % (KP / 4);
@@ -226,37 +267,28 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(const float *__restrict__
                     h[t] = f2bf(v[t]);
                     l[t] = f2bf(v[t] - bf2f(h[t]));
                 }
-                uint2 hv, lv;
+                uint2 hv, lv2;
                 hv.x = h[0] | ((unsigned)h[1] << 16); hv.y = h[2] | ((unsigned)h[3] << 16);
-                lv.x = l[0] | ((unsigned)l[1] << 16); lv.y = l[2] | ((unsigned)l[3] << 16);
+                lv2.x = l[0] | ((unsigned)l[1] << 16); lv2.y = l[2] | ((unsigned)l[3] << 16);
                 *reinterpret_cast<uint2 *>(hi + box * SB + 4 * q) = hv;
                 if constexpr (MODE == MMS_M2L_BF16X3)
-                    *reinterpret_cast<uint2 *>(hi + WX * WY * SB + box * SB + 4 * q) = lv;
+                    *reinterpret_cast<uint2 *>(hi + WX * WY * SB + box * SB + 4 * q) = lv2;
             }
         }
-        __syncthreads();
-        const int dzi = zt - zs + zd - 1;
-        const unsigned char *offs = c_m2l_offs[__builtin_amdgcn_readfirstlane(wave)];
-        if constexpr (MODE == MMS_M2L_FP32) {
-            const float *T = reinterpret_cast<const float *>(Tfrag);
-            const float *lds = reinterpret_cast<const float *>(smem);
-            f32x4 acur[RT][G / 4], anxt[RT][G / 4];
-            auto loadA = [&](f32x4 (&a)[RT][G / 4], int code) {
-                const long fbase = ((long)dzi * 49 + (code >> 4) * 7 + (code & 15)) * RT;
-#pragma unroll
-                for (int r = 0; r < RT; r++)
-#pragma unroll
-                    for (int u4 = 0; u4 < G / 4; u4++)
-                        a[r][u4] = *reinterpret_cast<const f32x4 *>(T + ((fbase + r) * 64 + lane) * G + 4 * u4);
-            };
-            loadA(acur, offs[0]);
+    }
+    __syncthreads();
+
+    // 27 offsets, three operator blocks in flight (ring of registers)
 #pragma unroll 1
-            for (int oi = 0; oi < 27; oi++) {
-                const int code = offs[oi];
-                if (oi < 26) loadA(anxt, offs[oi + 1]);  // prefetch the next offset's operator
-                const int woff = ((code >> 4) - 3) * WX + ((code & 15) - 3);
+    for (int ob = 0; ob < 27; ob += 3) {
 #pragma unroll
-                for (int u4 = 0; u4 < G / 4; u4++) {
+        for (int k = 0; k < 3; k++) {
+            const int code = offs[ob + k];
+            const int woff = ((code >> 4) - 3) * WX + ((code & 15) - 3);
+            if constexpr (MODE == MMS_M2L_FP32) {
+                const float *lds = reinterpret_cast<const float *>(smem);
+#pragma unroll
+                for (int u4 = 0; u4 < NK; u4++) {
                     f32x4 b[NC];
 #pragma unroll
                     for (int c = 0; c < NC; c++)
@@ -267,36 +299,10 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(const float *__restrict__
                         for (int r = 0; r < RT; r++)
 #pragma unroll
                             for (int c = 0; c < NC; c++)
-                                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(acur[r][u4][u], b[c][u], acc[r][c], 0, 0, 0);
+                                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(abuf[k][0][r][u4][u], b[c][u], acc[r][c], 0, 0, 0);
                 }
-#pragma unroll
-                for (int r = 0; r < RT; r++)
-#pragma unroll
-                    for (int u4 = 0; u4 < G / 4; u4++) acur[r][u4] = anxt[r][u4];
-            }
-        } else {
-            const unsigned short *T = reinterpret_cast<const unsigned short *>(Tfrag);
-            const unsigned short *hi = reinterpret_cast<const unsigned short *>(smem);
-            constexpr int NK = G / 8;  // 32-wide k steps
-            constexpr int NP = (MODE == MMS_M2L_BF16X3) ? 2 : 1;
-            const long lo_base = (long)RT * 64 * G * 49 * (2 * zd - 1);
-            bf16x8 acur[NP][RT][NK], anxt[NP][RT][NK];
-            auto loadA = [&](bf16x8 (&a)[NP][RT][NK], int code) {
-                const long fbase = ((long)dzi * 49 + (code >> 4) * 7 + (code & 15)) * RT;
-#pragma unroll
-                for (int q = 0; q < NP; q++)
-#pragma unroll
-                    for (int r = 0; r < RT; r++)
-#pragma unroll
-                        for (int sk = 0; sk < NK; sk++)
-                            a[q][r][sk] = *reinterpret_cast<const bf16x8 *>(T + q * lo_base + ((fbase + r) * 64 + lane) * G + 8 * sk);
-            };
-            loadA(acur, offs[0]);
-#pragma unroll 1
-            for (int oi = 0; oi < 27; oi++) {
-                const int code = offs[oi];
-                if (oi < 26) loadA(anxt, offs[oi + 1]);
-                const int woff = ((code >> 4) - 3) * WX + ((code & 15) - 3);
+            } else {
+                const unsigned short *hi = reinterpret_cast<const unsigned short *>(smem);
 #pragma unroll
                 for (int sk = 0; sk < NK; sk++) {
                     bf16x8 bh[NC];
@@ -307,7 +313,7 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(const float *__restrict__
                     for (int r = 0; r < RT; r++)
 #pragma unroll
                         for (int c = 0; c < NC; c++)
-                            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[0][r][sk], bh[c], acc[r][c], 0, 0, 0);
+                            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(abuf[k][0][r][sk], bh[c], acc[r][c], 0, 0, 0);
                     if constexpr (MODE == MMS_M2L_BF16X3) {
                         bf16x8 bl[NC];
 #pragma unroll
@@ -317,25 +323,52 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(const float *__restrict__
                         for (int r = 0; r < RT; r++)
 #pragma unroll
                             for (int c = 0; c < NC; c++)
-                                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[0][r][sk], bl[c], acc[r][c], 0, 0, 0);
+                                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(abuf[k][0][r][sk], bl[c], acc[r][c], 0, 0, 0);
 #pragma unroll
                         for (int r = 0; r < RT; r++)
 #pragma unroll
                             for (int c = 0; c < NC; c++)
-                                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(acur[NP - 1][r][sk], bh[c], acc[r][c], 0, 0, 0);
+                                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(abuf[k][NP - 1][r][sk], bh[c], acc[r][c], 0, 0, 0);
                     }
                 }
-#pragma unroll
-                for (int q = 0; q < NP; q++)
-#pragma unroll
-                    for (int r = 0; r < RT; r++)
-#pragma unroll
-                        for (int sk = 0; sk < NK; sk++) acur[q][r][sk] = anxt[q][r][sk];
             }
+            if (ob + 3 < 27) loadA(abuf[k], offs[ob + 3 + k]);
         }
     }
-    const int half = nb >> 1;
-    if (Lpar) {  // L2L from the parent (same layer, one level coarser), FP32 MFMA
+    // D tile row 4g + q (q = 0..3), column j16: slots 16r + 4g .. +3 of target j16
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const int t = 16 * c + j16;
+        const int bx = x0 + 2 * (t % SX) + px, by = y0 + 2 * (t / SX) + py;
+        if (bx >= nb || by >= nb) continue;
+        f32x4 *dst = reinterpret_cast<f32x4 *>(Lz + (((long)zt * nb + by) * nb + bx) * KP);
+#pragma unroll
+        for (int r = 0; r < RT; r++) dst[4 * r + g] = acc[r][c];
+    }
+}
+
+// Local expansions of one level: the source-layer partial sums of M2L plus
+// L2L from the parent level (same layer), FP32 MFMA.  Wave w = child parity;
+// 64 consecutive parents of one row per workgroup, 16 per column tile.
+template <int KP>
+__global__ __launch_bounds__(256) void l2l_sum_kernel(const float *__restrict__ Part, long part_stride,
+                                                      float *__restrict__ Lall, long off, long off_par,
+                                                      const float *__restrict__ l2lfrag, int nb, int zd,
+                                                      long film_stride) {
+    constexpr int G = KP / 4, RT = KP / 16, NC = 4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int px = wave & 1, py = wave >> 1, j16 = lane & 15, g = lane >> 4;
+    const int film = blockIdx.z / zd, z = blockIdx.z % zd, half = nb >> 1;
+    const int pby = blockIdx.y, pbx0 = blockIdx.x * 64;
+    float *L = Lall + film * film_stride + off;
+    const float *Pz = Part + film * film_stride + off;
+    f32x4 acc[RT][NC];
+#pragma unroll
+    for (int r = 0; r < RT; r++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if (off_par >= 0) {
+        const float *Lp = Lall + film * film_stride + off_par;
 #pragma unroll
         for (int u4 = 0; u4 < G / 4; u4++) {
             f32x4 a[RT], b[NC];
@@ -344,12 +377,10 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(const float *__restrict__
                 a[r] = *reinterpret_cast<const f32x4 *>(l2lfrag + (((long)wave * RT + r) * 64 + lane) * G + 4 * u4);
 #pragma unroll
             for (int c = 0; c < NC; c++) {
-                const int t = 16 * c + j16;
-                const int bx = x0 + 2 * (t % SX) + px, by = y0 + 2 * (t / SX) + py;
+                const int pbx = pbx0 + 16 * c + j16;
                 b[c] = f32x4{0.f, 0.f, 0.f, 0.f};
-                if (bx < nb && by < nb)
-                    b[c] = *reinterpret_cast<const f32x4 *>(
-                        Lpar + (((long)zt * half + (by >> 1)) * half + (bx >> 1)) * KP + g * G + 4 * u4);
+                if (pbx < half)
+                    b[c] = *reinterpret_cast<const f32x4 *>(Lp + (((long)z * half + pby) * half + pbx) * KP + g * G + 4 * u4);
             }
 #pragma unroll
             for (int u = 0; u < 4; u++)
@@ -360,15 +391,17 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(const float *__restrict__
                         acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r][u], b[c][u], acc[r][c], 0, 0, 0);
         }
     }
-    // D tile row 4g + q (q = 0..3), column j16: slots 16r + 4g .. +3 of target j16
 #pragma unroll
     for (int c = 0; c < NC; c++) {
-        const int t = 16 * c + j16;
-        const int bx = x0 + 2 * (t % SX) + px, by = y0 + 2 * (t / SX) + py;
-        if (bx >= nb || by >= nb) continue;
-        f32x4 *dst = reinterpret_cast<f32x4 *>(L + (((long)zt * nb + by) * nb + bx) * KP);
+        const int pbx = pbx0 + 16 * c + j16;
+        if (pbx >= half) continue;
+        const long e = (((long)z * nb + 2 * pby + py) * nb + 2 * pbx + px) * KP;
 #pragma unroll
-        for (int r = 0; r < RT; r++) dst[4 * r + g] = acc[r][c];
+        for (int r = 0; r < RT; r++) {
+            f32x4 v = acc[r][c];
+            for (int s = 0; s < zd; s++) v += *reinterpret_cast<const f32x4 *>(Pz + s * part_stride + e + 4 * (4 * r + g));
+            *reinterpret_cast<f32x4 *>(L + e + 4 * (4 * r + g)) = v;
+        }
     }
 }
 
@@ -530,8 +563,7 @@ __global__ __launch_bounds__(256) void p2p_l2p_mfma_kernel(const float *__restri
 
 // -------------------------------------------------------------- dispatch --
 template <int KP, int MODE, int NC>
-int launch_m2l(dim3 block_grid_unused, const FmmPlan &pl, int L, hipStream_t s) {
-    (void)block_grid_unused;
+int launch_m2l(const FmmPlan &pl, hipStream_t s) {
     typedef M2LCfg<KP, MODE, NC> C;
     static int attr_done = 0;
     if (!attr_done) {
@@ -539,25 +571,48 @@ int launch_m2l(dim3 block_grid_unused, const FmmPlan &pl, int L, hipStream_t s) 
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::lds_bytes));
         attr_done = 1;
     }
-    const int nb = 1 << L, zd = pl.zd;
-    dim3 grid((nb + C::TX - 1) / C::TX, (nb + 15) / 16, zd * pl.nfilm);
-    const float *Lpar = (L > 2) ? pl.loc + pl.lvl_off[L - 1] : nullptr;
-    const char *T = reinterpret_cast<const char *>(pl.d_m2l) + (size_t)(L - 2) * pl.m2l_level_bytes;
-    hipLaunchKernelGGL((m2l_mfma_kernel<KP, MODE, NC>), grid, dim3(256), (C::lds_bytes), s, pl.mult + pl.lvl_off[L],
-                       pl.loc + pl.lvl_off[L], Lpar, (const void *)T, pl.d_l2l, nb, zd, pl.film_stride);
+    M2LLevels lv;
+    lv.n = 0;
+    int tiles = 0;
+    // finest level first: its tiles are the bulk of the launch
+    for (int L = pl.Lleaf; L >= 2; L--) {
+        MMS_CHECK(lv.n < M2LLevels::kMax, "too many FMM levels");
+        const int nb = 1 << L, i = lv.n++;
+        lv.tile0[i] = tiles;
+        lv.ntx[i] = (nb + C::TX - 1) / C::TX;
+        lv.nb[i] = nb;
+        lv.off[i] = pl.lvl_off[L];
+        lv.toff[i] = (long)(L - 2) * (long)pl.m2l_level_bytes;
+        tiles += lv.ntx[i] * ((nb + 15) / 16);
+    }
+    lv.tile0[lv.n] = tiles;
+    const long zz = (long)pl.nfilm * pl.zd * pl.zd;
+    MMS_CHECK(zz <= 65535, "FMM batch too large for one M2L launch (films x layers^2 > 65535)");
+    hipLaunchKernelGGL((m2l_mfma_kernel<KP, MODE, NC>), dim3(tiles, 1, (unsigned)zz), dim3(256), (C::lds_bytes), s,
+                       pl.mult, pl.part, reinterpret_cast<const char *>(pl.d_m2l), lv, pl.zd, pl.film_stride,
+                       pl.film_stride * pl.nfilm);
     MMS_HIP(hipGetLastError());
     return 0;
 }
 
 template <int KP, int MODE>
-int launch_m2l_nc(const FmmPlan &pl, int L, hipStream_t s) {
-    // wide tiles (8 column tiles per wave) once a level has enough boxes
+int launch_m2l_nc(const FmmPlan &pl, hipStream_t s) {
     // wide tiles (8 column tiles per wave) when the source window fits in LDS
     if constexpr (M2LCfg<KP, MODE, 8>::lds_bytes <= 160 * 1024) {
-        if (pl.m2l_cols == 8 && (1 << L) >= 32) return launch_m2l<KP, MODE, 8>(dim3(), pl, L, s);
+        if (pl.m2l_cols == 8) return launch_m2l<KP, MODE, 8>(pl, s);
     }
     static_assert(M2LCfg<KP, MODE, 4>::lds_bytes <= 160 * 1024, "M2L source window exceeds LDS");
-    return launch_m2l<KP, MODE, 4>(dim3(), pl, L, s);
+    return launch_m2l<KP, MODE, 4>(pl, s);
+}
+
+template <int KP>
+int launch_l2l_sum(const FmmPlan &pl, int L, hipStream_t s) {
+    const int nb = 1 << L, half = nb >> 1;
+    dim3 grid((half + 63) / 64, half, pl.zd * pl.nfilm);
+    hipLaunchKernelGGL((l2l_sum_kernel<KP>), grid, dim3(256), 0, s, pl.part, pl.film_stride * pl.nfilm, pl.loc,
+                       pl.lvl_off[L], L > 2 ? pl.lvl_off[L - 1] : -1L, pl.d_l2l, nb, pl.zd, pl.film_stride);
+    MMS_HIP(hipGetLastError());
+    return 0;
 }
 
 template <int K, int W>
@@ -580,15 +635,19 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
                            pl.mult + pl.lvl_off[L], pl.d_m2m, nbp, pl.films());
         MMS_HIP(hipGetLastError());
     }
-    for (int L = 2; L <= pl.Lleaf; L++) {
+    {
         KTimer kt(prof, K_M2L);
         int st;
         switch (pl.precision) {
-        case MMS_M2L_BF16: st = launch_m2l_nc<KP, MMS_M2L_BF16>(pl, L, s); break;
-        case MMS_M2L_BF16X3: st = launch_m2l_nc<KP, MMS_M2L_BF16X3>(pl, L, s); break;
-        default: st = launch_m2l_nc<KP, MMS_M2L_FP32>(pl, L, s);
+        case MMS_M2L_BF16: st = launch_m2l_nc<KP, MMS_M2L_BF16>(pl, s); break;
+        case MMS_M2L_BF16X3: st = launch_m2l_nc<KP, MMS_M2L_BF16X3>(pl, s); break;
+        default: st = launch_m2l_nc<KP, MMS_M2L_FP32>(pl, s);
         }
         if (st) return 1;
+    }
+    for (int L = 2; L <= pl.Lleaf; L++) {
+        KTimer kt(prof, K_L2L);
+        if (launch_l2l_sum<KP>(pl, L, s)) return 1;
     }
     {
         KTimer kt(prof, K_P2P_L2P);
@@ -696,6 +755,7 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
     film_stride = tot;
     MMS_HIP(hipMalloc(&mult, sizeof(float) * tot * nfilm));
     MMS_HIP(hipMalloc(&loc, sizeof(float) * tot * nfilm));
+    MMS_HIP(hipMalloc(&part, sizeof(float) * tot * nfilm * zd));
     MMS_HIP(hipMemset(mult, 0, sizeof(float) * tot * nfilm));
     MMS_HIP(hipMemset(loc, 0, sizeof(float) * tot * nfilm));
     FmmHostOps ops;
@@ -776,10 +836,10 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
 }
 
 void FmmPlan::release() {
-    void *bufs[] = {mult, loc, d_p2m, d_l2p, d_l2pf, d_m2m, d_l2l, d_m2l, d_near};
+    void *bufs[] = {mult, loc, part, d_p2m, d_l2p, d_l2pf, d_m2m, d_l2l, d_m2l, d_near};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    mult = loc = d_p2m = d_l2p = d_l2pf = d_m2m = d_l2l = d_near = nullptr;
+    mult = loc = part = d_p2m = d_l2p = d_l2pf = d_m2m = d_l2l = d_near = nullptr;
     d_m2l = nullptr;
     m2l_level_bytes = 0;
     if (d_films) (void)hipFree(d_films);

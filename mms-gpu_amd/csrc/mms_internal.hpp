@@ -42,7 +42,7 @@ struct Grid {
 // ------------------------------------------------------------- profiling --
 // Kernel classes timed with hipEvents when profiling is on.
 enum KClass {
-    K_DIV = 0, K_STAGE, K_DIRECT, K_DIRECT_REDUCE, K_P2M, K_M2M, K_M2L, K_P2P_L2P,
+    K_DIV = 0, K_STAGE, K_DIRECT, K_DIRECT_REDUCE, K_P2M, K_M2M, K_M2L, K_L2L, K_P2P_L2P,
     K_HFIELD, K_ENERGY, K_PROBES, K_MISC, K_COUNT
 };
 const char *kclass_name(int k);
@@ -99,6 +99,7 @@ struct FmmPlan {
     // device buffers; box vectors have KP slots (K rounded up to 32, zero padded)
     float *mult = nullptr;   // all levels, [level][z][by][bx][KP]
     float *loc = nullptr;    // all levels, same layout
+    float *part = nullptr;   // M2L partial sums per source layer: [zs][film][level layout of loc]
     std::vector<long> lvl_off;  // float offset of each level in mult/loc
     float *d_p2m = nullptr;  // [K][w*w]
     float *d_l2p = nullptr;  // [w*w][K]

@@ -3,7 +3,7 @@
 (P = 6) on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--m2l fp32|bf16|bf16x3]
-                    [--leaf 4|8] [--no-cpu] [--no-error]
+                    [--leaf 4|8] [--probes N] [--no-cpu] [--no-error] [--no-forms]
 
 One step = one fixed-dt (5e-14 s) RK4 step = 4 demag evaluations + 4 fused
 field/LLG stage passes, state resident in HBM.  Prints ONE JSON line with the
@@ -42,18 +42,48 @@ def stage_bytes_per_cell(stage):
     return 77 if stage == 0 else 101
 
 
-def m2l_flops_per_eval(side, zd, P, w):
-    """Issued FMA flops of the M2L pass: every target box and layer contracts
-    27 offsets x zd source layers of K x K operator blocks (2 flops per MAC)."""
+def m2l_flops_per_eval(side, zd, P, w, m2l):
+    """M2L GEMMs of one evaluation (one launch covers every level): each
+    target box and layer contracts 27 offsets x zd source layers of operator
+    blocks.  Real embedding: only coefficients with n + m even are kept, K =
+    (P+1)(P+2)/2 reals per box.  Returns (useful, issued): useful counts
+    2 K^2 per block product; issued counts the MFMA work actually run, blocks
+    padded to KP = 32 ceil(K / 32) and three BF16 products per block in the
+    split (bf16x3) form."""
     K = (P + 1) * (P + 2) // 2
+    KP = 32 * ((K + 31) // 32)
     Lleaf = int(math.log2(side)) - int(math.log2(w))
-    return sum((4 ** L) * zd * 27 * zd * 2 * K * K for L in range(2, Lleaf + 1)), Lleaf - 1
+    blocks = sum((4 ** L) * zd * 27 * zd for L in range(2, Lleaf + 1))
+    return blocks * 2 * K * K, blocks * 2 * KP * KP * (3 if m2l == "bf16x3" else 1)
 
 
 def p2p_pairs_per_eval(side, zd, w):
     """Near-field pairs: each target against its 3x3 neighbour leaves in
     every layer (self excluded), edge leaves counted with their full window."""
     return side * side * zd * (9 * w * w * zd - 1)
+
+
+def p2p_l2p_flops_per_eval(side, zd, w, P):
+    """FP32 MFMA work of the fused near-field + L2P kernel: one MAC per
+    (target, window source) pair -- the 1/r table lookup replaces the
+    distance arithmetic -- plus the L2P product (KP coefficients per cell)."""
+    K = (P + 1) * (P + 2) // 2
+    KP = 32 * ((K + 31) // 32)
+    return 2 * side * side * zd * (9 * w * w * zd) + 2 * side * side * zd * KP
+
+
+def pmc_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by the reading of a
+    plain device copy of known size); None when absent."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 4 and f[0] == "traffic" and f[1] == kernel and f[2] == workload:
+            return float(f[3])
+    return None
 
 
 # ----------------------------------------------------------- CPU leg --
@@ -129,10 +159,12 @@ def cpu_baseline(w, ch, budget_s=15.0):
 
 
 # --------------------------------------------------------- field error --
-def field_error(mg, f, w, nprobe=512):
+def field_error(mg, f, w, nprobe=8192):
     """Relative L2 error of the demag field H = grad(phi)/4pi (reference
-    central differences) at seeded probe cells, FMM vs an FP64 direct sum run
-    on the device (the probe potentials at each probe and its 6 neighbours)."""
+    central differences) at seeded interior probe cells (nprobe distinct
+    cells with all six neighbours inside the grid), FMM vs an FP64 direct
+    sum run on the device (the probe potentials at each probe and its 6
+    neighbours)."""
     import numpy as np
     from mms_gpu import workloads as wl
     M = f.get_m()
@@ -152,10 +184,9 @@ def field_error(mg, f, w, nprobe=512):
     ch = (((sh(m[..., 0], 2, 1) - sh(m[..., 0], 2, -1)) / w.dx + (sh(m[..., 1], 1, 1) - sh(m[..., 1], 1, -1)) / w.dy
            + (sh(m[..., 2], 0, 1) - sh(m[..., 2], 0, -1)) / w.dz) * np.float32(0.5)).astype(np.float32).ravel()
     phi = f.potential(ch)
-    pr = wl.probes(7, w.n, nprobe)
-    pz, py, px = pr // (xd * yd), (pr // xd) % yd, pr % xd
-    keep = (px > 0) & (px < xd - 1) & (py > 0) & (py < yd - 1) & (pz > 0) & (pz < zd - 1)
-    pr, pz, py, px = pr[keep], pz[keep], py[keep], px[keep]
+    ix, iy, iz = xd - 2, yd - 2, zd - 2
+    pr = wl.probes(7, ix * iy * iz, nprobe)
+    pz, py, px = 1 + pr // (ix * iy), 1 + (pr // ix) % iy, 1 + pr % ix
     offs = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
     idx = np.stack([((pz + oz) * yd + (py + oy)) * xd + (px + ox) for ox, oy, oz in offs], 1)
     ex = f.potential_probes_f64(ch, idx.ravel()).reshape(-1, 6)
@@ -169,18 +200,96 @@ def field_error(mg, f, w, nprobe=512):
 
 
 # ---------------------------------------------------------------- main --
+PREC_NAMES = ("fp32", "bf16", "bf16x3")
+
+
+def make_film(mg, wl, w, P, m2l, leaf):
+    prec = {"fp32": mg.M2L_FP32, "bf16": mg.M2L_BF16, "bf16x3": mg.M2L_BF16X3}[m2l]
+    f = mg.Film(*w.dims, *w.d, solver=mg.SOLVER_FMM, P=P, m2l_precision=prec, leaf_width=leaf,
+                mu_0=wl.MU_0, Ms=wl.MS, Aexch=wl.AEXCH, alfa=wl.ALFA, gamma=wl.GAMMA, material=w.material)
+    f.set_m(w.M)
+    return f
+
+
+def timed_steps(f, w, warmup, steps, graph):
+    """Steps/s over `steps` fixed-dt RK4 steps after `warmup` untimed ones,
+    synchronised on both sides of the timed region."""
+    f.steps(warmup, w.dt, use_graph=graph)
+    f.sync()
+    t0 = time.perf_counter()
+    f.steps(steps, w.dt, use_graph=graph)
+    f.sync()
+    return steps / (time.perf_counter() - t0), time.perf_counter() - t0
+
+
+def kernel_rooflines(prof, side, zd, P, leaf, m2l):
+    """Achieved rate of each kernel class: algorithmic work per launch over
+    its mean HIP-event duration on the solver's stream."""
+    n = side * side * zd
+    kernels = {}
+    if "stage" in prof:
+        avg_b = (77 + 3 * 101) / 4 * n
+        ms, _ = prof["stage"]
+        kernels["stage"] = dict(bound="hbm", achieved=avg_b / (ms * 1e-3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                                per_launch=avg_b, ms=ms)
+    if "div" in prof:
+        ms, _ = prof["div"]
+        kernels["div"] = dict(bound="hbm", achieved=16 * n / (ms * 1e-3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                              per_launch=16 * n, ms=ms)
+    if "m2l" in prof:
+        ms, _ = prof["m2l"]
+        useful, issued = m2l_flops_per_eval(side, zd, P, leaf, m2l)
+        peak = BF16_PEAK_TFLOPS if m2l != "fp32" else FP32_PEAK_TFLOPS
+        kernels["m2l"] = dict(bound="mfma", achieved=issued / (ms * 1e-3) / 1e12, peak=peak, unit="TFLOP/s",
+                              per_launch=issued, ms=ms, useful_tflops=useful / (ms * 1e-3) / 1e12)
+    if "p2p_l2p" in prof:
+        ms, _ = prof["p2p_l2p"]
+        fl = p2p_l2p_flops_per_eval(side, zd, leaf, P)
+        pairs = p2p_pairs_per_eval(side, zd, leaf)
+        kernels["p2p_l2p"] = dict(bound="mfma", achieved=fl / (ms * 1e-3) / 1e12, peak=FP32_PEAK_TFLOPS,
+                                  unit="TFLOP/s", per_launch=fl, ms=ms, pairs_per_s=pairs / (ms * 1e-3))
+    return kernels
+
+
+def pmc_calibration_copies():
+    """Device-to-device copies of known size (256 MiB, then 1 GiB, beyond the
+    256 MB MALL) whose FETCH_SIZE / WRITE_SIZE readings scale the counters
+    of the film kernels to bytes.  Plain HIP runtime calls through ctypes:
+    hipMemset fills the source, hipMemcpy (device to device) is the copy."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    for nbytes in (256 << 20, 1 << 30):
+        src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+        if hip.hipMalloc(ctypes.byref(src), nbytes) or hip.hipMalloc(ctypes.byref(dst), nbytes):
+            raise RuntimeError("hipMalloc failed for the calibration copy")
+        ok = (hip.hipMemset(src, 0, nbytes) == 0 and hip.hipDeviceSynchronize() == 0
+              and hip.hipMemcpy(dst, src, nbytes, 3) == 0 and hip.hipDeviceSynchronize() == 0)
+        hip.hipFree(src)
+        hip.hipFree(dst)
+        if not ok:
+            raise RuntimeError("calibration copy failed")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--m2l", default="fp32", choices=["fp32", "bf16", "bf16x3"])
-    ap.add_argument("--leaf", type=int, default=4)
+    ap.add_argument("--m2l", default="bf16x3", choices=PREC_NAMES)
+    ap.add_argument("--leaf", type=int, default=8)
     ap.add_argument("--P", type=int, default=6)
     ap.add_argument("--side", type=int, default=1024)
+    ap.add_argument("--probes", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="skip the timing of the other M2L precisions")
     ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--pmc-calibration", action="store_true",
+                    help="end with plain device copies of 256 MiB and 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -190,83 +299,73 @@ def main():
     import mms_gpu as mg
     from mms_gpu import workloads as wl
     w = wl.c3_domains(side=args.side, zd=4)
-    prec = {"fp32": mg.M2L_FP32, "bf16": mg.M2L_BF16, "bf16x3": mg.M2L_BF16X3}[args.m2l]
     log(f"workload {w.name}: {w.n} cells, FMM P={args.P} leaf={args.leaf} m2l={args.m2l}")
-    f = mg.Film(*w.dims, *w.d, solver=mg.SOLVER_FMM, P=args.P, m2l_precision=prec, leaf_width=args.leaf,
-                mu_0=wl.MU_0, Ms=wl.MS, Aexch=wl.AEXCH, alfa=wl.ALFA, gamma=wl.GAMMA, material=w.material)
-    f.set_m(w.M)
-    f.steps(args.warmup, w.dt, use_graph=bool(args.graph))
-    f.sync()
-    # timed region: K steps, synchronised on both sides (one rank: no barrier needed)
-    t0 = time.perf_counter()
-    f.steps(args.steps, w.dt, use_graph=bool(args.graph))
-    f.sync()
-    elapsed = time.perf_counter() - t0
-    steps_per_s = args.steps / elapsed
+    f = make_film(mg, wl, w, args.P, args.m2l, args.leaf)
+    steps_per_s, elapsed = timed_steps(f, w, args.warmup, args.steps, bool(args.graph))
     log(f"timed {args.steps} steps in {elapsed:.3f} s -> {steps_per_s:.2f} steps/s")
 
     # per-kernel HIP-event durations over a profiled window of the same steps
     nprof = max(2, min(args.steps, 5))
     f.profile_begin()
-    tp = time.perf_counter()
     f.steps(nprof, w.dt, use_graph=False)
     prof = f.profile_end()
-    tp = time.perf_counter() - tp
     per_step = {k: v[0] * v[1] / nprof for k, v in prof.items()}
     log("per-step kernel ms:", {k: round(v, 4) for k, v in per_step.items()})
     m = f.get_m().reshape(-1, 3).astype(np.float64)
     mag_err = float(np.abs(np.linalg.norm(m, axis=1) / float(wl.MS) - 1).max())
 
     side, zd = args.side, 4
-    n = side * side * zd
-    fl, nlev = m2l_flops_per_eval(side, zd, args.P, args.leaf)
-    kernels = {}
-    if "stage" in prof:
-        avg_b = (77 + 3 * 101) / 4 * n
-        ms, cnt = prof["stage"]
-        kernels["stage"] = dict(bound="hbm", achieved=avg_b / (ms * 1e-3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
-                                per_launch=avg_b, ms=ms)
-    if "div" in prof:
-        ms, cnt = prof["div"]
-        kernels["div"] = dict(bound="hbm", achieved=16 * n / (ms * 1e-3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
-                              per_launch=16 * n, ms=ms)
-    if "m2l" in prof:
-        ms, cnt = prof["m2l"]
-        per_launch = fl / nlev
-        peak = BF16_PEAK_TFLOPS if args.m2l != "fp32" else FP32_PEAK_TFLOPS
-        kernels["m2l"] = dict(bound="mfma", achieved=per_launch / (ms * 1e-3) / 1e12, peak=peak, unit="TFLOP/s",
-                              per_launch=per_launch, ms=ms)
-    if "p2p_l2p" in prof:
-        ms, cnt = prof["p2p_l2p"]
-        pairs = p2p_pairs_per_eval(side, zd, args.leaf)
-        kernels["p2p_l2p"] = dict(bound="mfma", achieved=11 * pairs / (ms * 1e-3) / 1e12, peak=FP32_PEAK_TFLOPS,
-                                  unit="TFLOP/s", per_launch=11 * pairs, ms=ms,
-                                  pairs_per_s=pairs / (ms * 1e-3))
+    kernels = kernel_rooflines(prof, side, zd, args.P, args.leaf, args.m2l)
     dom = max(kernels, key=lambda k: per_step.get(k, 0.0)) if kernels else None
     roof = None
     if dom:
         k = kernels[dom]
         roof = {"kernel": dom, "bound": k["bound"], "achieved": round(k["achieved"], 3), "peak": k["peak"],
-                "unit": k["unit"], "frac": round(k["achieved"] / k["peak"], 4), "traffic": None,
+                "unit": k["unit"], "frac": round(k["achieved"] / k["peak"], 4),
+                "traffic": pmc_traffic(dom, f"c3_leaf{args.leaf}_{args.m2l}"),
                 "mean_launch_ms": round(k["ms"], 5)}
-    others = {name: {"achieved": round(k["achieved"], 3), "unit": k["unit"], "frac": round(k["achieved"] / k["peak"], 4),
-                     "mean_launch_ms": round(k["ms"], 5)} for name, k in kernels.items()}
+    others = {}
+    for name, k in kernels.items():
+        e = {"achieved": round(k["achieved"], 3), "unit": k["unit"], "frac": round(k["achieved"] / k["peak"], 4),
+             "mean_launch_ms": round(k["ms"], 5)}
+        if "pairs_per_s" in k:
+            e["pairs_per_s"] = float(f"{k['pairs_per_s']:.4g}")
+        if "useful_tflops" in k:
+            e["useful_tflops"] = round(k["useful_tflops"], 3)
+        others[name] = e
 
     err = None
     if not args.no_error:
-        e_l2, e_max, npr = field_error(mg, f, w)
-        err = {"demag_field_rel_l2": e_l2, "demag_field_rel_max": e_max, "probes": npr, "oracle": "FP64 direct sum (device)"}
+        e_l2, e_max, npr = field_error(mg, f, w, args.probes)
+        err = {"demag_field_rel_l2": e_l2, "demag_field_rel_max": e_max, "probes": npr, "oracle": "FP64 direct sum (device)",
+               "gate_1e-3": bool(e_l2 <= 1e-3)}
         log(f"demag field error vs FP64: rel L2 {e_l2:.2e}, max {e_max:.2e} on {npr} probes")
+    f.close()
+
+    # the other M2L input precisions on the same workload (short timing + error)
+    forms = {args.m2l: {"steps_per_s": round(steps_per_s, 3),
+                        "field_rel_l2": err["demag_field_rel_l2"] if err else None}}
+    if not args.no_forms:
+        for m2l in PREC_NAMES:
+            if m2l == args.m2l:
+                continue
+            g = make_film(mg, wl, w, args.P, m2l, args.leaf)
+            sps, _ = timed_steps(g, w, 2, 10, bool(args.graph))
+            e = field_error(mg, g, w, args.probes)[0] if not args.no_error else None
+            g.close()
+            forms[m2l] = {"steps_per_s": round(sps, 3), "field_rel_l2": e}
+            log(f"form m2l={m2l}: {sps:.2f} steps/s, field rel L2 {e}")
+
     cpu = None
     if not args.no_cpu:
-        from mms_gpu import workloads as wl2
-        ch = np.zeros(w.n, np.float32)
         import oracle
         ch = oracle.divergence_3d(w.M, w.dims, w.d)
         cpu = cpu_baseline(w, ch)
         if cpu:
             log(f"cpu baseline {cpu['value']:.3g} steps/s ({cpu['sample']})")
-    f.close()
+
+    if args.pmc_calibration:
+        pmc_calibration_copies()
 
     out = {
         "metric": "LLG RK4 steps/s at 1024²×4 cells, surrogate+FMM vs plain FMM at ≤1e-3 field err",
@@ -288,6 +387,7 @@ def main():
         "kernels": others,
         "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},
         "field_error": err,
+        "m2l_forms": forms,
         "max_abs_m_norm_error": mag_err,
         "cpu_baseline": cpu,
     }

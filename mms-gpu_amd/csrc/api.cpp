@@ -21,8 +21,8 @@ static thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
 
 const char *kclass_name(int k) {
-    static const char *names[K_COUNT] = {"div",      "stage",   "direct",  "direct_reduce", "p2m", "m2m",
-                                         "m2l",      "l2l",     "p2p_l2p", "hfield",  "energy",        "probes", "misc"};
+    static const char *names[K_COUNT] = {"div",      "stage",   "direct",  "direct_reduce", "upward", "m2m",
+                                         "m2l",      "downward",     "p2p_l2p", "hfield",  "energy",        "probes", "misc"};
     return (k >= 0 && k < K_COUNT) ? names[k] : "?";
 }
 

@@ -147,6 +147,19 @@ static int batch_build(mms_batch *b, const mms_film_spec *specs, const unsigned 
     }
     MMS_HIP(hipMalloc(&b->d_films, sizeof(BatchFilm) * b->nfilm));
     MMS_HIP(hipMemcpy(b->d_films, b->films.data(), sizeof(BatchFilm) * b->nfilm, hipMemcpyHostToDevice));
+    {
+        // XCD-aware order: workgroup i runs on XCD i mod 8, so list slot i
+        // holds logical tile (i mod 8) * per + i / 8 and each XCD walks a
+        // contiguous run of tiles (whole films, tile rows in order); padding
+        // slots carry film -1
+        const long nt = (long)tiles.size(), per = (nt + 7) / 8;
+        std::vector<int4> order(8 * per);
+        for (long i = 0; i < 8 * per; i++) {
+            const long t = (i & 7) * per + (i >> 3);
+            order[i] = (t < nt) ? tiles[t] : int4{-1, 0, 0, 0};
+        }
+        tiles.swap(order);
+    }
     b->ntiles = (int)tiles.size();
     MMS_HIP(hipMalloc(&b->d_tiles, sizeof(int4) * tiles.size()));
     MMS_HIP(hipMemcpy(b->d_tiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice));

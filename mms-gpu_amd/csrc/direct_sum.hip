@@ -74,28 +74,56 @@ __global__ __launch_bounds__(256) void direct_reduce_kernel(const double *__rest
     phi[t] = (float)(s * scale);
 }
 
-// FP64 probe oracle on the device: one block per probe, all sources.
+// FP64 probe oracle on the device: a 2D grid of (block of 256 probes) x
+// (source chunk); sources are staged through LDS as (x, y, z, rho) once per
+// tile and every pair is evaluated in FP64.  Chunk partials [nchunk][np] are
+// added in fixed order by probes_f64_reduce_kernel.
 __global__ __launch_bounds__(256) void probes_f64_kernel(const float *__restrict__ rho, const int64_t *__restrict__ probes,
-                                                         double *__restrict__ out, int xd, int yd, int zd,
-                                                         double dx, double dy, double dz) {
-    __shared__ double red[256];
+                                                         double *__restrict__ part, int np, int xd, int yd, int zd,
+                                                         double dx, double dy, double dz, long chunk) {
+    __shared__ double4 src[TILE];
     const long n = (long)xd * yd * zd, plane = (long)xd * yd;
-    const long i = probes[blockIdx.x];
-    const int tx = (int)(i % xd), ty = (int)((i / xd) % yd), tz = (int)(i / plane);
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    double tx = 0.0, ty = 0.0, tz = 0.0;
+    if (p < np) {
+        const long i = probes[p];
+        tx = (double)(i % xd) * dx;
+        ty = (double)((i / xd) % yd) * dy;
+        tz = (double)(i / plane) * dz;
+    }
+    const long c0 = (long)blockIdx.y * chunk;
+    const long c1 = (c0 + chunk < n) ? c0 + chunk : n;
     double acc = 0.0;
-    for (long j = threadIdx.x; j < n; j += 256) {
-        const int sx = (int)(j % xd), sy = (int)((j / xd) % yd), sz = (int)(j / plane);
-        const double rx = (double)(tx - sx) * dx, ry = (double)(ty - sy) * dy, rz = (double)(tz - sz) * dz;
-        const double r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 > 0.0) acc += (double)rho[j] / sqrt(r2);
-    }
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    for (long base = c0; base < c1; base += TILE) {
+        const long j = base + threadIdx.x;
+        double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (j < c1) {
+            v.x = (double)(j % xd) * dx;
+            v.y = (double)((j / xd) % yd) * dy;
+            v.z = (double)(j / plane) * dz;
+            v.w = (double)rho[j];
+        }
         __syncthreads();
+        src[threadIdx.x] = v;
+        __syncthreads();
+        const int cnt = (int)((c1 - base) < TILE ? (c1 - base) : TILE);
+        for (int k = 0; k < cnt; k++) {
+            const double4 q = src[k];
+            const double rx = tx - q.x, ry = ty - q.y, rz = tz - q.z;
+            const double r2 = rx * rx + ry * ry + rz * rz;
+            if (r2 > 0.0) acc += q.w / sqrt(r2);
+        }
     }
-    if (threadIdx.x == 0) out[blockIdx.x] = red[0] * dx * dy * dz;
+    if (p < np) part[(long)blockIdx.y * np + p] = acc;
+}
+
+__global__ __launch_bounds__(256) void probes_f64_reduce_kernel(const double *__restrict__ part, double *__restrict__ out,
+                                                                int np, int nchunk, double dv) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= np) return;
+    double s = 0.0;
+    for (int c = 0; c < nchunk; c++) s += part[(long)c * np + p];
+    out[p] = s * dv;
 }
 
 // Batched small films: block b takes work item items[b] = (film, target block)
@@ -204,11 +232,32 @@ int direct_potential(const Grid &g, DirectPlan &plan, const float *rho, float *p
 
 int direct_probes_f64(const Grid &g, const float *rho, const int64_t *probes, int np, double *out,
                       hipStream_t s, Profiler *prof) {
-    KTimer kt(prof, K_PROBES);
     if (np <= 0) return 0;
-    hipLaunchKernelGGL(probes_f64_kernel, dim3(np), dim3(256), 0, s, rho, probes, out, g.xd, g.yd, g.zd,
-                       (double)g.dx, (double)g.dy, (double)g.dz);
-    MMS_HIP(hipGetLastError());
+    const long n = g.n();
+    const int nbp = (np + 255) / 256;
+    int nchunk = 2048 / nbp;
+    if (nchunk < 1) nchunk = 1;
+    if (nchunk > 64) nchunk = 64;
+    long chunk = (n + nchunk - 1) / nchunk;
+    chunk = ((chunk + TILE - 1) / TILE) * TILE;
+    nchunk = (int)((n + chunk - 1) / chunk);
+    double *part = nullptr;
+    MMS_HIP(hipMalloc(&part, sizeof(double) * (size_t)nchunk * np));
+    int st = 0;
+    {
+        KTimer kt(prof, K_PROBES);
+        hipLaunchKernelGGL(probes_f64_kernel, dim3(nbp, nchunk), dim3(256), 0, s, rho, probes, part, np, g.xd, g.yd,
+                           g.zd, (double)g.dx, (double)g.dy, (double)g.dz, chunk);
+        if (hipGetLastError() != hipSuccess) st = 1;
+        if (!st) {
+            hipLaunchKernelGGL(probes_f64_reduce_kernel, dim3(nbp), dim3(256), 0, s, part, out, np, nchunk,
+                               (double)g.dx * (double)g.dy * (double)g.dz);
+            if (hipGetLastError() != hipSuccess) st = 1;
+        }
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) st = 1;
+    (void)hipFree(part);
+    MMS_CHECK(st == 0, "probe kernel launch failed");
     return 0;
 }
 

@@ -32,6 +32,7 @@
 //   BF16    v_mfma_f32_16x16x32_bf16          (BF16 operands, FP32 accumulation)
 //   BF16X3  hi/lo BF16 split of both operands, C += Th Mh + Th Ml + Tl Mh
 // L2L is always FP32.
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <type_traits>
@@ -53,42 +54,6 @@ __device__ inline unsigned short f2bf(float x) {  // round to nearest even
     return (unsigned short)(u >> 16);
 }
 __device__ inline float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
-
-// ------------------------------------------------------------------ P2M --
-template <int K, int KP, int W>
-__global__ __launch_bounds__(256) void p2m_leaf_kernel(const float *__restrict__ rho, float *__restrict__ M,
-                                                       const float *__restrict__ p2m, int xd, int yd, int nb,
-                                                       FilmSet fs) {
-    const int bx = blockIdx.x * 256 + threadIdx.x, by = blockIdx.y;
-    const int f = blockIdx.z / fs.zd, z = blockIdx.z % fs.zd;
-    if (fs.desc) {
-        const FilmDesc d = fs.desc[f];
-        xd = d.xd; yd = d.yd; rho += d.off;
-    }
-    M += (long)f * fs.stride;
-    if (bx >= nb) return;
-    const long plane = (long)xd * yd;
-    float q[W * W];
-#pragma unroll
-    for (int iy = 0; iy < W; iy++) {
-        const int y = by * W + iy;
-#pragma unroll
-        for (int ix = 0; ix < W; ix++) {
-            const int x = bx * W + ix;
-            q[iy * W + ix] = (x < xd && y < yd) ? rho[(long)z * plane + (long)y * xd + x] : 0.0f;
-        }
-    }
-    float *out = M + (((long)z * nb + by) * nb + bx) * KP;
-#pragma unroll 4
-    for (int k = 0; k < K; k++) {
-        float a = 0.0f;
-#pragma unroll
-        for (int c = 0; c < W * W; c++) a = fmaf(p2m[k * W * W + c], q[c], a);
-        out[k] = a;
-    }
-#pragma unroll
-    for (int k = K; k < KP; k++) out[k] = 0.0f;
-}
 
 // ------------------------------------------------------------------ M2M --
 // One wave per 64 parent boxes of one layer (4 MFMA column tiles of 16):
@@ -144,6 +109,222 @@ __global__ __launch_bounds__(256) void m2m_mfma_kernel(const float *__restrict__
         f32x4 *dst = reinterpret_cast<f32x4 *>(Mp + (((long)z * nbp + by) * nbp + bx) * KP);
 #pragma unroll
         for (int r = 0; r < RT; r++) dst[4 * r + g] = acc[r][c];
+    }
+}
+
+// ------------------------------------------------ upward / downward subtrees --
+// Float offsets of every level in mult / loc / part (per film).
+struct LevelOffsets { long off[16]; };
+
+// Upward pass of one subtree: the workgroup owns one box of level R of one
+// layer and the S x S leaves under it (S = 2^D, D = Lleaf - R <= 4).  P2M of
+// the leaves as FP32 MFMA [KP x W^2] . [W^2 x leaves] with the charges staged
+// in LDS, then D levels of M2M on the LDS copy; every level's multipoles are
+// written to HBM.  Wave w takes MFMA column tiles w, w + 4, ... of 16 boxes.
+template <int K, int KP, int W>
+__global__ __launch_bounds__(256) void up_subtree_kernel(const float *__restrict__ rho, float *__restrict__ Mall,
+                                                         const float *__restrict__ p2mf,
+                                                         const float *__restrict__ m2mfrag, LevelOffsets lo, int R,
+                                                         int D, int xd, int yd, long bufA_floats, FilmSet fs) {
+    constexpr int G = KP / 4, RT = KP / 16, GP = W * W / 4, SB = KP + 4;
+    extern __shared__ __attribute__((aligned(16))) float lds[];
+    const int S = 1 << D, CS = S * W, nR = 1 << R, Lleaf = R + D;
+    const int rbx = blockIdx.x % nR, rby = blockIdx.x / nR;
+    const int f = blockIdx.y / fs.zd, z = blockIdx.y % fs.zd;
+    if (fs.desc) {
+        const FilmDesc d = fs.desc[f];
+        xd = d.xd; yd = d.yd; rho += d.off;
+    }
+    float *M = Mall + (long)f * fs.stride;
+    const long plane = (long)xd * yd;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j16 = lane & 15, g = lane >> 4;
+    const int cx0 = rbx * CS, cy0 = rby * CS;
+    float *q = lds;  // CS x CS charges, later the leaf multipoles [leaf][SB]
+    for (int e = threadIdx.x; e < CS * CS; e += 256) {
+        const int x = cx0 + e % CS, y = cy0 + e / CS;
+        q[e] = (x < xd && y < yd) ? rho[(long)z * plane + (long)y * xd + x] : 0.0f;
+    }
+    float a[RT][GP];
+#pragma unroll
+    for (int r = 0; r < RT; r++)
+#pragma unroll
+        for (int u = 0; u < GP; u++) a[r][u] = p2mf[((long)r * 64 + lane) * GP + u];
+    __syncthreads();
+    const int nleaf = S * S, ntile = (nleaf + 15) / 16;
+    f32x4 acc[RT][4];
+#pragma unroll
+    for (int r = 0; r < RT; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++) {
+        const int c = wave + 4 * ct;
+        if (c < ntile) {
+            const int n = 16 * c + j16, nx = n % S, ny = n / S;
+            const bool ok = n < nleaf;
+#pragma unroll
+            for (int u = 0; u < GP; u++) {
+                const int cell = GP * g + u;
+                const float b = ok ? q[(ny * W + cell / W) * CS + nx * W + cell % W] : 0.0f;
+#pragma unroll
+                for (int r = 0; r < RT; r++)
+                    acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r][u], b, acc[r][ct], 0, 0, 0);
+            }
+        }
+    }
+    __syncthreads();  // charges consumed: the leaf multipoles go over them
+    {
+        const int nbl = 1 << Lleaf;
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++) {
+            const int c = wave + 4 * ct, n = 16 * c + j16;
+            if (c < ntile && n < nleaf) {
+                const int gx = rbx * S + n % S, gy = rby * S + n / S;
+                float *dst = M + lo.off[Lleaf] + (((long)z * nbl + gy) * nbl + gx) * KP;
+#pragma unroll
+                for (int r = 0; r < RT; r++) {
+                    *reinterpret_cast<f32x4 *>(dst + 16 * r + 4 * g) = acc[r][ct];
+                    *reinterpret_cast<f32x4 *>(q + n * SB + 16 * r + 4 * g) = acc[r][ct];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    float *ch = q, *pa = lds + bufA_floats;
+    int s = S;
+    for (int l = Lleaf - 1; l >= R; l--) {
+        const int sp = s >> 1, np = sp * sp, ntp = (np + 15) / 16, nbp = 1 << l;
+        for (int c = wave; c < ntp; c += 4) {
+            const int p = 16 * c + j16, px = p % sp, py = p / sp;
+            const bool ok = p < np;
+            f32x4 acc2[RT];
+#pragma unroll
+            for (int r = 0; r < RT; r++) acc2[r] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+            for (int ci = 0; ci < 4; ci++) {
+                const int chn = (2 * py + (ci >> 1)) * s + 2 * px + (ci & 1);
+#pragma unroll
+                for (int u4 = 0; u4 < G / 4; u4++) {
+                    f32x4 av[RT];
+#pragma unroll
+                    for (int r = 0; r < RT; r++)
+                        av[r] = *reinterpret_cast<const f32x4 *>(m2mfrag + (((long)ci * RT + r) * 64 + lane) * G + 4 * u4);
+                    const f32x4 b = ok ? *reinterpret_cast<const f32x4 *>(ch + chn * SB + g * G + 4 * u4)
+                                       : f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int u = 0; u < 4; u++)
+#pragma unroll
+                        for (int r = 0; r < RT; r++)
+                            acc2[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[r][u], b[u], acc2[r], 0, 0, 0);
+                }
+            }
+            if (ok) {
+                const int gx = rbx * sp + px, gy = rby * sp + py;
+                float *dst = M + lo.off[l] + (((long)z * nbp + gy) * nbp + gx) * KP;
+#pragma unroll
+                for (int r = 0; r < RT; r++) {
+                    *reinterpret_cast<f32x4 *>(dst + 16 * r + 4 * g) = acc2[r];
+                    *reinterpret_cast<f32x4 *>(pa + p * SB + 16 * r + 4 * g) = acc2[r];
+                }
+            }
+        }
+        __syncthreads();
+        float *t = ch; ch = pa; pa = t;
+        s = sp;
+    }
+}
+
+// Downward pass of one subtree: the workgroup owns one box of level R of one
+// layer.  Each level's local expansions are the M2L partial sums over source
+// layers plus L2L from the parent (FP32 MFMA, wave w = child position w);
+// the root takes its parent from HBM (level R - 1), the levels between stay
+// in LDS and only the leaf locals are written to HBM.
+template <int KP>
+__global__ __launch_bounds__(256) void down_subtree_kernel(const float *__restrict__ part, long part_stride,
+                                                           float *__restrict__ Lall, const float *__restrict__ l2lfrag,
+                                                           LevelOffsets lo, int R, int D, int zd, long film_stride) {
+    constexpr int G = KP / 4, RT = KP / 16, SB = KP + 4;
+    extern __shared__ __attribute__((aligned(16))) float lds[];
+    const int nR = 1 << R, Lleaf = R + D;
+    const int rbx = blockIdx.x % nR, rby = blockIdx.x / nR;
+    const int f = blockIdx.y / zd, z = blockIdx.y % zd;
+    const float *Pf = part + (long)f * film_stride;
+    float *L = Lall + (long)f * film_stride;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j16 = lane & 15, g = lane >> 4;
+    float *pa = lds, *ch = lds + 64 * SB;
+    // root box (level R): wave 0, column 0
+    if (wave == 0) {
+        f32x4 acc[RT];
+#pragma unroll
+        for (int r = 0; r < RT; r++) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
+        if (R > 2) {
+            const int ci = (rbx & 1) + 2 * (rby & 1), half = nR >> 1;
+            const float *Lp = L + lo.off[R - 1] + (((long)z * half + (rby >> 1)) * half + (rbx >> 1)) * KP;
+#pragma unroll
+            for (int u4 = 0; u4 < G / 4; u4++) {
+                f32x4 av[RT];
+#pragma unroll
+                for (int r = 0; r < RT; r++)
+                    av[r] = *reinterpret_cast<const f32x4 *>(l2lfrag + (((long)ci * RT + r) * 64 + lane) * G + 4 * u4);
+                const f32x4 b = (j16 == 0) ? *reinterpret_cast<const f32x4 *>(Lp + g * G + 4 * u4)
+                                           : f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int r = 0; r < RT; r++) acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[r][u], b[u], acc[r], 0, 0, 0);
+            }
+        }
+        if (j16 == 0) {
+            const long e = lo.off[R] + (((long)z * nR + rby) * nR + rbx) * KP;
+#pragma unroll
+            for (int r = 0; r < RT; r++) {
+                f32x4 v = acc[r];
+                for (int s = 0; s < zd; s++) v += *reinterpret_cast<const f32x4 *>(Pf + s * part_stride + e + 16 * r + 4 * g);
+                if (D == 0) *reinterpret_cast<f32x4 *>(L + e + 16 * r + 4 * g) = v;
+                else *reinterpret_cast<f32x4 *>(pa + 16 * r + 4 * g) = v;
+            }
+        }
+    }
+    __syncthreads();
+    const int cxp = wave & 1, cyp = wave >> 1;
+    int sp = 1;
+    for (int l = R + 1; l <= Lleaf; l++) {
+        const int s = 2 * sp, np = sp * sp, ntp = (np + 15) / 16, nb = 1 << l;
+        for (int c = 0; c < ntp; c++) {
+            const int p = 16 * c + j16, px = p % sp, py = p / sp;
+            const bool ok = p < np;
+            f32x4 acc[RT];
+#pragma unroll
+            for (int r = 0; r < RT; r++) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int u4 = 0; u4 < G / 4; u4++) {
+                f32x4 av[RT];
+#pragma unroll
+                for (int r = 0; r < RT; r++)
+                    av[r] = *reinterpret_cast<const f32x4 *>(l2lfrag + (((long)wave * RT + r) * 64 + lane) * G + 4 * u4);
+                const f32x4 b = ok ? *reinterpret_cast<const f32x4 *>(pa + p * SB + g * G + 4 * u4)
+                                   : f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int r = 0; r < RT; r++) acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[r][u], b[u], acc[r], 0, 0, 0);
+            }
+            if (ok) {
+                const int cx = 2 * px + cxp, cy = 2 * py + cyp;
+                const long e = lo.off[l] + (((long)z * nb + rby * s + cy) * nb + rbx * s + cx) * KP;
+#pragma unroll
+                for (int r = 0; r < RT; r++) {
+                    f32x4 v = acc[r];
+                    for (int zs = 0; zs < zd; zs++)
+                        v += *reinterpret_cast<const f32x4 *>(Pf + zs * part_stride + e + 16 * r + 4 * g);
+                    if (l == Lleaf) *reinterpret_cast<f32x4 *>(L + e + 16 * r + 4 * g) = v;
+                    else *reinterpret_cast<f32x4 *>(ch + (cy * s + cx) * SB + 16 * r + 4 * g) = v;
+                }
+            }
+        }
+        __syncthreads();
+        float *t = pa; pa = ch; ch = t;
+        sp = s;
     }
 }
 
@@ -617,17 +798,30 @@ int launch_l2l_sum(const FmmPlan &pl, int L, hipStream_t s) {
 
 template <int K, int W>
 int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_t s, Profiler *prof) {
-    constexpr int KP = KPad<K>::KP;
+    constexpr int KP = KPad<K>::KP, SB = KP + 4;
     const int zd = g.zd;
+    LevelOffsets lo{};
+    for (int L = 2; L <= pl.Lleaf; L++) lo.off[L] = pl.lvl_off[L];
+    // subtrees rooted at level R hold up to 16 x 16 leaves
+    const int R = pl.Lleaf - 4 > 2 ? pl.Lleaf - 4 : 2, D = pl.Lleaf - R, nR = 1 << R;
     {
         KTimer kt(prof, K_P2M);
-        const int nb = 1 << pl.Lleaf;
-        dim3 grid((nb + 255) / 256, nb, zd * pl.nfilm);
-        hipLaunchKernelGGL((p2m_leaf_kernel<K, KP, W>), grid, dim3(256), 0, s, rho, pl.mult + pl.lvl_off[pl.Lleaf],
-                           pl.d_p2m, g.xd, g.yd, nb, pl.films());
+        const int S = 1 << D, CS = S * W;
+        const long bufA = std::max((long)CS * CS, (long)S * S * SB);
+        const long bufB = (long)std::max(1, (S / 2) * (S / 2)) * SB;
+        const size_t lds = sizeof(float) * (size_t)(bufA + bufB);
+        MMS_CHECK(lds <= 160 * 1024, "FMM upward pass: subtree exceeds LDS");
+        static int attr = 0;
+        if (!attr) {
+            MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&up_subtree_kernel<K, KP, W>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            attr = 1;
+        }
+        hipLaunchKernelGGL((up_subtree_kernel<K, KP, W>), dim3(nR * nR, zd * pl.nfilm), dim3(256), lds, s, rho,
+                           pl.mult, pl.d_p2mf, pl.d_m2m, lo, R, D, g.xd, g.yd, bufA, pl.films());
         MMS_HIP(hipGetLastError());
     }
-    for (int L = pl.Lleaf - 1; L >= 2; L--) {
+    for (int L = R - 1; L >= 2; L--) {
         KTimer kt(prof, K_M2M);
         const int nbp = 1 << L;
         dim3 grid((nbp + 255) / 256, nbp, zd * pl.nfilm);
@@ -645,9 +839,20 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
         }
         if (st) return 1;
     }
-    for (int L = 2; L <= pl.Lleaf; L++) {
+    {
         KTimer kt(prof, K_L2L);
-        if (launch_l2l_sum<KP>(pl, L, s)) return 1;
+        for (int L = 2; L < R; L++)
+            if (launch_l2l_sum<KP>(pl, L, s)) return 1;
+        const size_t lds = sizeof(float) * 2 * 64 * SB;
+        static int attr = 0;
+        if (!attr) {
+            MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&down_subtree_kernel<KP>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            attr = 1;
+        }
+        hipLaunchKernelGGL((down_subtree_kernel<KP>), dim3(nR * nR, zd * pl.nfilm), dim3(256), lds, s, pl.part,
+                           pl.film_stride * pl.nfilm, pl.loc, pl.d_l2l, lo, R, D, zd, pl.film_stride);
+        MMS_HIP(hipGetLastError());
     }
     {
         KTimer kt(prof, K_P2P_L2P);
@@ -765,9 +970,18 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
         MMS_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
         return 0;
     };
-    std::vector<float> f(ops.p2m.begin(), ops.p2m.end());                      // [K][w*w]
-    if (upload((void **)&d_p2m, f.data(), sizeof(float) * f.size())) return 1;
-    f.assign(ops.l2p.begin(), ops.l2p.end());                                  // [w*w][K]
+    {   // P2M as MFMA fragments [KP/16][64][w*w/4]: P2M[16 r + lane % 16][(w*w/4) (lane / 16) + u]
+        const int gp = w * w / 4, rtk = KP / 16;
+        std::vector<float> pf((size_t)rtk * 64 * gp, 0.0f);
+        for (int r = 0; r < rtk; r++)
+            for (int lane = 0; lane < 64; lane++)
+                for (int u = 0; u < gp; u++) {
+                    const int row = 16 * r + lane % 16, col = gp * (lane / 16) + u;
+                    if (row < K) pf[((size_t)r * 64 + lane) * gp + u] = (float)ops.p2m[(size_t)row * w * w + col];
+                }
+        if (upload((void **)&d_p2mf, pf.data(), sizeof(float) * pf.size())) return 1;
+    }
+    std::vector<float> f(ops.l2p.begin(), ops.l2p.end());                      // [w*w][K]
     if (upload((void **)&d_l2p, f.data(), sizeof(float) * f.size())) return 1;
     // MFMA A-fragments: frag[blk][rt][lane][u] = Op[16 rt + lane % 16][G (lane / 16) + u]
     const int G = KP / 4, RT = KP / 16;
@@ -836,10 +1050,10 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
 }
 
 void FmmPlan::release() {
-    void *bufs[] = {mult, loc, part, d_p2m, d_l2p, d_l2pf, d_m2m, d_l2l, d_m2l, d_near};
+    void *bufs[] = {mult, loc, part, d_p2mf, d_l2p, d_l2pf, d_m2m, d_l2l, d_m2l, d_near};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    mult = loc = part = d_p2m = d_l2p = d_l2pf = d_m2m = d_l2l = d_near = nullptr;
+    mult = loc = part = d_p2mf = d_l2p = d_l2pf = d_m2m = d_l2l = d_near = nullptr;
     d_m2l = nullptr;
     m2l_level_bytes = 0;
     if (d_films) (void)hipFree(d_films);

@@ -101,7 +101,7 @@ struct FmmPlan {
     float *loc = nullptr;    // all levels, same layout
     float *part = nullptr;   // M2L partial sums per source layer: [zs][film][level layout of loc]
     std::vector<long> lvl_off;  // float offset of each level in mult/loc
-    float *d_p2m = nullptr;  // [K][w*w]
+    float *d_p2mf = nullptr; // P2M as MFMA fragments [KP/16][64][w*w/4]
     float *d_l2p = nullptr;  // [w*w][K]
     float *d_l2pf = nullptr; // leaf width 4: L2P as MFMA fragments [64][KP/4]
     float *d_m2m = nullptr;  // MFMA fragments [4 child positions][KP/16][64][KP/4]

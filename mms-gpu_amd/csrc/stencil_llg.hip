@@ -22,6 +22,21 @@ namespace {
 
 constexpr int BX = 64, BY = 4;
 
+// XCD-aware tile order for a 1D grid of 8 * ceil(ntiles / 8) workgroups.
+// Workgroup i runs on XCD i mod 8 (round-robin dispatch); it takes logical
+// tile (i mod 8) * per + i / 8, so each XCD walks one contiguous band of
+// tile rows in order and the y-halo rows of a tile are still in that XCD's
+// L2 from the tile row above.  Returns false for the padding workgroups.
+__device__ __forceinline__ bool xcd_tile(int ntx, int ntiles, int &tx, int &ty) {
+    const int per = (ntiles + 7) >> 3;
+    const int i = blockIdx.x;
+    const int t = (i & 7) * per + (i >> 3);
+    if (t >= ntiles) return false;
+    tx = t % ntx;
+    ty = t / ntx;
+    return true;
+}
+
 template <int ZD>
 __device__ __forceinline__ void div_column(const float *__restrict__ m, float *__restrict__ rho, int x, int y, int xd,
                                            int yd, int zd_rt, long n, float dx, float dy, float dz) {
@@ -48,8 +63,10 @@ __device__ __forceinline__ void div_column(const float *__restrict__ m, float *_
 template <int ZD>
 __global__ __launch_bounds__(BX * BY) void div_kernel(const float *__restrict__ m, float *__restrict__ rho, int xd,
                                                       int yd, int zd, float dx, float dy, float dz) {
-    div_column<ZD>(m, rho, blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y, xd, yd, zd,
-                   (long)xd * yd * zd, dx, dy, dz);
+    const int ntx = (xd + BX - 1) / BX, nty = (yd + BY - 1) / BY;
+    int tx, ty;
+    if (!xcd_tile(ntx, ntx * nty, tx, ty)) return;
+    div_column<ZD>(m, rho, tx * BX + threadIdx.x, ty * BY + threadIdx.y, xd, yd, zd, (long)xd * yd * zd, dx, dy, dz);
 }
 
 template <int ZD>
@@ -58,6 +75,7 @@ __global__ __launch_bounds__(BX * BY) void div_batch_kernel(const float *__restr
                                                             const BatchFilm *__restrict__ films, long pstride,
                                                             float dx, float dy, float dz) {
     const int4 t = tiles[blockIdx.x];
+    if (t.x < 0) return;  // padding slot of the XCD-ordered tile list
     const BatchFilm f = films[t.x];
     div_column<ZD>(m + f.off, rho + f.off, t.y * BX + threadIdx.x, t.z * BY + threadIdx.y, f.xd, f.yd, f.zd, pstride,
                    dx, dy, dz);
@@ -224,8 +242,11 @@ __device__ __forceinline__ void stage_column(const StageArgs &A, int x, int y, i
 
 template <int ZD>
 __global__ __launch_bounds__(BX * BY) void stage_kernel(StageArgs A) {
-    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y;
-    stage_column<ZD>(A, x, y, A.xd, A.yd, A.zd, 0, (long)A.xd * A.yd * A.zd, A.hx, A.hy, A.hz);
+    const int ntx = (A.xd + BX - 1) / BX, nty = (A.yd + BY - 1) / BY;
+    int tx, ty;
+    if (!xcd_tile(ntx, ntx * nty, tx, ty)) return;
+    stage_column<ZD>(A, tx * BX + threadIdx.x, ty * BY + threadIdx.y, A.xd, A.yd, A.zd, 0, (long)A.xd * A.yd * A.zd,
+                     A.hx, A.hy, A.hz);
 }
 
 // Batched films: block b handles tile tiles[b] = (film, tile x, tile y) of a
@@ -234,6 +255,7 @@ template <int ZD>
 __global__ __launch_bounds__(BX * BY) void stage_batch_kernel(StageArgs A, const int4 *__restrict__ tiles,
                                                               const BatchFilm *__restrict__ films, long pstride) {
     const int4 t = tiles[blockIdx.x];
+    if (t.x < 0) return;  // padding slot of the XCD-ordered tile list
     const BatchFilm f = films[t.x];
     stage_column<ZD>(A, t.y * BX + threadIdx.x, t.z * BY + threadIdx.y, f.xd, f.yd, f.zd, f.off, pstride, f.hx, f.hy,
                      f.hz);
@@ -369,11 +391,17 @@ StageArgs make_args(const Grid &g, const Material &mat) {
     return A;
 }
 
+// workgroups of the XCD-ordered 1D grid: 8 * ceil(tiles / 8)
+unsigned xcd_grid(const Grid &g) {
+    const long ntiles = (long)((g.xd + BX - 1) / BX) * ((g.yd + BY - 1) / BY);
+    return (unsigned)(8 * ((ntiles + 7) / 8));
+}
+
 }  // namespace
 
 int launch_div(const Grid &g, const float *m, float *rho, hipStream_t s, Profiler *prof) {
     KTimer kt(prof, K_DIV);
-    dim3 grid((g.xd + BX - 1) / BX, (g.yd + BY - 1) / BY), block(BX, BY);
+    dim3 grid(xcd_grid(g)), block(BX, BY);
     switch (g.zd) {
     case 3: hipLaunchKernelGGL(div_kernel<3>, grid, block, 0, s, m, rho, g.xd, g.yd, g.zd, g.dx, g.dy, g.dz); break;
     case 4: hipLaunchKernelGGL(div_kernel<4>, grid, block, 0, s, m, rho, g.xd, g.yd, g.zd, g.dx, g.dy, g.dz); break;
@@ -392,7 +420,7 @@ int launch_stage_ex(const Grid &g, const Material &mat, int stage, double dt, co
     A.t_dev = t_dev; A.stage = stage; A.dt = dt;
     A.h_given = ex.h_given; A.h_out = ex.h_out; A.k_out = ex.k_out; A.slope_out = ex.slope_out;
     A.normalize = ex.normalize; A.mn_out = ex.mn_out;
-    dim3 grid((g.xd + BX - 1) / BX, (g.yd + BY - 1) / BY), block(BX, BY);
+    dim3 grid(xcd_grid(g)), block(BX, BY);
     switch (g.zd) {
     case 3: hipLaunchKernelGGL(stage_kernel<3>, grid, block, 0, s, A); break;
     case 4: hipLaunchKernelGGL(stage_kernel<4>, grid, block, 0, s, A); break;

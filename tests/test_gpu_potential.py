@@ -126,8 +126,8 @@ def test_fmm_c2_dots_against_fp64_probes(mg, orc):
     slack = 2e-6 * np.abs(ex).max()
     assert np.all(err <= bound + slack), float((err / (bound + slack)).max())
     with mg.Film(*w.dims, *w.d) as f:
-        gpu64 = f.potential_probes_f64(ch, probes[:512])
-    assert np.abs(gpu64 - ex[:512]).max() <= 1e-10 * np.abs(ex).max()
+        gpu64 = f.potential_probes_f64(ch, probes)
+    assert np.abs(gpu64 - ex).max() <= 1e-10 * np.abs(ex).max()
 
 
 @pytest.mark.parametrize("P,wl_", [(4, 4), (6, 4), (6, 8), (8, 2)])

@@ -742,6 +742,121 @@ __global__ __launch_bounds__(256) void p2p_l2p_mfma_kernel(const float *__restri
     }
 }
 
+// Near field in the BF16 split form, leaf width 8.  For a target row pair
+// (iy = 2p, 2p + 1) and a source row sy of the 24 x 24 window of each box,
+// the 1/r sum over the row is one product with a constant operator
+//   A_{dz,dy}[8 r + ix][sx] = G(dz, dy - r, sx - ix),  dy = sy - 2p + 7,
+// (16 target cells x 32 sources, sources 24..31 zero), stored per (layer
+// offset, dy) as BF16 hi / lo MFMA fragments in lane order.  The column tile
+// is the 16 boxes of the block (B = the boxes' source row, BF16 hi / lo from
+// LDS), so per (dz, sy) a wave issues 4 pairs x 3 products of
+// v_mfma_f32_16x16x32_bf16 (Ah Bh + Ah Bl + Al Bh).  Walking sy with a
+// stride of 2, three of the four operators of the next row are the previous
+// row's: a four-deep register window, one new operator per row.
+// L2P is fused as in the FP32 kernel.
+template <int K, int KP>
+__global__ __launch_bounds__(256) void p2p_l2p_bf16_kernel(const float *__restrict__ rho, float *__restrict__ phi,
+                                                           const float *__restrict__ Lleaf,
+                                                           const float *__restrict__ l2pfrag,
+                                                           const unsigned short *__restrict__ afrag, int xd, int yd,
+                                                           int zd, int nb, float scale, FilmSet fs) {
+    constexpr int W = 8, TW = 32, SW = 48, SP = 56;  // LDS row pitch (elements)
+    if (fs.desc) {
+        const FilmDesc d = fs.desc[blockIdx.z];
+        xd = d.xd; yd = d.yd; rho += d.off; phi += d.off;
+        if ((int)blockIdx.x * TW >= xd || (int)blockIdx.y * TW >= yd) return;  // block-uniform
+    }
+    Lleaf += (long)blockIdx.z * fs.stride;
+    constexpr int G = KP / 4, RT = 4;
+    extern __shared__ __attribute__((aligned(16))) unsigned short sh[];
+    unsigned short *qh = sh, *ql = sh + zd * SW * SP;  // [zd][SW][SP] hi, lo
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j16 = lane & 15, g = lane >> 4;
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TW;
+    const long plane = (long)xd * yd;
+    for (int e = threadIdx.x; e < zd * SW * SW; e += 256) {
+        const int zs = e / (SW * SW), r = e % (SW * SW), sy = r / SW, sx = r % SW;
+        const int gx = x0 - W + sx, gy = y0 - W + sy;
+        const float v = (gx >= 0 && gx < xd && gy >= 0 && gy < yd) ? rho[(long)zs * plane + (long)gy * xd + gx] : 0.0f;
+        const unsigned short h = f2bf(v);
+        qh[(zs * SW + sy) * SP + sx] = h;
+        ql[(zs * SW + sy) * SP + sx] = f2bf(v - bf2f(h));
+    }
+    __syncthreads();
+    const int jx = j16 & 3, jy = j16 >> 2;
+    const bool kpad = (g == 3);  // sources 24..31 of the 32-wide k step
+    const bf16x8 zero8 = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    for (int zt = wave; zt < zd; zt += 4) {
+        f32x4 acc[RT];
+#pragma unroll
+        for (int r = 0; r < RT; r++) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+        for (int zs = 0; zs < zd; zs++) {
+            const int dzi = zs - zt + zd - 1;
+            const unsigned short *Ab = afrag + (long)dzi * 31 * 2 * 64 * 8 + lane * 8;
+            auto loadA = [&](bf16x8 &h, bf16x8 &l, int dy) {
+                h = *reinterpret_cast<const bf16x8 *>(Ab + (long)(dy * 2) * 64 * 8);
+                l = *reinterpret_cast<const bf16x8 *>(Ab + (long)(dy * 2 + 1) * 64 * 8);
+            };
+            const unsigned short *bh0 = qh + (zs * SW + W * jy) * SP + W * jx + 8 * g;
+            const unsigned short *bl0 = ql + (zs * SW + W * jy) * SP + W * jx + 8 * g;
+#pragma unroll
+            for (int s0 = 0; s0 < 2; s0++) {
+                // window: A[j] = operator of dy = sy + 7 - 2 j (pair p = j)
+                bf16x8 Ah[4], Al[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) loadA(Ah[j], Al[j], s0 + 7 - 2 * j);
+#pragma unroll
+                for (int i = 0; i < 12; i++) {
+                    const int sy = s0 + 2 * i;
+                    const bf16x8 bh = kpad ? zero8 : *reinterpret_cast<const bf16x8 *>(bh0 + sy * SP);
+                    const bf16x8 bl = kpad ? zero8 : *reinterpret_cast<const bf16x8 *>(bl0 + sy * SP);
+#pragma unroll
+                    for (int p = 0; p < 4; p++) {
+                        acc[p] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[p], bh, acc[p], 0, 0, 0);
+                        acc[p] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[p], bl, acc[p], 0, 0, 0);
+                        acc[p] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al[p], bh, acc[p], 0, 0, 0);
+                    }
+                    if (i < 11) {  // slide: the next row needs dy + 2 for pair 0
+#pragma unroll
+                        for (int j = 3; j > 0; j--) { Ah[j] = Ah[j - 1]; Al[j] = Al[j - 1]; }
+                        loadA(Ah[0], Al[0], sy + 9);
+                    }
+                }
+            }
+        }
+        // far field: L2P[64 x KP] * L[KP x 16 boxes]
+        const int bxg = blockIdx.x * 4 + jx, byg = blockIdx.y * 4 + jy;
+        const bool boxok = (bxg < nb && byg < nb);
+        const float *Lv = Lleaf + (((long)zt * nb + (boxok ? byg : 0)) * nb + (boxok ? bxg : 0)) * KP + g * G;
+#pragma unroll
+        for (int u4 = 0; u4 < G / 4; u4++) {
+            f32x4 b = boxok ? *reinterpret_cast<const f32x4 *>(Lv + 4 * u4) : f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int r = 0; r < RT; r++) {
+                const f32x4 a = *reinterpret_cast<const f32x4 *>(l2pfrag + ((long)r * 64 + lane) * G + 4 * u4);
+#pragma unroll
+                for (int u = 0; u < 4; u++) acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], acc[r], 0, 0, 0);
+            }
+        }
+        // D row 4g + q of tile r = cell 16 r + 4g + q of box j16: 4 consecutive x cells
+#pragma unroll
+        for (int r = 0; r < RT; r++) {
+            const int c = 16 * r + 4 * g;
+            const int y = y0 + W * jy + c / W, xb = x0 + W * jx + c % W;
+            if (y >= yd) continue;
+            float *dst = phi + (long)zt * plane + (long)y * xd + xb;
+            if (xb + 3 < xd && (((long)zt * plane + (long)y * xd + xb) & 3) == 0) {
+                *reinterpret_cast<f32x4 *>(dst) = acc[r] * scale;
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    if (xb + q < xd) dst[q] = acc[r][q] * scale;
+            }
+        }
+    }
+}
+
 // -------------------------------------------------------------- dispatch --
 template <int KP, int MODE, int NC>
 int launch_m2l(const FmmPlan &pl, hipStream_t s) {
@@ -857,7 +972,26 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
     {
         KTimer kt(prof, K_P2P_L2P);
         const float scale = (float)((double)(g.dx * g.dy * g.dz) / (double)g.dx);
-        if constexpr (W == 4 || W == 8) {
+        bool done = false;
+        if constexpr (W == 8) {
+            if (pl.precision != MMS_M2L_FP32) {
+                const size_t lds = sizeof(unsigned short) * 2 * (size_t)zd * 48 * 56;
+                MMS_CHECK(lds <= 160 * 1024, "FMM near field: too many layers for the LDS tile");
+                static int attr = 0;
+                if (!attr) {
+                    MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&p2p_l2p_bf16_kernel<K, KP>),
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                    attr = 1;
+                }
+                dim3 grid((g.xd + 31) / 32, (g.yd + 31) / 32, pl.nfilm);
+                hipLaunchKernelGGL((p2p_l2p_bf16_kernel<K, KP>), grid, dim3(256), lds, s, rho, phi,
+                                   pl.loc + pl.lvl_off[pl.Lleaf], pl.d_l2pf, pl.d_p2pA, g.xd, g.yd, zd,
+                                   1 << pl.Lleaf, scale, pl.films());
+                done = true;
+            }
+        }
+        if (done) {
+        } else if constexpr (W == 4 || W == 8) {
             constexpr int TW = 4 * W, SW = TW + 2 * W, TS = 4 * W - 1;
             const size_t lds = sizeof(float) * ((size_t)zd * SW * SW + (size_t)(2 * zd - 1) * TS * TS);
             static int attr = 0;
@@ -1046,15 +1180,38 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
                     r2 > 0 ? (float)(1.0 / std::sqrt(r2)) : 0.0f;
             }
     if (upload((void **)&d_near, tab.data(), sizeof(float) * tab.size())) return 1;
+    if (w == 8) {
+        // BF16 split near-field operators [2zd-1][dy 0..30][hi, lo][64 lanes][8]:
+        // lane (m = 8 r + ix, k group gg) holds G(dz, dy - r, 8 gg + t - ix) for
+        // t = 0..7, zero for sources 24..31 (the 32-wide k step pads the 24-cell row)
+        const int nd = 2 * zd - 1, ts = near_side;
+        std::vector<unsigned short> af((size_t)nd * 31 * 2 * 64 * 8, 0);
+        for (int dzi = 0; dzi < nd; dzi++)
+            for (int dy = 0; dy < 31; dy++)
+                for (int lane = 0; lane < 64; lane++) {
+                    const int m = lane % 16, gg = lane / 16, r = m / 8, ix = m % 8;
+                    for (int t = 0; t < 8; t++) {
+                        const int sx = 8 * gg + t;
+                        float v = 0.0f;
+                        if (sx < 24 && dy - r >= 0) v = tab[((size_t)dzi * ts + (dy - r)) * ts + (sx - ix + 7)];
+                        const unsigned short h = host_bf16(v);
+                        const size_t o = (((size_t)(dzi * 31 + dy) * 2) * 64 + lane) * 8 + t;
+                        af[o] = h;
+                        af[o + 64 * 8] = host_bf16(v - host_bf2f(h));
+                    }
+                }
+        if (upload((void **)&d_p2pA, af.data(), sizeof(unsigned short) * af.size())) return 1;
+    }
     return 0;
 }
 
 void FmmPlan::release() {
-    void *bufs[] = {mult, loc, part, d_p2mf, d_l2p, d_l2pf, d_m2m, d_l2l, d_m2l, d_near};
+    void *bufs[] = {mult, loc, part, d_p2pA, d_p2mf, d_l2p, d_l2pf, d_m2m, d_l2l, d_m2l, d_near};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     mult = loc = part = d_p2mf = d_l2p = d_l2pf = d_m2m = d_l2l = d_near = nullptr;
     d_m2l = nullptr;
+    d_p2pA = nullptr;
     m2l_level_bytes = 0;
     if (d_films) (void)hipFree(d_films);
     d_films = nullptr;

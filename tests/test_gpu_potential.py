@@ -176,22 +176,42 @@ def test_fmm_linearity_and_sign(mg):
     assert _rel(pab, pa.astype(float) + pb) < 5e-6   # FP32 rounding of a + b and of the sums
 
 
+@pytest.mark.parametrize("leaf", [4, 8])
 @pytest.mark.parametrize("mode,tol", [("fp32", 2e-6), ("bf16x3", 2e-5), ("bf16", 3e-3)])
-def test_fmm_m2l_precision_modes(mg, mode, tol):
+def test_fmm_m2l_precision_modes(mg, mode, tol, leaf):
     """M2L on the matrix cores in each operand precision vs the FP64 FMM
     oracle (same P, leaf width, tree): FP32 to rounding; hi/lo BF16 split
-    to ~2^-16 of the far field; plain BF16 to ~2^-8 of the far field."""
+    to ~2^-16 of the far field; plain BF16 to ~2^-8 of the far field.  At
+    leaf width 8 the BF16 modes also run the near field in the hi/lo BF16
+    split (three products), which stays at the split's ~2^-16."""
     from oracle import fmm_oracle as fo
     rng = np.random.default_rng(21)
     xd, zd, P = 128, 4, 6
     q = (rng.normal(size=(zd, xd, xd)) * 1e14).astype(F)
     prec = {"fp32": mg.M2L_FP32, "bf16": mg.M2L_BF16, "bf16x3": mg.M2L_BF16X3}[mode]
-    with mg.Film(xd, xd, zd, *D5, solver=mg.SOLVER_FMM, P=P, m2l_precision=prec) as f:
+    with mg.Film(xd, xd, zd, *D5, solver=mg.SOLVER_FMM, P=P, m2l_precision=prec, leaf_width=leaf) as f:
         pot = f.potential(q.ravel())
-    ref = fo.FmmOracle(xd, xd, zd, P, w=4).potential_units(q.astype(float)).ravel() * float(D5[0]) ** 2
+    ref = fo.FmmOracle(xd, xd, zd, P, w=leaf).potential_units(q.astype(float)).ravel() * float(D5[0]) ** 2
     err = _rel(pot, ref)
-    _say(f"m2l {mode}: rel err vs FP64 FMM {err:.2e}")
+    _say(f"m2l {mode} leaf {leaf}: rel err vs FP64 FMM {err:.2e}")
     assert err < tol
+
+
+def test_bf16_near_field_ragged_films(mg):
+    """Leaf width 8 with the BF16 split near field on non-square,
+    non-power-of-two films with zero and sign-alternating layers: matches
+    the FP32 near field to the split's precision."""
+    rng = np.random.default_rng(5)
+    for xd, yd, zd in [(72, 40, 3), (100, 130, 4), (64, 64, 5)]:
+        q = (rng.normal(size=(zd, yd, xd)) * 1e14).astype(F)
+        q[1] = 0.0
+        pots = []
+        for prec in (mg.M2L_FP32, mg.M2L_BF16X3):
+            with mg.Film(xd, yd, zd, *D5, solver=mg.SOLVER_FMM, P=6, m2l_precision=prec, leaf_width=8) as f:
+                pots.append(f.potential(q.ravel()))
+        err = _rel(pots[1], pots[0])
+        _say(f"bf16 near field {xd}x{yd}x{zd}: rel diff vs FP32 {err:.2e}")
+        assert err < 2e-5
 
 
 @pytest.mark.parametrize("solver", ["fmm", "direct"])

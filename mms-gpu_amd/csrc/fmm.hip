@@ -754,42 +754,45 @@ __global__ __launch_bounds__(256) void p2p_l2p_mfma_kernel(const float *__restri
 // stride of 2, three of the four operators of the next row are the previous
 // row's: a four-deep register window, one new operator per row.
 // L2P is fused as in the FP32 kernel.
-template <int K, int KP>
+template <int K, int KP, int NCB>
 __global__ __launch_bounds__(256) void p2p_l2p_bf16_kernel(const float *__restrict__ rho, float *__restrict__ phi,
                                                            const float *__restrict__ Lleaf,
                                                            const float *__restrict__ l2pfrag,
                                                            const unsigned short *__restrict__ afrag, int xd, int yd,
                                                            int zd, int nb, float scale, FilmSet fs) {
-    constexpr int W = 8, TW = 32, SW = 48, SP = 56;  // LDS row pitch (elements)
+    // block = (4 NCB) x 4 boxes; NCB column tiles of 16 boxes share each operator load
+    constexpr int W = 8, TWX = 32 * NCB, TWY = 32, SWX = TWX + 16, SWY = 48, SP = SWX + 8;
     if (fs.desc) {
         const FilmDesc d = fs.desc[blockIdx.z];
         xd = d.xd; yd = d.yd; rho += d.off; phi += d.off;
-        if ((int)blockIdx.x * TW >= xd || (int)blockIdx.y * TW >= yd) return;  // block-uniform
+        if ((int)blockIdx.x * TWX >= xd || (int)blockIdx.y * TWY >= yd) return;  // block-uniform
     }
     Lleaf += (long)blockIdx.z * fs.stride;
     constexpr int G = KP / 4, RT = 4;
     extern __shared__ __attribute__((aligned(16))) unsigned short sh[];
-    unsigned short *qh = sh, *ql = sh + zd * SW * SP;  // [zd][SW][SP] hi, lo
+    unsigned short *qh = sh, *ql = sh + zd * SWY * SP;  // [zd][SWY][SP] hi, lo
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j16 = lane & 15, g = lane >> 4;
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TW;
+    const int x0 = blockIdx.x * TWX, y0 = blockIdx.y * TWY;
     const long plane = (long)xd * yd;
-    for (int e = threadIdx.x; e < zd * SW * SW; e += 256) {
-        const int zs = e / (SW * SW), r = e % (SW * SW), sy = r / SW, sx = r % SW;
+    for (int e = threadIdx.x; e < zd * SWY * SWX; e += 256) {
+        const int zs = e / (SWY * SWX), r = e % (SWY * SWX), sy = r / SWX, sx = r % SWX;
         const int gx = x0 - W + sx, gy = y0 - W + sy;
         const float v = (gx >= 0 && gx < xd && gy >= 0 && gy < yd) ? rho[(long)zs * plane + (long)gy * xd + gx] : 0.0f;
         const unsigned short h = f2bf(v);
-        qh[(zs * SW + sy) * SP + sx] = h;
-        ql[(zs * SW + sy) * SP + sx] = f2bf(v - bf2f(h));
+        qh[(zs * SWY + sy) * SP + sx] = h;
+        ql[(zs * SWY + sy) * SP + sx] = f2bf(v - bf2f(h));
     }
     __syncthreads();
-    const int jx = j16 & 3, jy = j16 >> 2;
+    const int jy = j16 >> 2;
     const bool kpad = (g == 3);  // sources 24..31 of the 32-wide k step
     const bf16x8 zero8 = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
     for (int zt = wave; zt < zd; zt += 4) {
-        f32x4 acc[RT];
+        f32x4 acc[RT][NCB];
 #pragma unroll
-        for (int r = 0; r < RT; r++) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int r = 0; r < RT; r++)
+#pragma unroll
+            for (int c = 0; c < NCB; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 1
         for (int zs = 0; zs < zd; zs++) {
             const int dzi = zs - zt + zd - 1;
@@ -798,60 +801,73 @@ __global__ __launch_bounds__(256) void p2p_l2p_bf16_kernel(const float *__restri
                 h = *reinterpret_cast<const bf16x8 *>(Ab + (long)(dy * 2) * 64 * 8);
                 l = *reinterpret_cast<const bf16x8 *>(Ab + (long)(dy * 2 + 1) * 64 * 8);
             };
-            const unsigned short *bh0 = qh + (zs * SW + W * jy) * SP + W * jx + 8 * g;
-            const unsigned short *bl0 = ql + (zs * SW + W * jy) * SP + W * jx + 8 * g;
+            const int boff = (zs * SWY + W * jy) * SP + 8 * g;
 #pragma unroll
             for (int s0 = 0; s0 < 2; s0++) {
-                // window: A[j] = operator of dy = sy + 7 - 2 j (pair p = j)
-                bf16x8 Ah[4], Al[4];
+                // window: A[j] = operator of dy = sy + 7 - 2 j (pair p = j); An = next row's pair 0
+                bf16x8 Ah[4], Al[4], Anh, Anl;
 #pragma unroll
                 for (int j = 0; j < 4; j++) loadA(Ah[j], Al[j], s0 + 7 - 2 * j);
+                loadA(Anh, Anl, s0 + 9);
 #pragma unroll
                 for (int i = 0; i < 12; i++) {
                     const int sy = s0 + 2 * i;
-                    const bf16x8 bh = kpad ? zero8 : *reinterpret_cast<const bf16x8 *>(bh0 + sy * SP);
-                    const bf16x8 bl = kpad ? zero8 : *reinterpret_cast<const bf16x8 *>(bl0 + sy * SP);
+                    bf16x8 bh[NCB], bl[NCB];
 #pragma unroll
-                    for (int p = 0; p < 4; p++) {
-                        acc[p] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[p], bh, acc[p], 0, 0, 0);
-                        acc[p] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[p], bl, acc[p], 0, 0, 0);
-                        acc[p] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al[p], bh, acc[p], 0, 0, 0);
+                    for (int c = 0; c < NCB; c++) {
+                        const int jx = 4 * c + (j16 & 3);
+                        bh[c] = kpad ? zero8 : *reinterpret_cast<const bf16x8 *>(qh + boff + sy * SP + W * jx);
+                        bl[c] = kpad ? zero8 : *reinterpret_cast<const bf16x8 *>(ql + boff + sy * SP + W * jx);
                     }
-                    if (i < 11) {  // slide: the next row needs dy + 2 for pair 0
+#pragma unroll
+                    for (int p = 0; p < 4; p++)
+#pragma unroll
+                        for (int c = 0; c < NCB; c++) {
+                            acc[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[p], bh[c], acc[p][c], 0, 0, 0);
+                            acc[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[p], bl[c], acc[p][c], 0, 0, 0);
+                            acc[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al[p], bh[c], acc[p][c], 0, 0, 0);
+                        }
+                    if (i < 11) {  // slide: pair p of the next row uses pair p - 1's operator
 #pragma unroll
                         for (int j = 3; j > 0; j--) { Ah[j] = Ah[j - 1]; Al[j] = Al[j - 1]; }
-                        loadA(Ah[0], Al[0], sy + 9);
+                        Ah[0] = Anh; Al[0] = Anl;
+                        if (i < 10) loadA(Anh, Anl, sy + 11);
                     }
                 }
             }
         }
-        // far field: L2P[64 x KP] * L[KP x 16 boxes]
-        const int bxg = blockIdx.x * 4 + jx, byg = blockIdx.y * 4 + jy;
-        const bool boxok = (bxg < nb && byg < nb);
-        const float *Lv = Lleaf + (((long)zt * nb + (boxok ? byg : 0)) * nb + (boxok ? bxg : 0)) * KP + g * G;
 #pragma unroll
-        for (int u4 = 0; u4 < G / 4; u4++) {
-            f32x4 b = boxok ? *reinterpret_cast<const f32x4 *>(Lv + 4 * u4) : f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < NCB; c++) {
+            const int jx = 4 * c + (j16 & 3);
+            // far field: L2P[64 x KP] * L[KP x 16 boxes]
+            const int bxg = blockIdx.x * 4 * NCB + jx, byg = blockIdx.y * 4 + jy;
+            const bool boxok = (bxg < nb && byg < nb);
+            const float *Lv = Lleaf + (((long)zt * nb + (boxok ? byg : 0)) * nb + (boxok ? bxg : 0)) * KP + g * G;
+#pragma unroll
+            for (int u4 = 0; u4 < G / 4; u4++) {
+                f32x4 b = boxok ? *reinterpret_cast<const f32x4 *>(Lv + 4 * u4) : f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int r = 0; r < RT; r++) {
+                    const f32x4 a = *reinterpret_cast<const f32x4 *>(l2pfrag + ((long)r * 64 + lane) * G + 4 * u4);
+#pragma unroll
+                    for (int u = 0; u < 4; u++)
+                        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], acc[r][c], 0, 0, 0);
+                }
+            }
+            // D row 4g + q of tile r = cell 16 r + 4g + q of box j16: 4 consecutive x cells
 #pragma unroll
             for (int r = 0; r < RT; r++) {
-                const f32x4 a = *reinterpret_cast<const f32x4 *>(l2pfrag + ((long)r * 64 + lane) * G + 4 * u4);
+                const int cc = 16 * r + 4 * g;
+                const int y = y0 + W * jy + cc / W, xb = x0 + W * jx + cc % W;
+                if (y >= yd) continue;
+                float *dst = phi + (long)zt * plane + (long)y * xd + xb;
+                if (xb + 3 < xd && (((long)zt * plane + (long)y * xd + xb) & 3) == 0) {
+                    *reinterpret_cast<f32x4 *>(dst) = acc[r][c] * scale;
+                } else {
 #pragma unroll
-                for (int u = 0; u < 4; u++) acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], acc[r], 0, 0, 0);
-            }
-        }
-        // D row 4g + q of tile r = cell 16 r + 4g + q of box j16: 4 consecutive x cells
-#pragma unroll
-        for (int r = 0; r < RT; r++) {
-            const int c = 16 * r + 4 * g;
-            const int y = y0 + W * jy + c / W, xb = x0 + W * jx + c % W;
-            if (y >= yd) continue;
-            float *dst = phi + (long)zt * plane + (long)y * xd + xb;
-            if (xb + 3 < xd && (((long)zt * plane + (long)y * xd + xb) & 3) == 0) {
-                *reinterpret_cast<f32x4 *>(dst) = acc[r] * scale;
-            } else {
-#pragma unroll
-                for (int q = 0; q < 4; q++)
-                    if (xb + q < xd) dst[q] = acc[r][q] * scale;
+                    for (int q = 0; q < 4; q++)
+                        if (xb + q < xd) dst[q] = acc[r][c][q] * scale;
+                }
             }
         }
     }
@@ -907,6 +923,25 @@ int launch_l2l_sum(const FmmPlan &pl, int L, hipStream_t s) {
     dim3 grid((half + 63) / 64, half, pl.zd * pl.nfilm);
     hipLaunchKernelGGL((l2l_sum_kernel<KP>), grid, dim3(256), 0, s, pl.part, pl.film_stride * pl.nfilm, pl.loc,
                        pl.lvl_off[L], L > 2 ? pl.lvl_off[L - 1] : -1L, pl.d_l2l, nb, pl.zd, pl.film_stride);
+    MMS_HIP(hipGetLastError());
+    return 0;
+}
+
+template <int K, int KP, int NCB>
+int launch_p2p_bf16(const Grid &g, const FmmPlan &pl, const float *rho, float *phi, float scale, hipStream_t s) {
+    constexpr int SP = 32 * NCB + 16 + 8;
+    const size_t lds = sizeof(unsigned short) * 2 * (size_t)g.zd * 48 * SP;
+    MMS_CHECK(lds <= 160 * 1024, "FMM near field: too many layers for the LDS tile");
+    static int attr = 0;
+    if (!attr) {
+        MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&p2p_l2p_bf16_kernel<K, KP, NCB>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr = 1;
+    }
+    dim3 grid((g.xd + 32 * NCB - 1) / (32 * NCB), (g.yd + 31) / 32, pl.nfilm);
+    hipLaunchKernelGGL((p2p_l2p_bf16_kernel<K, KP, NCB>), grid, dim3(256), lds, s, rho, phi,
+                       pl.loc + pl.lvl_off[pl.Lleaf], pl.d_l2pf, pl.d_p2pA, g.xd, g.yd, g.zd, 1 << pl.Lleaf, scale,
+                       pl.films());
     MMS_HIP(hipGetLastError());
     return 0;
 }
@@ -975,18 +1010,7 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
         bool done = false;
         if constexpr (W == 8) {
             if (pl.precision != MMS_M2L_FP32) {
-                const size_t lds = sizeof(unsigned short) * 2 * (size_t)zd * 48 * 56;
-                MMS_CHECK(lds <= 160 * 1024, "FMM near field: too many layers for the LDS tile");
-                static int attr = 0;
-                if (!attr) {
-                    MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&p2p_l2p_bf16_kernel<K, KP>),
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-                    attr = 1;
-                }
-                dim3 grid((g.xd + 31) / 32, (g.yd + 31) / 32, pl.nfilm);
-                hipLaunchKernelGGL((p2p_l2p_bf16_kernel<K, KP>), grid, dim3(256), lds, s, rho, phi,
-                                   pl.loc + pl.lvl_off[pl.Lleaf], pl.d_l2pf, pl.d_p2pA, g.xd, g.yd, zd,
-                                   1 << pl.Lleaf, scale, pl.films());
+                if (launch_p2p_bf16<K, KP, 2>(g, pl, rho, phi, scale, s)) return 1;
                 done = true;
             }
         }

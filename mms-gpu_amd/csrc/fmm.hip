@@ -328,6 +328,130 @@ __global__ __launch_bounds__(256) void down_subtree_kernel(const float *__restri
     }
 }
 
+// Coarse levels of the upward pass in one workgroup per (film, layer):
+// levels Ltop - 1 .. 2 by M2M, the children of the first level read from HBM
+// (level Ltop <= 5), the coarser ones from LDS; every level is written.
+template <int KP>
+__global__ __launch_bounds__(256) void up_coarse_kernel(float *__restrict__ Mall, const float *__restrict__ m2mfrag,
+                                                        LevelOffsets lo, int Ltop, int zd, long film_stride) {
+    constexpr int G = KP / 4, RT = KP / 16, SB = KP + 4;
+    extern __shared__ __attribute__((aligned(16))) float lds[];
+    const int f = blockIdx.x / zd, z = blockIdx.x % zd;
+    float *M = Mall + (long)f * film_stride;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j16 = lane & 15, g = lane >> 4;
+    float *const b0 = lds, *const b1 = lds + 256 * SB;  // level l goes to b1 if l is odd, else b0
+    int s = 1 << Ltop;
+    for (int l = Ltop - 1; l >= 2; l--) {
+        const int sp = s >> 1, np = sp * sp, ntp = (np + 15) / 16;
+        const float *ch = ((l + 1) & 1) ? b1 : b0;
+        float *pa = (l & 1) ? b1 : b0;
+        const bool from_hbm = (l == Ltop - 1);
+        for (int c = wave; c < ntp; c += 4) {
+            const int p = 16 * c + j16, px = p % sp, py = p / sp;
+            const bool ok = p < np;
+            f32x4 acc[RT];
+#pragma unroll
+            for (int r = 0; r < RT; r++) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+            for (int ci = 0; ci < 4; ci++) {
+                const int cx = 2 * px + (ci & 1), cy = 2 * py + (ci >> 1);
+                const float *bsrc = from_hbm ? M + lo.off[l + 1] + (((long)z * s + cy) * s + cx) * KP
+                                             : ch + (cy * s + cx) * SB;
+#pragma unroll
+                for (int u4 = 0; u4 < G / 4; u4++) {
+                    f32x4 av[RT];
+#pragma unroll
+                    for (int r = 0; r < RT; r++)
+                        av[r] = *reinterpret_cast<const f32x4 *>(m2mfrag + (((long)ci * RT + r) * 64 + lane) * G + 4 * u4);
+                    const f32x4 b = ok ? *reinterpret_cast<const f32x4 *>(bsrc + g * G + 4 * u4) : f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int u = 0; u < 4; u++)
+#pragma unroll
+                        for (int r = 0; r < RT; r++) acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[r][u], b[u], acc[r], 0, 0, 0);
+                }
+            }
+            if (ok) {
+                float *dst = M + lo.off[l] + (((long)z * sp + py) * sp + px) * KP;
+#pragma unroll
+                for (int r = 0; r < RT; r++) {
+                    *reinterpret_cast<f32x4 *>(dst + 16 * r + 4 * g) = acc[r];
+                    *reinterpret_cast<f32x4 *>(pa + p * SB + 16 * r + 4 * g) = acc[r];
+                }
+            }
+        }
+        __syncthreads();
+        s = sp;
+    }
+}
+
+// Coarse levels of the downward pass in one workgroup per (film, layer):
+// levels 2 .. Lbot (Lbot <= 4): the M2L partial sums plus L2L from the
+// parent (LDS), every level written to HBM.
+template <int KP>
+__global__ __launch_bounds__(256) void down_coarse_kernel(const float *__restrict__ part, long part_stride,
+                                                          float *__restrict__ Lall, const float *__restrict__ l2lfrag,
+                                                          LevelOffsets lo, int Lbot, int zd, long film_stride) {
+    constexpr int G = KP / 4, RT = KP / 16, SB = KP + 4;
+    extern __shared__ __attribute__((aligned(16))) float lds[];
+    const int f = blockIdx.x / zd, z = blockIdx.x % zd;
+    const float *Pf = part + (long)f * film_stride;
+    float *L = Lall + (long)f * film_stride;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j16 = lane & 15, g = lane >> 4;
+    float *const b0 = lds, *const b1 = lds + 256 * SB;  // level l goes to b1 if l is odd, else b0
+    // level 2: the partial sums only (16 boxes, one per lane of wave 0's column tile)
+    if (wave == 0) {
+        const long e = lo.off[2] + ((long)z * 16 + j16) * KP;
+#pragma unroll
+        for (int r = 0; r < RT; r++) {
+            f32x4 v = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int zs = 0; zs < zd; zs++) v += *reinterpret_cast<const f32x4 *>(Pf + zs * part_stride + e + 16 * r + 4 * g);
+            *reinterpret_cast<f32x4 *>(L + e + 16 * r + 4 * g) = v;
+            *reinterpret_cast<f32x4 *>(b0 + j16 * SB + 16 * r + 4 * g) = v;
+        }
+    }
+    __syncthreads();
+    const int cxp = wave & 1, cyp = wave >> 1;
+    int sp = 4;
+    for (int l = 3; l <= Lbot; l++) {
+        const int s = 2 * sp, np = sp * sp, ntp = (np + 15) / 16;
+        const float *pa = ((l - 1) & 1) ? b1 : b0;
+        float *ch = (l & 1) ? b1 : b0;
+        for (int c = 0; c < ntp; c++) {
+            const int p = 16 * c + j16, px = p % sp, py = p / sp;
+            const bool ok = p < np;
+            f32x4 acc[RT];
+#pragma unroll
+            for (int r = 0; r < RT; r++) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int u4 = 0; u4 < G / 4; u4++) {
+                f32x4 av[RT];
+#pragma unroll
+                for (int r = 0; r < RT; r++)
+                    av[r] = *reinterpret_cast<const f32x4 *>(l2lfrag + (((long)wave * RT + r) * 64 + lane) * G + 4 * u4);
+                const f32x4 b = ok ? *reinterpret_cast<const f32x4 *>(pa + p * SB + g * G + 4 * u4) : f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int r = 0; r < RT; r++) acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[r][u], b[u], acc[r], 0, 0, 0);
+            }
+            if (ok) {
+                const int cx = 2 * px + cxp, cy = 2 * py + cyp;
+                const long e = lo.off[l] + (((long)z * s + cy) * s + cx) * KP;
+#pragma unroll
+                for (int r = 0; r < RT; r++) {
+                    f32x4 v = acc[r];
+                    for (int zs = 0; zs < zd; zs++)
+                        v += *reinterpret_cast<const f32x4 *>(Pf + zs * part_stride + e + 16 * r + 4 * g);
+                    *reinterpret_cast<f32x4 *>(L + e + 16 * r + 4 * g) = v;
+                    *reinterpret_cast<f32x4 *>(ch + (cy * s + cx) * SB + 16 * r + 4 * g) = v;
+                }
+            }
+        }
+        __syncthreads();
+        sp = s;
+    }
+}
+
 // -------------------------------------------------------- M2L + L2L --
 // A workgroup owns a dense TX x 16 block of target boxes (TX = 16 or 32) of one
 // target layer; wave w handles child parity (w & 1, w >> 1): an SX x 8
@@ -952,8 +1076,12 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
     const int zd = g.zd;
     LevelOffsets lo{};
     for (int L = 2; L <= pl.Lleaf; L++) lo.off[L] = pl.lvl_off[L];
-    // subtrees rooted at level R hold up to 16 x 16 leaves
-    const int R = pl.Lleaf - 4 > 2 ? pl.Lleaf - 4 : 2, D = pl.Lleaf - R, nR = 1 << R;
+    // subtrees rooted at level R hold up to 8 x 8 leaves (16 x 16 when the
+    // grid is so deep that the coarse levels would pass level 4)
+    int D = (pl.Lleaf - 3 <= 5) ? 3 : 4;
+    if (pl.Lleaf - D < 2) D = pl.Lleaf - 2;
+    const int R = pl.Lleaf - D, nR = 1 << R;
+    const int Lc = R - 1 < 4 ? R - 1 : 4;  // coarse levels 2..Lc in one workgroup per layer
     {
         KTimer kt(prof, K_P2M);
         const int S = 1 << D, CS = S * W;
@@ -971,13 +1099,26 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
                            pl.mult, pl.d_p2mf, pl.d_m2m, lo, R, D, g.xd, g.yd, bufA, pl.films());
         MMS_HIP(hipGetLastError());
     }
-    for (int L = R - 1; L >= 2; L--) {
+    {
         KTimer kt(prof, K_M2M);
-        const int nbp = 1 << L;
-        dim3 grid((nbp + 255) / 256, nbp, zd * pl.nfilm);
-        hipLaunchKernelGGL((m2m_mfma_kernel<KP>), grid, dim3(256), 0, s, pl.mult + pl.lvl_off[L + 1],
-                           pl.mult + pl.lvl_off[L], pl.d_m2m, nbp, pl.films());
-        MMS_HIP(hipGetLastError());
+        for (int L = R - 1; L > Lc; L--) {
+            const int nbp = 1 << L;
+            dim3 grid((nbp + 255) / 256, nbp, zd * pl.nfilm);
+            hipLaunchKernelGGL((m2m_mfma_kernel<KP>), grid, dim3(256), 0, s, pl.mult + pl.lvl_off[L + 1],
+                               pl.mult + pl.lvl_off[L], pl.d_m2m, nbp, pl.films());
+            MMS_HIP(hipGetLastError());
+        }
+        if (Lc >= 2) {
+            static int attr = 0;
+            if (!attr) {
+                MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&up_coarse_kernel<KP>),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                attr = 1;
+            }
+            hipLaunchKernelGGL((up_coarse_kernel<KP>), dim3(zd * pl.nfilm), dim3(256),
+                               sizeof(float) * (256 + 64) * SB, s, pl.mult, pl.d_m2m, lo, Lc + 1, zd, pl.film_stride);
+            MMS_HIP(hipGetLastError());
+        }
     }
     {
         KTimer kt(prof, K_M2L);
@@ -991,7 +1132,19 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
     }
     {
         KTimer kt(prof, K_L2L);
-        for (int L = 2; L < R; L++)
+        if (Lc >= 2) {
+            static int attr = 0;
+            if (!attr) {
+                MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&down_coarse_kernel<KP>),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                attr = 1;
+            }
+            hipLaunchKernelGGL((down_coarse_kernel<KP>), dim3(zd * pl.nfilm), dim3(256),
+                               sizeof(float) * (256 + 256) * SB, s, pl.part, pl.film_stride * pl.nfilm, pl.loc,
+                               pl.d_l2l, lo, Lc, zd, pl.film_stride);
+            MMS_HIP(hipGetLastError());
+        }
+        for (int L = Lc + 1; L < R; L++)
             if (launch_l2l_sum<KP>(pl, L, s)) return 1;
         const size_t lds = sizeof(float) * 2 * 64 * SB;
         static int attr = 0;

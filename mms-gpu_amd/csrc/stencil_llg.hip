@@ -206,20 +206,26 @@ __device__ __forceinline__ void stage_column(const StageArgs &A, int x, int y, i
             float kx, ky, kz;
             llg_rhs(mxc, myc, mzc, Hx, Hy, Hz, A.alfa, A.gamma, A.Ms, kx, ky, kz);
             if (A.k_out) { A.k_out[3 * c] = kx; A.k_out[3 * c + 1] = ky; A.k_out[3 * c + 2] = kz; }
+            // the FP64 base and slope accumulator are touched once per stage:
+            // non-temporal loads and stores keep them from evicting the
+            // neighbour-reused FP32 m and phi from the caches
             double *bx = mn_, *by = mn_ + n, *bz = mn_ + 2 * n;
             double *ax = acc_, *ay = acc_ + n, *az = acc_ + 2 * n;
-            const double b0 = bx[c], b1 = by[c], b2 = bz[c];
+            const double b0 = __builtin_nontemporal_load(&bx[c]), b1 = __builtin_nontemporal_load(&by[c]),
+                         b2 = __builtin_nontemporal_load(&bz[c]);
             double s0, s1, s2;
             if (A.stage == 0) { s0 = kx; s1 = ky; s2 = kz; }
-            else { s0 = ax[c] + (A.stage == 3 ? 1.0 : 2.0) * kx;
-                   s1 = ay[c] + (A.stage == 3 ? 1.0 : 2.0) * ky;
-                   s2 = az[c] + (A.stage == 3 ? 1.0 : 2.0) * kz; }
+            else { s0 = __builtin_nontemporal_load(&ax[c]) + (A.stage == 3 ? 1.0 : 2.0) * kx;
+                   s1 = __builtin_nontemporal_load(&ay[c]) + (A.stage == 3 ? 1.0 : 2.0) * ky;
+                   s2 = __builtin_nontemporal_load(&az[c]) + (A.stage == 3 ? 1.0 : 2.0) * kz; }
             if (A.slope_out) { A.slope_out[3 * c] = (float)s0; A.slope_out[3 * c + 1] = (float)s1; A.slope_out[3 * c + 2] = (float)s2; }
             double n0, n1, n2;
             if (A.stage < 3) {
                 const double h = (A.stage < 2 ? 0.5 : 1.0) * A.dt;
                 n0 = b0 + h * (double)kx; n1 = b1 + h * (double)ky; n2 = b2 + h * (double)kz;
-                ax[c] = s0; ay[c] = s1; az[c] = s2;
+                __builtin_nontemporal_store(s0, &ax[c]);
+                __builtin_nontemporal_store(s1, &ay[c]);
+                __builtin_nontemporal_store(s2, &az[c]);
             } else {
                 const double h = A.dt / 6.0;
                 n0 = b0 + h * s0; n1 = b1 + h * s1; n2 = b2 + h * s2;
@@ -228,7 +234,9 @@ __device__ __forceinline__ void stage_column(const StageArgs &A, int x, int y, i
                     n0 *= f; n1 *= f; n2 *= f;
                 }
                 double *o64 = mn_out_ ? mn_out_ : mn_;
-                o64[c] = n0; o64[n + c] = n1; o64[2 * n + c] = n2;
+                __builtin_nontemporal_store(n0, &o64[c]);
+                __builtin_nontemporal_store(n1, &o64[n + c]);
+                __builtin_nontemporal_store(n2, &o64[2 * n + c]);
             }
             ox[c] = (float)n0; oy[c] = (float)n1; oz[c] = (float)n2;
         } else {

@@ -1,7 +1,8 @@
 """ctypes binding of libmmsgpu.so (the gfx950 HIP library).
 
-The library is built in-tree (``mms-gpu_amd/libmmsgpu.so``).  Loading fails
-loudly when it is missing: there is no CPU fallback.
+The library is built in-tree (``mms-gpu_amd/libmmsgpu.so``); the environment
+variable MMS_GPU_LIB may name another build of it.  Loading fails loudly when
+it is missing: there is no CPU fallback.
 """
 import ctypes as C
 import os
@@ -9,7 +10,7 @@ import os
 import numpy as np
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_ROOT, "libmmsgpu.so")
+LIB_PATH = os.environ.get("MMS_GPU_LIB") or os.path.join(PKG_ROOT, "libmmsgpu.so")
 
 _lib = None
 

@@ -63,13 +63,16 @@ def p2p_pairs_per_eval(side, zd, w):
     return side * side * zd * (9 * w * w * zd - 1)
 
 
-def p2p_l2p_flops_per_eval(side, zd, w, P):
-    """FP32 MFMA work of the fused near-field + L2P kernel: one MAC per
-    (target, window source) pair -- the 1/r table lookup replaces the
-    distance arithmetic -- plus the L2P product (KP coefficients per cell)."""
-    K = (P + 1) * (P + 2) // 2
-    KP = 32 * ((K + 31) // 32)
-    return 2 * side * side * zd * (9 * w * w * zd) + 2 * side * side * zd * KP
+def p2p_flops_per_eval(side, zd, w, m2l):
+    """MFMA work of the near-field kernel (L2P runs in its own kernel).
+    FP32 form: one MAC per (target, window source) pair -- the 1/r table
+    lookup replaces the distance arithmetic.  BF16 split form (leaf 8): per
+    target cell, source row and layer one 32-wide k step (24 sources + 8 zero
+    pad) in three BF16 products.  Returns (flops, peak TFLOP/s)."""
+    cells = side * side * zd
+    if w == 8 and m2l != "fp32":
+        return 2 * cells * zd * (3 * w) * 32 * 3, BF16_PEAK_TFLOPS
+    return 2 * cells * zd * 9 * w * w, FP32_PEAK_TFLOPS
 
 
 def pmc_traffic(kernel, workload):
@@ -242,12 +245,12 @@ def kernel_rooflines(prof, side, zd, P, leaf, m2l):
         peak = BF16_PEAK_TFLOPS if m2l != "fp32" else FP32_PEAK_TFLOPS
         kernels["m2l"] = dict(bound="mfma", achieved=issued / (ms * 1e-3) / 1e12, peak=peak, unit="TFLOP/s",
                               per_launch=issued, ms=ms, useful_tflops=useful / (ms * 1e-3) / 1e12)
-    if "p2p_l2p" in prof:
-        ms, _ = prof["p2p_l2p"]
-        fl = p2p_l2p_flops_per_eval(side, zd, leaf, P)
+    if "p2p" in prof:
+        ms, _ = prof["p2p"]
+        fl, peak = p2p_flops_per_eval(side, zd, leaf, m2l)
         pairs = p2p_pairs_per_eval(side, zd, leaf)
-        kernels["p2p_l2p"] = dict(bound="mfma", achieved=fl / (ms * 1e-3) / 1e12, peak=FP32_PEAK_TFLOPS,
-                                  unit="TFLOP/s", per_launch=fl, ms=ms, pairs_per_s=pairs / (ms * 1e-3))
+        kernels["p2p"] = dict(bound="mfma", achieved=fl / (ms * 1e-3) / 1e12, peak=peak,
+                              unit="TFLOP/s", per_launch=fl, ms=ms, pairs_per_s=pairs / (ms * 1e-3))
     return kernels
 
 

@@ -22,7 +22,7 @@ void set_error(const std::string &msg) { g_err = msg; }
 
 const char *kclass_name(int k) {
     static const char *names[K_COUNT] = {"div",      "stage",   "direct",  "direct_reduce", "upward", "m2m",
-                                         "m2l",      "downward",     "p2p_l2p", "hfield",  "energy",        "probes", "misc"};
+                                         "m2l",      "downward",     "p2p",     "l2p", "hfield",  "energy",        "probes", "misc"};
     return (k >= 0 && k < K_COUNT) ? names[k] : "?";
 }
 

@@ -778,7 +778,7 @@ __global__ __launch_bounds__(256) void p2p_l2p_kernel(const float *__restrict__ 
 // neighbour leaves in every layer) and G[i][k] = 1/r is read from the LDS
 // offset table (self entry 0).  FP32 MFMA 16x16x4 with k = (zs, sy, sx) and
 // sx in groups of 4 matching the 4 lane groups.
-template <int K, int KP, int W>
+template <int K, int KP, int W, bool FAR>
 __global__ __launch_bounds__(256) void p2p_l2p_mfma_kernel(const float *__restrict__ rho, float *__restrict__ phi,
                                                            const float *__restrict__ Lleaf,
                                                            const float *__restrict__ l2pfrag,
@@ -834,18 +834,21 @@ __global__ __launch_bounds__(256) void p2p_l2p_mfma_kernel(const float *__restri
                 }
             }
         }
-        // far field: L2P[W*W x KP] * L[KP x 16 boxes]
+        // far field: L2P[W*W x KP] * L[KP x 16 boxes] (skipped when the far
+        // field is added by l2p_add_kernel after a concurrent FMM pass)
         const int bxg = blockIdx.x * 4 + jx, byg = blockIdx.y * 4 + jy;
         const bool boxok = (bxg < nb && byg < nb);
         const float *Lv = Lleaf + (((long)zt * nb + (boxok ? byg : 0)) * nb + (boxok ? bxg : 0)) * KP + g * G;
+        if constexpr (FAR) {
 #pragma unroll
-        for (int u4 = 0; u4 < G / 4; u4++) {
-            f32x4 b = boxok ? *reinterpret_cast<const f32x4 *>(Lv + 4 * u4) : f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int u4 = 0; u4 < G / 4; u4++) {
+                f32x4 b = boxok ? *reinterpret_cast<const f32x4 *>(Lv + 4 * u4) : f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int r = 0; r < RT; r++) {
-                const f32x4 a = *reinterpret_cast<const f32x4 *>(l2pfrag + ((long)r * 64 + lane) * G + 4 * u4);
+                for (int r = 0; r < RT; r++) {
+                    const f32x4 a = *reinterpret_cast<const f32x4 *>(l2pfrag + ((long)r * 64 + lane) * G + 4 * u4);
 #pragma unroll
-                for (int u = 0; u < 4; u++) acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], acc[r], 0, 0, 0);
+                    for (int u = 0; u < 4; u++) acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], acc[r], 0, 0, 0);
+                }
             }
         }
         // D row 4g + q of tile r = cell 16 r + 4g + q of box j16: 4 consecutive x cells
@@ -878,7 +881,7 @@ __global__ __launch_bounds__(256) void p2p_l2p_mfma_kernel(const float *__restri
 // stride of 2, three of the four operators of the next row are the previous
 // row's: a four-deep register window, one new operator per row.
 // L2P is fused as in the FP32 kernel.
-template <int K, int KP, int NCB>
+template <int K, int KP, int NCB, bool FAR>
 __global__ __launch_bounds__(256) void p2p_l2p_bf16_kernel(const float *__restrict__ rho, float *__restrict__ phi,
                                                            const float *__restrict__ Lleaf,
                                                            const float *__restrict__ l2pfrag,
@@ -967,15 +970,17 @@ __global__ __launch_bounds__(256) void p2p_l2p_bf16_kernel(const float *__restri
             const int bxg = blockIdx.x * 4 * NCB + jx, byg = blockIdx.y * 4 + jy;
             const bool boxok = (bxg < nb && byg < nb);
             const float *Lv = Lleaf + (((long)zt * nb + (boxok ? byg : 0)) * nb + (boxok ? bxg : 0)) * KP + g * G;
+            if constexpr (FAR) {
 #pragma unroll
-            for (int u4 = 0; u4 < G / 4; u4++) {
-                f32x4 b = boxok ? *reinterpret_cast<const f32x4 *>(Lv + 4 * u4) : f32x4{0.f, 0.f, 0.f, 0.f};
+                for (int u4 = 0; u4 < G / 4; u4++) {
+                    f32x4 b = boxok ? *reinterpret_cast<const f32x4 *>(Lv + 4 * u4) : f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int r = 0; r < RT; r++) {
-                    const f32x4 a = *reinterpret_cast<const f32x4 *>(l2pfrag + ((long)r * 64 + lane) * G + 4 * u4);
+                    for (int r = 0; r < RT; r++) {
+                        const f32x4 a = *reinterpret_cast<const f32x4 *>(l2pfrag + ((long)r * 64 + lane) * G + 4 * u4);
 #pragma unroll
-                    for (int u = 0; u < 4; u++)
-                        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], acc[r][c], 0, 0, 0);
+                        for (int u = 0; u < 4; u++)
+                            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], acc[r][c], 0, 0, 0);
+                    }
                 }
             }
             // D row 4g + q of tile r = cell 16 r + 4g + q of box j16: 4 consecutive x cells
@@ -993,6 +998,59 @@ __global__ __launch_bounds__(256) void p2p_l2p_bf16_kernel(const float *__restri
                         if (xb + q < xd) dst[q] = acc[r][c][q] * scale;
                 }
             }
+        }
+    }
+}
+
+// Far field added to the near field: phi[cells of leaf box] += scale *
+// L2P[W*W x KP] * L[KP x 16 boxes] (FP32 MFMA), for leaf widths 4 and 8.
+// Wave w of the block takes leaves (4 blockIdx.x + w) * 16 .. + 15 of leaf row
+// blockIdx.y; grid z = film * zd + layer.
+template <int K, int KP, int W>
+__global__ __launch_bounds__(256) void l2p_add_kernel(float *__restrict__ phi, const float *__restrict__ Lleaf,
+                                                      const float *__restrict__ l2pfrag, int xd, int yd, int zd,
+                                                      int nb, float scale, FilmSet fs) {
+    constexpr int G = KP / 4, RT = W * W / 16;
+    const int f = blockIdx.z / zd, z = blockIdx.z % zd;
+    if (fs.desc) {
+        const FilmDesc d = fs.desc[f];
+        xd = d.xd; yd = d.yd; phi += d.off;
+    }
+    Lleaf += (long)f * fs.stride;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j16 = lane & 15, g = lane >> 4;
+    const int bx0 = (blockIdx.x * 4 + wave) * 16, by = blockIdx.y;
+    if (bx0 * W >= xd || by * W >= yd) return;  // wave-uniform: no leaf of this wave holds film cells
+    const int bx = bx0 + j16;
+    const bool boxok = bx < nb;
+    const float *Lv = Lleaf + (((long)z * nb + by) * nb + (boxok ? bx : 0)) * KP + g * G;
+    f32x4 acc[RT];
+#pragma unroll
+    for (int r = 0; r < RT; r++) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int u4 = 0; u4 < G / 4; u4++) {
+        const f32x4 b = boxok ? *reinterpret_cast<const f32x4 *>(Lv + 4 * u4) : f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int r = 0; r < RT; r++) {
+            const f32x4 a = *reinterpret_cast<const f32x4 *>(l2pfrag + ((long)r * 64 + lane) * G + 4 * u4);
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], acc[r], 0, 0, 0);
+        }
+    }
+    const long plane = (long)xd * yd;
+    // D row 4g + q of tile r = cell 16 r + 4g + q of box j16: 4 consecutive x cells
+#pragma unroll
+    for (int r = 0; r < RT; r++) {
+        const int c = 16 * r + 4 * g;
+        const int y = by * W + c / W, xb = bx * W + c % W;
+        if (!boxok || y >= yd) continue;
+        float *dst = phi + (long)z * plane + (long)y * xd + xb;
+        if (xb + 3 < xd && (((long)z * plane + (long)y * xd + xb) & 3) == 0) {
+            f32x4 v = *reinterpret_cast<const f32x4 *>(dst);
+            *reinterpret_cast<f32x4 *>(dst) = v + acc[r] * scale;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (xb + q < xd) dst[q] += acc[r][q] * scale;
         }
     }
 }
@@ -1051,21 +1109,65 @@ int launch_l2l_sum(const FmmPlan &pl, int L, hipStream_t s) {
     return 0;
 }
 
-template <int K, int KP, int NCB>
-int launch_p2p_bf16(const Grid &g, const FmmPlan &pl, const float *rho, float *phi, float scale, hipStream_t s) {
+// Dynamic LDS asked by a near-field launch that runs beside the multipole
+// passes: at least 82 KB, so one near-field workgroup and one M2L workgroup
+// (77 KB) share a CU instead of two near-field workgroups filling it.
+constexpr size_t kNearLdsShared = 82 * 1024;
+
+template <int K, int KP, int NCB, bool FAR>
+int launch_p2p_bf16(const Grid &g, const FmmPlan &pl, const float *rho, float *phi, float scale, bool shared,
+                    hipStream_t s) {
     constexpr int SP = 32 * NCB + 16 + 8;
-    const size_t lds = sizeof(unsigned short) * 2 * (size_t)g.zd * 48 * SP;
+    size_t lds = sizeof(unsigned short) * 2 * (size_t)g.zd * 48 * SP;
+    if (shared && lds < kNearLdsShared) lds = kNearLdsShared;
     MMS_CHECK(lds <= 160 * 1024, "FMM near field: too many layers for the LDS tile");
     static int attr = 0;
     if (!attr) {
-        MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&p2p_l2p_bf16_kernel<K, KP, NCB>),
+        MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&p2p_l2p_bf16_kernel<K, KP, NCB, FAR>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         attr = 1;
     }
     dim3 grid((g.xd + 32 * NCB - 1) / (32 * NCB), (g.yd + 31) / 32, pl.nfilm);
-    hipLaunchKernelGGL((p2p_l2p_bf16_kernel<K, KP, NCB>), grid, dim3(256), lds, s, rho, phi,
+    hipLaunchKernelGGL((p2p_l2p_bf16_kernel<K, KP, NCB, FAR>), grid, dim3(256), lds, s, rho, phi,
                        pl.loc + pl.lvl_off[pl.Lleaf], pl.d_l2pf, pl.d_p2pA, g.xd, g.yd, g.zd, 1 << pl.Lleaf, scale,
                        pl.films());
+    MMS_HIP(hipGetLastError());
+    return 0;
+}
+
+// Near field (+ L2P when FAR; leaf width 2 always adds it).  FAR is a
+// template parameter: a run-time branch around the L2P products made the
+// compiler rotate the near-field accumulators across the loop back edge.
+template <int K, int KP, int W, bool FAR>
+int launch_near(const Grid &g, const FmmPlan &pl, const float *rho, float *phi, float scale, bool shared,
+                hipStream_t s) {
+    const int zd = g.zd;
+    if constexpr (W == 8) {
+        if (pl.precision != MMS_M2L_FP32) return launch_p2p_bf16<K, KP, 2, FAR>(g, pl, rho, phi, scale, shared, s);
+    }
+    if constexpr (W == 4 || W == 8) {
+        constexpr int TW = 4 * W, SW = TW + 2 * W, TS = 4 * W - 1;
+        size_t lds = sizeof(float) * ((size_t)zd * SW * SW + (size_t)(2 * zd - 1) * TS * TS);
+        if (shared && lds < kNearLdsShared) lds = kNearLdsShared;
+        static int attr = 0;
+        if (!attr) {
+            MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&p2p_l2p_mfma_kernel<K, KP, W, FAR>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            attr = 1;
+        }
+        MMS_CHECK(lds <= 160 * 1024, "FMM near field: too many layers for the LDS tile");
+        dim3 grid((g.xd + TW - 1) / TW, (g.yd + TW - 1) / TW, pl.nfilm);
+        hipLaunchKernelGGL((p2p_l2p_mfma_kernel<K, KP, W, FAR>), grid, dim3(256), lds, s, rho, phi,
+                           pl.loc + pl.lvl_off[pl.Lleaf], pl.d_l2pf, pl.d_near, g.xd, g.yd, zd, 1 << pl.Lleaf,
+                           scale, pl.films());
+    } else {
+        constexpr int TW = 16, SW = TW + 2 * W, TS = 4 * W - 1;
+        const size_t lds = sizeof(float) * ((size_t)zd * SW * SW + (size_t)(2 * zd - 1) * TS * TS);
+        dim3 grid((g.xd + TW - 1) / TW, (g.yd + TW - 1) / TW, zd * pl.nfilm);
+        hipLaunchKernelGGL((p2p_l2p_kernel<K, KP, W>), grid, dim3(256), lds, s, rho, phi,
+                           pl.loc + pl.lvl_off[pl.Lleaf], pl.d_l2p, pl.d_near, g.xd, g.yd, zd, 1 << pl.Lleaf, scale,
+                           pl.films());
+    }
     MMS_HIP(hipGetLastError());
     return 0;
 }
@@ -1082,6 +1184,24 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
     if (pl.Lleaf - D < 2) D = pl.Lleaf - 2;
     const int R = pl.Lleaf - D, nR = 1 << R;
     const int Lc = R - 1 < 4 ? R - 1 : 4;  // coarse levels 2..Lc in one workgroup per layer
+    const float scale = (float)((double)(g.dx * g.dy * g.dz) / (double)g.dx);
+    // Leaf widths 4 and 8: the near field (no L2P) writes phi first, on the
+    // plan's second stream concurrently with the multipole passes unless a
+    // profiling window wants one kernel at a time; L2P adds the far field
+    // after the join.  Leaf width 2 keeps the fused near field + L2P at the end.
+    constexpr bool split = (W == 4 || W == 8);
+    const bool conc = split && pl.s_near && !(prof && prof->on);
+    if constexpr (split) {
+        if (conc) {
+            MMS_HIP(hipEventRecord(pl.ev_fork, s));
+            MMS_HIP(hipStreamWaitEvent(pl.s_near, pl.ev_fork, 0));
+            if (launch_near<K, KP, W, false>(g, pl, rho, phi, scale, true, pl.s_near)) return 1;
+            MMS_HIP(hipEventRecord(pl.ev_join, pl.s_near));
+        } else {
+            KTimer kt(prof, K_P2P_L2P);
+            if (launch_near<K, KP, W, false>(g, pl, rho, phi, scale, false, s)) return 1;
+        }
+    }
     {
         KTimer kt(prof, K_P2M);
         const int S = 1 << D, CS = S * W;
@@ -1157,40 +1277,17 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
                            pl.film_stride * pl.nfilm, pl.loc, pl.d_l2l, lo, R, D, zd, pl.film_stride);
         MMS_HIP(hipGetLastError());
     }
-    {
-        KTimer kt(prof, K_P2P_L2P);
-        const float scale = (float)((double)(g.dx * g.dy * g.dz) / (double)g.dx);
-        bool done = false;
-        if constexpr (W == 8) {
-            if (pl.precision != MMS_M2L_FP32) {
-                if (launch_p2p_bf16<K, KP, 2>(g, pl, rho, phi, scale, s)) return 1;
-                done = true;
-            }
-        }
-        if (done) {
-        } else if constexpr (W == 4 || W == 8) {
-            constexpr int TW = 4 * W, SW = TW + 2 * W, TS = 4 * W - 1;
-            const size_t lds = sizeof(float) * ((size_t)zd * SW * SW + (size_t)(2 * zd - 1) * TS * TS);
-            static int attr = 0;
-            if (!attr) {
-                MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&p2p_l2p_mfma_kernel<K, KP, W>),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-                attr = 1;
-            }
-            MMS_CHECK(lds <= 160 * 1024, "FMM near field: too many layers for the LDS tile");
-            dim3 grid((g.xd + TW - 1) / TW, (g.yd + TW - 1) / TW, pl.nfilm);
-            hipLaunchKernelGGL((p2p_l2p_mfma_kernel<K, KP, W>), grid, dim3(256), lds, s, rho, phi,
-                               pl.loc + pl.lvl_off[pl.Lleaf], pl.d_l2pf, pl.d_near, g.xd, g.yd, zd, 1 << pl.Lleaf,
-                               scale, pl.films());
-        } else {
-            constexpr int TW = 16, SW = TW + 2 * W, TS = 4 * W - 1;
-            const size_t lds = sizeof(float) * ((size_t)zd * SW * SW + (size_t)(2 * zd - 1) * TS * TS);
-            dim3 grid((g.xd + TW - 1) / TW, (g.yd + TW - 1) / TW, zd * pl.nfilm);
-            hipLaunchKernelGGL((p2p_l2p_kernel<K, KP, W>), grid, dim3(256), lds, s, rho, phi,
-                               pl.loc + pl.lvl_off[pl.Lleaf], pl.d_l2p, pl.d_near, g.xd, g.yd, zd, 1 << pl.Lleaf,
-                               scale, pl.films());
-        }
+    if constexpr (split) {
+        if (conc) MMS_HIP(hipStreamWaitEvent(s, pl.ev_join, 0));
+        KTimer kt(prof, K_L2P);
+        const int nb = 1 << pl.Lleaf;
+        dim3 grid((nb + 63) / 64, nb, zd * pl.nfilm);
+        hipLaunchKernelGGL((l2p_add_kernel<K, KP, W>), grid, dim3(256), 0, s, phi, pl.loc + pl.lvl_off[pl.Lleaf],
+                           pl.d_l2pf, g.xd, g.yd, zd, nb, scale, pl.films());
         MMS_HIP(hipGetLastError());
+    } else {
+        KTimer kt(prof, K_P2P_L2P);
+        if (launch_near<K, KP, W, true>(g, pl, rho, phi, scale, false, s)) return 1;
     }
     return 0;
 }
@@ -1246,6 +1343,11 @@ int FmmPlan::init_batch(const Grid &g, int P_, int w_, int precision_, int cols_
 
 int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int nfilm_) {
     release();
+    if (w_ == 4 || w_ == 8) {  // second stream (default priority) and fork/join events for the near field
+        MMS_HIP(hipStreamCreateWithFlags(&s_near, hipStreamNonBlocking));
+        MMS_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        MMS_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    }
     nfilm = nfilm_ > 0 ? nfilm_ : 1;
     MMS_CHECK(P_ >= 1 && P_ <= 9, "FMM expansion order must be 1..9");
     MMS_CHECK(w_ == 2 || w_ == 4 || w_ == 8, "FMM leaf width must be 2, 4 or 8");
@@ -1393,6 +1495,12 @@ void FmmPlan::release() {
     if (d_films) (void)hipFree(d_films);
     d_films = nullptr;
     nfilm = 1;
+    if (s_near) (void)hipStreamSynchronize(s_near);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (s_near) (void)hipStreamDestroy(s_near);
+    s_near = nullptr;
+    ev_fork = ev_join = nullptr;
 }
 
 }  // namespace mms

@@ -42,7 +42,7 @@ struct Grid {
 // ------------------------------------------------------------- profiling --
 // Kernel classes timed with hipEvents when profiling is on.
 enum KClass {
-    K_DIV = 0, K_STAGE, K_DIRECT, K_DIRECT_REDUCE, K_P2M, K_M2M, K_M2L, K_L2L, K_P2P_L2P,
+    K_DIV = 0, K_STAGE, K_DIRECT, K_DIRECT_REDUCE, K_P2M, K_M2M, K_M2L, K_L2L, K_P2P_L2P, K_L2P,
     K_HFIELD, K_ENERGY, K_PROBES, K_MISC, K_COUNT
 };
 const char *kclass_name(int k);
@@ -115,6 +115,8 @@ struct FmmPlan {
     int nfilm = 1;           // films in the batch (1 = single film)
     long film_stride = 0;    // floats per film in mult/loc
     FilmDesc *d_films = nullptr;
+    hipStream_t s_near = nullptr;         // near field runs here, concurrently with the multipole passes
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     FilmSet films() const { return FilmSet{d_films, film_stride, zd}; }
     int init(const Grid &g, int P, int w, int precision, int cols = 8, int nfilm = 1);
     int init_batch(const Grid &g, int P, int w, int precision, int cols, const std::vector<FilmDesc> &films);

@@ -288,8 +288,9 @@ def bench_c5(args, mg, wl):
     films = wl.c5_films()
     mat = np.concatenate([f.material for f in films])
     M0 = np.concatenate([f.M for f in films])
-    b = mg.FilmBatch(films, 3, *films[0].d, solver=mg.SOLVER_FMM, P=4, material=mat, mu_0=wl.MU_0, Ms=wl.MS,
-                     Aexch=wl.AEXCH, alfa=wl.ALFA, gamma=wl.GAMMA)
+    prec = {"fp32": mg.M2L_FP32, "bf16": mg.M2L_BF16, "bf16x3": mg.M2L_BF16X3}[args.m2l]
+    b = mg.FilmBatch(films, 3, *films[0].d, solver=mg.SOLVER_FMM, P=4, m2l_precision=prec, material=mat,
+                     mu_0=wl.MU_0, Ms=wl.MS, Aexch=wl.AEXCH, alfa=wl.ALFA, gamma=wl.GAMMA)
     info = b.info()
     b.set_m(M0)
     tune = b.autotune(reps=2)
@@ -355,7 +356,8 @@ def bench_c5(args, mg, wl):
         "vs_baseline": None, "dtype": "f32 (FP64 RK4 combine)", "data": "synthetic (seeded SplitMix64, seed 2010)",
         "config": {"workload": "c5_films_512", "films": len(films), "cells": ncell,
                    "direct_films": info["direct_films"], "fmm_films": info["fmm_films"],
-                   "fmm_size_classes": info["fmm_size_classes"], "P": 4, "dt": float(dt), "graph": bool(args.graph),
+                   "fmm_size_classes": info["fmm_size_classes"], "P": 4, "m2l": args.m2l, "dt": float(dt),
+                   "graph": bool(args.graph),
                    "film_steps_per_s": round(sps * len(films), 1)},
         "roofline": roof,
         "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},

@@ -197,6 +197,53 @@ def test_fmm_m2l_precision_modes(mg, mode, tol, leaf):
     assert err < tol
 
 
+@pytest.mark.parametrize("leaf,prec", [(8, "bf16x3"), (8, "fp32"), (4, "fp32")])
+def test_concurrent_near_field_matches_sequential(mg, leaf, prec):
+    """The near field runs on a second stream beside the multipole passes
+    (fork/join) except inside a profiling window, where every kernel runs in
+    order on one stream: both orders give bitwise identical potentials, in
+    eager calls and in replayed step graphs."""
+    from mms_gpu import workloads as wl
+    w = wl.c3_domains(side=128, zd=4, domain=16)
+    p = {"fp32": mg.M2L_FP32, "bf16x3": mg.M2L_BF16X3}[prec]
+    ch = (np.random.default_rng(3).normal(size=w.n) * 1e14).astype(F)
+    with mg.Film(*w.dims, *w.d, solver=mg.SOLVER_FMM, P=6, m2l_precision=p, leaf_width=leaf,
+                 material=w.material) as f:
+        conc = f.potential(ch)
+        f.profile_begin()
+        seq = f.potential(ch)
+        f.profile_end()
+        assert np.array_equal(conc, seq)
+        f.set_m(w.M)
+        f.steps(3, w.dt, use_graph=True)
+        m_graph = f.get_m()
+    with mg.Film(*w.dims, *w.d, solver=mg.SOLVER_FMM, P=6, m2l_precision=p, leaf_width=leaf,
+                 material=w.material) as f:
+        f.set_m(w.M)
+        f.profile_begin()
+        f.steps(3, w.dt, use_graph=False)
+        f.profile_end()
+        m_seq = f.get_m()
+    assert np.array_equal(m_graph, m_seq)
+
+
+@pytest.mark.parametrize("P", [7, 9])
+def test_fmm_high_order_bf16_split(mg, P):
+    """P = 7..9 (64-coefficient operator blocks) in the BF16 split form at
+    leaf widths 4 and 8 against the FP64 FMM oracle."""
+    from oracle import fmm_oracle as fo
+    rng = np.random.default_rng(P)
+    xd, zd = 64, 3
+    q = (rng.normal(size=(zd, xd, xd)) * 1e14).astype(F)
+    for leaf in (4, 8):
+        with mg.Film(xd, xd, zd, *D5, solver=mg.SOLVER_FMM, P=P, m2l_precision=mg.M2L_BF16X3, leaf_width=leaf) as f:
+            pot = f.potential(q.ravel())
+        ref = fo.FmmOracle(xd, xd, zd, P, w=leaf).potential_units(q.astype(float)).ravel() * float(D5[0]) ** 2
+        err = _rel(pot, ref)
+        _say(f"P={P} leaf {leaf} bf16x3: rel err vs FP64 FMM {err:.2e}")
+        assert err < 2e-5
+
+
 def test_bf16_near_field_ragged_films(mg):
     """Leaf width 8 with the BF16 split near field on non-square,
     non-power-of-two films with zero and sign-alternating layers: matches

@@ -140,9 +140,20 @@ __global__ __launch_bounds__(256) void up_subtree_kernel(const float *__restrict
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j16 = lane & 15, g = lane >> 4;
     const int cx0 = rbx * CS, cy0 = rby * CS;
     float *q = lds;  // CS x CS charges, later the leaf multipoles [leaf][SB]
-    for (int e = threadIdx.x; e < CS * CS; e += 256) {
-        const int x = cx0 + e % CS, y = cy0 + e / CS;
-        q[e] = (x < xd && y < yd) ? rho[(long)z * plane + (long)y * xd + x] : 0.0f;
+#pragma unroll 1
+    for (int e0 = 0; e0 < CS * CS; e0 += 256 * 8) {  // 8 loads per thread in flight
+        float vb[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int e = e0 + threadIdx.x + 256 * k;
+            const int x = cx0 + e % CS, y = cy0 + e / CS;
+            vb[k] = (e < CS * CS && x < xd && y < yd) ? rho[(long)z * plane + (long)y * xd + x] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int e = e0 + threadIdx.x + 256 * k;
+            if (e < CS * CS) q[e] = vb[k];
+        }
     }
     float a[RT][GP];
 #pragma unroll
@@ -554,14 +565,29 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(const float *__restrict__
 #pragma unroll
     for (int k = 0; k < 3; k++) loadA(abuf[k], offs[k]);
 
-    // stage the source window of layer zs (zero outside the grid)
+    // stage the source window of layer zs (zero outside the grid), in
+    // batches of 8 loads per thread issued before any of them is used
     {
         const f32x4 *M4 = reinterpret_cast<const f32x4 *>(M);
-        for (int e = threadIdx.x; e < WX * WY * (KP / 4); e += 256) {
+        constexpr int NE = WX * WY * (KP / 4), BATCH = 8, NIT = (NE + 256 * BATCH - 1) / (256 * BATCH);
+#pragma unroll 1
+        for (int it = 0; it < NIT; it++) {
+        f32x4 vb[BATCH];
+#pragma unroll
+        for (int k = 0; k < BATCH; k++) {
+            const int e = threadIdx.x + 256 * (it * BATCH + k);
             const int box = e / (KP / 4), q = e % (KP / 4);
             const int gx = x0 - 3 + box % WX, gy = y0 - 3 + box / WX;
-            f32x4 v = f32x4{0.f, 0.f, 0.f, 0.f};
-            if (gx >= 0 && gx < nb && gy >= 0 && gy < nb) v = M4[(((long)zs * nb + gy) * nb + gx) * (KP / 4) + q];
+            vb[k] = f32x4{0.f, 0.f, 0.f, 0.f};
+            if (e < NE && gx >= 0 && gx < nb && gy >= 0 && gy < nb)
+                vb[k] = M4[(((long)zs * nb + gy) * nb + gx) * (KP / 4) + q];
+        }
+#pragma unroll
+        for (int k = 0; k < BATCH; k++) {
+            const int e = threadIdx.x + 256 * (it * BATCH + k);
+            if (e >= NE) break;
+            const int box = e / (KP / 4), q = e % (KP / 4);
+            const f32x4 v = vb[k];
             if constexpr (MODE == MMS_M2L_FP32) {
                 *reinterpret_cast<f32x4 *>(reinterpret_cast<float *>(smem) + box * SB + 4 * q) = v;
             } else {
@@ -579,6 +605,7 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(const float *__restrict__
                 if constexpr (MODE == MMS_M2L_BF16X3)
                     *reinterpret_cast<uint2 *>(hi + WX * WY * SB + box * SB + 4 * q) = lv2;
             }
+        }
         }
     }
     __syncthreads();
@@ -902,13 +929,30 @@ __global__ __launch_bounds__(256) void p2p_l2p_bf16_kernel(const float *__restri
     const int j16 = lane & 15, g = lane >> 4;
     const int x0 = blockIdx.x * TWX, y0 = blockIdx.y * TWY;
     const long plane = (long)xd * yd;
-    for (int e = threadIdx.x; e < zd * SWY * SWX; e += 256) {
-        const int zs = e / (SWY * SWX), r = e % (SWY * SWX), sy = r / SWX, sx = r % SWX;
-        const int gx = x0 - W + sx, gy = y0 - W + sy;
-        const float v = (gx >= 0 && gx < xd && gy >= 0 && gy < yd) ? rho[(long)zs * plane + (long)gy * xd + gx] : 0.0f;
-        const unsigned short h = f2bf(v);
-        qh[(zs * SWY + sy) * SP + sx] = h;
-        ql[(zs * SWY + sy) * SP + sx] = f2bf(v - bf2f(h));
+    {   // charges of the window, 16 loads per thread in flight per batch
+        constexpr int BATCH = 16;
+        const int ne = zd * SWY * SWX;
+#pragma unroll 1
+        for (int e0 = 0; e0 < ne; e0 += 256 * BATCH) {
+            float vb[BATCH];
+#pragma unroll
+            for (int k = 0; k < BATCH; k++) {
+                const int e = e0 + threadIdx.x + 256 * k;
+                const int zs = e / (SWY * SWX), r = e % (SWY * SWX), sy = r / SWX, sx = r % SWX;
+                const int gx = x0 - W + sx, gy = y0 - W + sy;
+                vb[k] = (e < ne && gx >= 0 && gx < xd && gy >= 0 && gy < yd) ? rho[(long)zs * plane + (long)gy * xd + gx]
+                                                                             : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < BATCH; k++) {
+                const int e = e0 + threadIdx.x + 256 * k;
+                if (e >= ne) break;
+                const int zs = e / (SWY * SWX), r = e % (SWY * SWX), sy = r / SWX, sx = r % SWX;
+                const unsigned short h = f2bf(vb[k]);
+                qh[(zs * SWY + sy) * SP + sx] = h;
+                ql[(zs * SWY + sy) * SP + sx] = f2bf(vb[k] - bf2f(h));
+            }
+        }
     }
     __syncthreads();
     const int jy = j16 >> 2;

@@ -290,6 +290,7 @@ __global__ __launch_bounds__(256) void down_subtree_kernel(const float *__restri
 #pragma unroll
             for (int r = 0; r < RT; r++) {
                 f32x4 v = acc[r];
+#pragma unroll 4
                 for (int s = 0; s < zd; s++) v += *reinterpret_cast<const f32x4 *>(Pf + s * part_stride + e + 16 * r + 4 * g);
                 if (D == 0) *reinterpret_cast<f32x4 *>(L + e + 16 * r + 4 * g) = v;
                 else *reinterpret_cast<f32x4 *>(pa + 16 * r + 4 * g) = v;
@@ -326,6 +327,7 @@ __global__ __launch_bounds__(256) void down_subtree_kernel(const float *__restri
 #pragma unroll
                 for (int r = 0; r < RT; r++) {
                     f32x4 v = acc[r];
+#pragma unroll 4
                     for (int zs = 0; zs < zd; zs++)
                         v += *reinterpret_cast<const f32x4 *>(Pf + zs * part_stride + e + 16 * r + 4 * g);
                     if (l == Lleaf) *reinterpret_cast<f32x4 *>(L + e + 16 * r + 4 * g) = v;
@@ -415,6 +417,7 @@ __global__ __launch_bounds__(256) void down_coarse_kernel(const float *__restric
 #pragma unroll
         for (int r = 0; r < RT; r++) {
             f32x4 v = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
             for (int zs = 0; zs < zd; zs++) v += *reinterpret_cast<const f32x4 *>(Pf + zs * part_stride + e + 16 * r + 4 * g);
             *reinterpret_cast<f32x4 *>(L + e + 16 * r + 4 * g) = v;
             *reinterpret_cast<f32x4 *>(b0 + j16 * SB + 16 * r + 4 * g) = v;
@@ -451,6 +454,7 @@ __global__ __launch_bounds__(256) void down_coarse_kernel(const float *__restric
 #pragma unroll
                 for (int r = 0; r < RT; r++) {
                     f32x4 v = acc[r];
+#pragma unroll 4
                     for (int zs = 0; zs < zd; zs++)
                         v += *reinterpret_cast<const f32x4 *>(Pf + zs * part_stride + e + 16 * r + 4 * g);
                     *reinterpret_cast<f32x4 *>(L + e + 16 * r + 4 * g) = v;
@@ -731,6 +735,7 @@ __global__ __launch_bounds__(256) void l2l_sum_kernel(const float *__restrict__ 
 #pragma unroll
         for (int r = 0; r < RT; r++) {
             f32x4 v = acc[r][c];
+#pragma unroll 4
             for (int s = 0; s < zd; s++) v += *reinterpret_cast<const f32x4 *>(Pz + s * part_stride + e + 4 * (4 * r + g));
             *reinterpret_cast<f32x4 *>(L + e + 4 * (4 * r + g)) = v;
         }
@@ -826,12 +831,40 @@ __global__ __launch_bounds__(256) void p2p_l2p_mfma_kernel(const float *__restri
     const int j16 = lane & 15, g = lane >> 4;
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TW;
     const long plane = (long)xd * yd;
-    for (int e = threadIdx.x; e < zd * SW * SW; e += 256) {
-        const int zs = e / (SW * SW), r = e % (SW * SW), sy = r / SW, sx = r % SW;
-        const int gx = x0 - W + sx, gy = y0 - W + sy;
-        src[e] = (gx >= 0 && gx < xd && gy >= 0 && gy < yd) ? rho[(long)zs * plane + (long)gy * xd + gx] : 0.0f;
+    {   // charges of the window and the 1/r table, 8 loads per thread in flight per batch
+        const int ne = zd * SW * SW, nt = (2 * zd - 1) * TS * TS;
+#pragma unroll 1
+        for (int e0 = 0; e0 < ne; e0 += 256 * 8) {
+            float vb[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int e = e0 + threadIdx.x + 256 * k;
+                const int zs = e / (SW * SW), r = e % (SW * SW), sy = r / SW, sx = r % SW;
+                const int gx = x0 - W + sx, gy = y0 - W + sy;
+                vb[k] = (e < ne && gx >= 0 && gx < xd && gy >= 0 && gy < yd) ? rho[(long)zs * plane + (long)gy * xd + gx]
+                                                                             : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int e = e0 + threadIdx.x + 256 * k;
+                if (e < ne) src[e] = vb[k];
+            }
+        }
+#pragma unroll 1
+        for (int e0 = 0; e0 < nt; e0 += 256 * 8) {
+            float vb[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int e = e0 + threadIdx.x + 256 * k;
+                vb[k] = e < nt ? table[e] : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int e = e0 + threadIdx.x + 256 * k;
+                if (e < nt) tab[e] = vb[k];
+            }
+        }
     }
-    for (int e = threadIdx.x; e < (2 * zd - 1) * TS * TS; e += 256) tab[e] = table[e];
     __syncthreads();
     // B column j16: box (jx, jy) of the block; A row j16 of tile r: cell 16 r + j16
     const int jx = j16 & 3, jy = j16 >> 2;

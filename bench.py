@@ -383,9 +383,13 @@ def main():
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--workload", default="c3", choices=["c3", "c5"],
                     help="c3: the 1024x1024x4 film (default); c5: the 512-film sweep")
+    ap.add_argument("--serial", action="store_true",
+                    help="run every kernel in order on one stream (MMS_GPU_SERIAL=1), e.g. under a kernel trace")
     ap.add_argument("--pmc-calibration", action="store_true",
                     help="end with plain device copies of 256 MiB and 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.serial:
+        os.environ["MMS_GPU_SERIAL"] = "1"
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(see DESIGN.md, 'How the path would shard')", file=sys.stderr)
@@ -478,6 +482,7 @@ def main():
         "dtype": "f32" if args.m2l == "fp32" else f"f32 ({args.m2l} M2L operands)",
         "data": "synthetic (seeded SplitMix64 domain pattern, seed 2010)",
         "config": {"workload": w.name, "form": "plain GPU FMM (no surrogate)", "cells": w.n, "P": args.P,
+                   "near_field_stream": "serial" if args.serial else "concurrent",
                    "leaf_width": args.leaf, "m2l": args.m2l, "dt": float(w.dt), "graph": bool(args.graph),
                    "precision_split": "FP32 fields and stage M, FP64 RK4 combine and renormalisation"},
         "roofline": roof,

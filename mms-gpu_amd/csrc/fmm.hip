@@ -1420,7 +1420,10 @@ int FmmPlan::init_batch(const Grid &g, int P_, int w_, int precision_, int cols_
 
 int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int nfilm_) {
     release();
-    if (w_ == 4 || w_ == 8) {  // second stream (default priority) and fork/join events for the near field
+    // second stream (default priority) and fork/join events for the near field;
+    // MMS_GPU_SERIAL=1 keeps every kernel on the caller's stream, in order
+    const char *serial = std::getenv("MMS_GPU_SERIAL");
+    if ((w_ == 4 || w_ == 8) && !(serial && serial[0] == '1')) {
         MMS_HIP(hipStreamCreateWithFlags(&s_near, hipStreamNonBlocking));
         MMS_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
         MMS_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));

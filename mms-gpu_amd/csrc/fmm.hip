@@ -542,7 +542,10 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(const float *__restrict__
 #pragma unroll
         for (int c = 0; c < NC; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    const int dzi = zt - zs + zd - 1;
+    // only n + m even coefficients are kept, for which the M2L operator is even
+    // in dz: the blocks of +dz and -dz are equal, so only the dz >= 0 half is
+    // read (half the operator data through L2)
+    const int dzi = (zt > zs ? zt - zs : zs - zt) + zd - 1;
     const unsigned char *offs = c_m2l_offs[__builtin_amdgcn_readfirstlane(wave)];
     // operator fragments of the first three offsets, in flight while the window is staged
     constexpr int NP = (MODE == MMS_M2L_BF16X3) ? 2 : 1;
@@ -999,7 +1002,7 @@ __global__ __launch_bounds__(256) void p2p_l2p_bf16_kernel(const float *__restri
             for (int c = 0; c < NCB; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 1
         for (int zs = 0; zs < zd; zs++) {
-            const int dzi = zs - zt + zd - 1;
+            const int dzi = (zs > zt ? zs - zt : zt - zs) + zd - 1;  // 1/r is even in dz
             const unsigned short *Ab = afrag + (long)dzi * 31 * 2 * 64 * 8 + lane * 8;
             auto loadA = [&](bf16x8 &h, bf16x8 &l, int dy) {
                 h = *reinterpret_cast<const bf16x8 *>(Ab + (long)(dy * 2) * 64 * 8);

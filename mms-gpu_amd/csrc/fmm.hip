@@ -510,21 +510,25 @@ struct M2LLevels {
 // layer zt and one source layer zs; the result is the zs partial of the
 // local expansions (summed over zs, with L2L from the parent, by
 // l2l_sum_kernel).
-template <int KP, int MODE, int NC>
-__global__ __launch_bounds__(256) void m2l_mfma_kernel(const float *__restrict__ Mall, float *__restrict__ Part,
-                                                       const char *__restrict__ Tall, M2LLevels lv, int zd,
-                                                       long film_stride, long part_stride) {
+template <int KP, int MODE, int NC, int ZP>
+__global__ __launch_bounds__(256 * ZP) void m2l_mfma_kernel(const float *__restrict__ Mall, float *__restrict__ Part,
+                                                            const char *__restrict__ Tall, M2LLevels lv, int zd,
+                                                            long film_stride, long part_stride) {
+    // ZP target layers share the staged source window: waves 4 zl .. 4 zl + 3
+    // take target layer zt = ZP * zg + zl
     typedef M2LCfg<KP, MODE, NC> C;
     int li = 0;
     while (li + 1 < lv.n && (int)blockIdx.x >= lv.tile0[li + 1]) li++;
     const int tl = blockIdx.x - lv.tile0[li], nb = lv.nb[li];
-    const int zz = blockIdx.z, film = zz / (zd * zd), zt = (zz / zd) % zd, zs = zz % zd;
+    const int nzg = (zd + ZP - 1) / ZP, zz = blockIdx.z;
+    const int zs = zz % zd, film = zz / (zd * nzg), zt = ((zz / zd) % nzg) * ZP + (int)(threadIdx.x >> 8);
+    const int ztc = zt < zd ? zt : zd - 1;  // operator loads of a padding wave stay in bounds
     const float *M = Mall + film * film_stride + lv.off[li];
     float *Lz = Part + zs * part_stride + film * film_stride + lv.off[li];
     const char *Tfrag = Tall + lv.toff[li];
     constexpr int G = C::G, RT = C::RT, SB = C::SB, SX = C::SX, WX = C::WX, WY = C::WY;
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) & 3;
     const int px = wave & 1, py = wave >> 1;
     const int x0 = (tl % lv.ntx[li]) * C::TX, y0 = (tl / lv.ntx[li]) * 16;
     const int j16 = lane & 15, g = lane >> 4;
@@ -545,7 +549,7 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(const float *__restrict__
     // only n + m even coefficients are kept, for which the M2L operator is even
     // in dz: the blocks of +dz and -dz are equal, so only the dz >= 0 half is
     // read (half the operator data through L2)
-    const int dzi = (zt > zs ? zt - zs : zs - zt) + zd - 1;
+    const int dzi = (ztc > zs ? ztc - zs : zs - ztc) + zd - 1;
     const unsigned char *offs = c_m2l_offs[__builtin_amdgcn_readfirstlane(wave)];
     // operator fragments of the first three offsets, in flight while the window is staged
     constexpr int NP = (MODE == MMS_M2L_BF16X3) ? 2 : 1;
@@ -576,13 +580,13 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(const float *__restrict__
     // batches of 8 loads per thread issued before any of them is used
     {
         const f32x4 *M4 = reinterpret_cast<const f32x4 *>(M);
-        constexpr int NE = WX * WY * (KP / 4), BATCH = 8, NIT = (NE + 256 * BATCH - 1) / (256 * BATCH);
+        constexpr int NT = 256 * ZP, NE = WX * WY * (KP / 4), BATCH = 8, NIT = (NE + NT * BATCH - 1) / (NT * BATCH);
 #pragma unroll 1
         for (int it = 0; it < NIT; it++) {
         f32x4 vb[BATCH];
 #pragma unroll
         for (int k = 0; k < BATCH; k++) {
-            const int e = threadIdx.x + 256 * (it * BATCH + k);
+            const int e = threadIdx.x + NT * (it * BATCH + k);
             const int box = e / (KP / 4), q = e % (KP / 4);
             const int gx = x0 - 3 + box % WX, gy = y0 - 3 + box / WX;
             vb[k] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -591,7 +595,7 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(const float *__restrict__
         }
 #pragma unroll
         for (int k = 0; k < BATCH; k++) {
-            const int e = threadIdx.x + 256 * (it * BATCH + k);
+            const int e = threadIdx.x + NT * (it * BATCH + k);
             if (e >= NE) break;
             const int box = e / (KP / 4), q = e % (KP / 4);
             const f32x4 v = vb[k];
@@ -616,6 +620,7 @@ __global__ __launch_bounds__(256) void m2l_mfma_kernel(const float *__restrict__
         }
     }
     __syncthreads();
+    if (zt >= zd) return;  // padding waves of the last layer group (after the only barrier)
 
     // 27 offsets, three operator blocks in flight (ring of registers)
 #pragma unroll 1
@@ -1139,9 +1144,12 @@ __global__ __launch_bounds__(256) void l2p_add_kernel(float *__restrict__ phi, c
 template <int KP, int MODE, int NC>
 int launch_m2l(const FmmPlan &pl, hipStream_t s) {
     typedef M2LCfg<KP, MODE, NC> C;
+    // target layers per staged source window (two for 32-coefficient blocks;
+    // 64-coefficient blocks need the registers of a 256-thread workgroup)
+    constexpr int ZP = (KP == 32) ? 2 : 1;
     static int attr_done = 0;
     if (!attr_done) {
-        MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&m2l_mfma_kernel<KP, MODE, NC>),
+        MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&m2l_mfma_kernel<KP, MODE, NC, ZP>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::lds_bytes));
         attr_done = 1;
     }
@@ -1160,9 +1168,9 @@ int launch_m2l(const FmmPlan &pl, hipStream_t s) {
         tiles += lv.ntx[i] * ((nb + 15) / 16);
     }
     lv.tile0[lv.n] = tiles;
-    const long zz = (long)pl.nfilm * pl.zd * pl.zd;
+    const long zz = (long)pl.nfilm * ((pl.zd + ZP - 1) / ZP) * pl.zd;
     MMS_CHECK(zz <= 65535, "FMM batch too large for one M2L launch (films x layers^2 > 65535)");
-    hipLaunchKernelGGL((m2l_mfma_kernel<KP, MODE, NC>), dim3(tiles, 1, (unsigned)zz), dim3(256), (C::lds_bytes), s,
+    hipLaunchKernelGGL((m2l_mfma_kernel<KP, MODE, NC, ZP>), dim3(tiles, 1, (unsigned)zz), dim3(256 * ZP), (C::lds_bytes), s,
                        pl.mult, pl.part, reinterpret_cast<const char *>(pl.d_m2l), lv, pl.zd, pl.film_stride,
                        pl.film_stride * pl.nfilm);
     MMS_HIP(hipGetLastError());

@@ -66,12 +66,12 @@ def p2p_pairs_per_eval(side, zd, w):
 def p2p_flops_per_eval(side, zd, w, m2l):
     """MFMA work of the near-field kernel (L2P runs in its own kernel).
     FP32 form: one MAC per (target, window source) pair -- the 1/r table
-    lookup replaces the distance arithmetic.  BF16 split form (leaf 8): per
-    target cell, source row and layer one 32-wide k step (24 sources + 8 zero
-    pad) in three BF16 products.  Returns (flops, peak TFLOP/s)."""
+    lookup replaces the distance arithmetic.  BF16 split form (leaf 8): the
+    24 x 24 source window of a target is packed into 18 k steps of 32 with no
+    padding, three BF16 products each.  Returns (flops, peak TFLOP/s)."""
     cells = side * side * zd
     if w == 8 and m2l != "fp32":
-        return 2 * cells * zd * (3 * w) * 32 * 3, BF16_PEAK_TFLOPS
+        return 2 * cells * zd * (3 * w) * (3 * w) * 3, BF16_PEAK_TFLOPS
     return 2 * cells * zd * 9 * w * w, FP32_PEAK_TFLOPS
 
 

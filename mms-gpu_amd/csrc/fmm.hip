@@ -997,8 +997,6 @@ __global__ __launch_bounds__(256) void p2p_l2p_bf16_kernel(const float *__restri
     }
     __syncthreads();
     const int jy = j16 >> 2;
-    const bool kpad = (g == 3);  // sources 24..31 of the 32-wide k step
-    const bf16x8 zero8 = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
     for (int zt = wave; zt < zd; zt += 4) {
         f32x4 acc[RT][NCB];
 #pragma unroll
@@ -1007,46 +1005,67 @@ __global__ __launch_bounds__(256) void p2p_l2p_bf16_kernel(const float *__restri
             for (int c = 0; c < NCB; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 1
         for (int zs = 0; zs < zd; zs++) {
+            // products of this source layer go to fresh accumulators that are
+            // added to acc at the end of the iteration: no MFMA accumulator is
+            // carried across the loop's back edge (see hazard_lint.py)
+            f32x4 accz[RT][NCB];
+#pragma unroll
+            for (int r = 0; r < RT; r++)
+#pragma unroll
+                for (int c = 0; c < NCB; c++) accz[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
             const int dzi = (zs > zt ? zs - zt : zt - zs) + zd - 1;  // 1/r is even in dz
-            const unsigned short *Ab = afrag + (long)dzi * 31 * 2 * 64 * 8 + lane * 8;
-            auto loadA = [&](bf16x8 &h, bf16x8 &l, int dy) {
-                h = *reinterpret_cast<const bf16x8 *>(Ab + (long)(dy * 2) * 64 * 8);
-                l = *reinterpret_cast<const bf16x8 *>(Ab + (long)(dy * 2 + 1) * 64 * 8);
+            const unsigned short *Ab = afrag + (long)dzi * 42 * 2 * 64 * 8 + lane * 8;
+            auto loadA = [&](bf16x8 &h, bf16x8 &l, int op) {
+                h = *reinterpret_cast<const bf16x8 *>(Ab + (long)(op * 2) * 64 * 8);
+                l = *reinterpret_cast<const bf16x8 *>(Ab + (long)(op * 2 + 1) * 64 * 8);
             };
-            const int boff = (zs * SWY + W * jy) * SP + 8 * g;
+            const int bbase = (zs * SWY + W * jy) * SP;  // row 0 of this lane's box window in layer zs
 #pragma unroll
-            for (int s0 = 0; s0 < 2; s0++) {
-                // window: A[j] = operator of dy = sy + 7 - 2 j (pair p = j); An = next row's pair 0
-                bf16x8 Ah[4], Al[4], Anh, Anl;
+            for (int ph = 0; ph < 3; ph++) {
+                // k step (a, ph) = flat window sources 96 a + 32 ph .. + 31 (rows of 24 packed
+                // back to back): lane group g reads 8 sources of row 4 a + srow at column scol
+                const int fl = 32 * ph + 8 * g, srow = fl / 24, scol = fl % 24;
+                // window: A[p] = operator (ph, d = 4 a - 2 p), index 14 ph + 2 a - p + 3;
+                // An0 / An1 = pairs 0 / 1 of step a + 1
+                bf16x8 Ah[4], Al[4], An0h, An0l, An1h, An1l;
 #pragma unroll
-                for (int j = 0; j < 4; j++) loadA(Ah[j], Al[j], s0 + 7 - 2 * j);
-                loadA(Anh, Anl, s0 + 9);
+                for (int p = 0; p < 4; p++) loadA(Ah[p], Al[p], 14 * ph + 3 - p);
+                loadA(An0h, An0l, 14 * ph + 5);
+                loadA(An1h, An1l, 14 * ph + 4);
 #pragma unroll
-                for (int i = 0; i < 12; i++) {
-                    const int sy = s0 + 2 * i;
+                for (int aa = 0; aa < 6; aa++) {
+                    const int ro = bbase + (4 * aa + srow) * SP + scol;
                     bf16x8 bh[NCB], bl[NCB];
 #pragma unroll
                     for (int c = 0; c < NCB; c++) {
                         const int jx = 4 * c + (j16 & 3);
-                        bh[c] = kpad ? zero8 : *reinterpret_cast<const bf16x8 *>(qh + boff + sy * SP + W * jx);
-                        bl[c] = kpad ? zero8 : *reinterpret_cast<const bf16x8 *>(ql + boff + sy * SP + W * jx);
+                        bh[c] = *reinterpret_cast<const bf16x8 *>(qh + ro + W * jx);
+                        bl[c] = *reinterpret_cast<const bf16x8 *>(ql + ro + W * jx);
                     }
 #pragma unroll
                     for (int p = 0; p < 4; p++)
 #pragma unroll
                         for (int c = 0; c < NCB; c++) {
-                            acc[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[p], bh[c], acc[p][c], 0, 0, 0);
-                            acc[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[p], bl[c], acc[p][c], 0, 0, 0);
-                            acc[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al[p], bh[c], acc[p][c], 0, 0, 0);
+                            accz[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[p], bh[c], accz[p][c], 0, 0, 0);
+                            accz[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[p], bl[c], accz[p][c], 0, 0, 0);
+                            accz[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al[p], bh[c], accz[p][c], 0, 0, 0);
                         }
-                    if (i < 11) {  // slide: pair p of the next row uses pair p - 1's operator
-#pragma unroll
-                        for (int j = 3; j > 0; j--) { Ah[j] = Ah[j - 1]; Al[j] = Al[j - 1]; }
-                        Ah[0] = Anh; Al[0] = Anl;
-                        if (i < 10) loadA(Anh, Anl, sy + 11);
+                    if (aa < 5) {  // slide: pairs 2, 3 of step a + 1 use pairs 0, 1 of step a
+                        Ah[3] = Ah[1]; Al[3] = Al[1];
+                        Ah[2] = Ah[0]; Al[2] = Al[0];
+                        Ah[0] = An0h; Al[0] = An0l;
+                        Ah[1] = An1h; Al[1] = An1l;
+                        if (aa < 4) {
+                            loadA(An0h, An0l, 14 * ph + 2 * aa + 7);
+                            loadA(An1h, An1l, 14 * ph + 2 * aa + 6);
+                        }
                     }
                 }
             }
+#pragma unroll
+            for (int r = 0; r < RT; r++)
+#pragma unroll
+                for (int c = 0; c < NCB; c++) acc[r][c] += accz[r][c];
         }
 #pragma unroll
         for (int c = 0; c < NCB; c++) {
@@ -1551,25 +1570,29 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
             }
     if (upload((void **)&d_near, tab.data(), sizeof(float) * tab.size())) return 1;
     if (w == 8) {
-        // BF16 split near-field operators [2zd-1][dy 0..30][hi, lo][64 lanes][8]:
-        // lane (m = 8 r + ix, k group gg) holds G(dz, dy - r, 8 gg + t - ix) for
-        // t = 0..7, zero for sources 24..31 (the 32-wide k step pads the 24-cell row)
+        // BF16 split near-field operators [2zd-1][phase 0..2][d index 0..13][hi, lo][64 lanes][8].
+        // The 24 x 24 source window of a leaf is read as rows packed back to
+        // back, 32 sources per k step: step (a, ph) holds flat sources
+        // 96 a + 32 ph + k, i.e. row 4 a + (32 ph + k) / 24, column (32 ph + k) % 24.
+        // For target row pair p (rows 2 p + r, r = 0, 1) the operator depends
+        // on d = 4 a - 2 p only: lane (m = 8 r + ix, k group gg) holds, for
+        // k = 8 gg + t, G(dz, (32 ph + k) / 24 + d - r + 7, (32 ph + k) % 24 - ix + 7).
         const int nd = 2 * zd - 1, ts = near_side;
-        std::vector<unsigned short> af((size_t)nd * 31 * 2 * 64 * 8, 0);
+        std::vector<unsigned short> af((size_t)nd * 42 * 2 * 64 * 8, 0);
         for (int dzi = 0; dzi < nd; dzi++)
-            for (int dy = 0; dy < 31; dy++)
-                for (int lane = 0; lane < 64; lane++) {
-                    const int m = lane % 16, gg = lane / 16, r = m / 8, ix = m % 8;
-                    for (int t = 0; t < 8; t++) {
-                        const int sx = 8 * gg + t;
-                        float v = 0.0f;
-                        if (sx < 24 && dy - r >= 0) v = tab[((size_t)dzi * ts + (dy - r)) * ts + (sx - ix + 7)];
-                        const unsigned short h = host_bf16(v);
-                        const size_t o = (((size_t)(dzi * 31 + dy) * 2) * 64 + lane) * 8 + t;
-                        af[o] = h;
-                        af[o + 64 * 8] = host_bf16(v - host_bf2f(h));
+            for (int ph = 0; ph < 3; ph++)
+                for (int di = 0; di < 14; di++)
+                    for (int lane = 0; lane < 64; lane++) {
+                        const int m = lane % 16, gg = lane / 16, r = m / 8, ix = m % 8, d = 2 * di - 6;
+                        for (int t = 0; t < 8; t++) {
+                            const int fl = 32 * ph + 8 * gg + t, row = fl / 24 + d - r + 7, col = fl % 24 - ix + 7;
+                            const float v = tab[((size_t)dzi * ts + row) * ts + col];
+                            const unsigned short h = host_bf16(v);
+                            const size_t o = (((size_t)((dzi * 3 + ph) * 14 + di) * 2) * 64 + lane) * 8 + t;
+                            af[o] = h;
+                            af[o + 64 * 8] = host_bf16(v - host_bf2f(h));
+                        }
                     }
-                }
         if (upload((void **)&d_p2pA, af.data(), sizeof(unsigned short) * af.size())) return 1;
     }
     return 0;

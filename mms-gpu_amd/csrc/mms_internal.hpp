@@ -111,7 +111,7 @@ struct FmmPlan {
     size_t m2l_level_bytes = 0;
     float *d_near = nullptr; // near-field kernel table 1/r over offsets
     int near_side = 0;       // 4w-1
-    unsigned short *d_p2pA = nullptr;  // leaf width 8: BF16 hi/lo near-field operators [2zd-1][31][2][64][8]
+    unsigned short *d_p2pA = nullptr;  // leaf width 8: BF16 hi/lo near-field operators [2zd-1][3][14][2][64][8]
     int nfilm = 1;           // films in the batch (1 = single film)
     long film_stride = 0;    // floats per film in mult/loc
     FilmDesc *d_films = nullptr;

@@ -104,6 +104,9 @@ static int ctx_alloc(mms_ctx *c, const unsigned char *material) {
     } else {
         c->host_matl.assign(n, 1);
     }
+    c->all_magnetic = true;
+    for (long i = 0; i < n; i++)
+        if (!c->host_matl[i]) { c->all_magnetic = false; break; }
     return 0;
 }
 
@@ -136,7 +139,9 @@ int mms_ctx::one_step(float dt) {
             if (launch_div(g, in[s], rho, stream, &prof)) return 1;
             if (potential(rho, phi)) return 1;
         }
-        if (launch_stage(g, mat, s, (double)dt, phi, in[s], out[s], mn64, acc64, matl, t_dev, stream, &prof))
+        // an all-magnetic film skips the per-cell material byte
+        if (launch_stage(g, mat, s, (double)dt, phi, in[s], out[s], mn64, acc64, all_magnetic ? nullptr : matl,
+                         t_dev, stream, &prof))
             return 1;
     }
     if (p.external == MMS_EXT_RAMP && launch_add_time(t_dev, (double)dt, stream)) return 1;

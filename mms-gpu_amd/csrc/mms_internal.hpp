@@ -207,6 +207,7 @@ struct mms_ctx {
     int cur = 0;               // slot of m32 holding the current state
     double t_host = 0.0;
     std::vector<unsigned char> host_matl;
+    bool all_magnetic = true;       // every cell magnetic: stage kernels skip the material bytes
     mms::DirectPlan direct;
     mms::FmmPlan fmm;
     mms::Profiler prof;

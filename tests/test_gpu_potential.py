@@ -227,6 +227,25 @@ def test_concurrent_near_field_matches_sequential(mg, leaf, prec):
     assert np.array_equal(m_graph, m_seq)
 
 
+@pytest.mark.parametrize("zd", [1, 2])
+def test_fmm_thin_films(mg, zd):
+    """One and two layers (the M2L pairs target layers per workgroup, the
+    near field runs one wave per layer): every precision and leaf width
+    against the FP64 FMM oracle."""
+    from oracle import fmm_oracle as fo
+    rng = np.random.default_rng(10 + zd)
+    xd = 96
+    q = (rng.normal(size=(zd, xd, xd)) * 1e14).astype(F)
+    for leaf in (4, 8):
+        ref = fo.FmmOracle(xd, xd, zd, 6, w=leaf).potential_units(q.astype(float)).ravel() * float(D5[0]) ** 2
+        for prec, tol in ((mg.M2L_FP32, 2e-6), (mg.M2L_BF16X3, 2e-5)):
+            with mg.Film(xd, xd, zd, *D5, solver=mg.SOLVER_FMM, P=6, m2l_precision=prec, leaf_width=leaf) as f:
+                pot = f.potential(q.ravel())
+            err = _rel(pot, ref)
+            _say(f"zd={zd} leaf {leaf} prec {prec}: rel err vs FP64 FMM {err:.2e}")
+            assert err < tol
+
+
 @pytest.mark.parametrize("P", [7, 9])
 def test_fmm_high_order_bf16_split(mg, P):
     """P = 7..9 (64-coefficient operator blocks) in the BF16 split form at

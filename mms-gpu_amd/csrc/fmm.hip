@@ -1301,9 +1301,6 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
     if constexpr (split) {
         if (conc) {
             MMS_HIP(hipEventRecord(pl.ev_fork, s));
-            MMS_HIP(hipStreamWaitEvent(pl.s_near, pl.ev_fork, 0));
-            if (launch_near<K, KP, W, false>(g, pl, rho, phi, scale, true, pl.s_near)) return 1;
-            MMS_HIP(hipEventRecord(pl.ev_join, pl.s_near));
         } else {
             KTimer kt(prof, K_P2P_L2P);
             if (launch_near<K, KP, W, false>(g, pl, rho, phi, scale, false, s)) return 1;
@@ -1325,6 +1322,17 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
         hipLaunchKernelGGL((up_subtree_kernel<K, KP, W>), dim3(nR * nR, zd * pl.nfilm), dim3(256), lds, s, rho,
                            pl.mult, pl.d_p2mf, pl.d_m2m, lo, R, D, g.xd, g.yd, bufA, pl.films());
         MMS_HIP(hipGetLastError());
+    }
+    if constexpr (split) {
+        // the near field depends on the fork point only (recorded before the
+        // upward pass); it is issued after the upward pass so that a captured
+        // step graph keeps the multipole chain, the critical path, on the
+        // caller's queue and puts the near field on the second one
+        if (conc) {
+            MMS_HIP(hipStreamWaitEvent(pl.s_near, pl.ev_fork, 0));
+            if (launch_near<K, KP, W, false>(g, pl, rho, phi, scale, true, pl.s_near)) return 1;
+            MMS_HIP(hipEventRecord(pl.ev_join, pl.s_near));
+        }
     }
     {
         KTimer kt(prof, K_M2M);

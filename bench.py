@@ -204,11 +204,11 @@ def field_error(mg, f, w, nprobe=8192):
 
 
 # ---------------------------------------------------------------- main --
-PREC_NAMES = ("fp32", "bf16", "bf16x3")
+PREC_NAMES = ("fp32", "bf16", "bf16x3", "fp16")
 
 
 def make_film(mg, wl, w, P, m2l, leaf):
-    prec = {"fp32": mg.M2L_FP32, "bf16": mg.M2L_BF16, "bf16x3": mg.M2L_BF16X3}[m2l]
+    prec = {"fp32": mg.M2L_FP32, "bf16": mg.M2L_BF16, "bf16x3": mg.M2L_BF16X3, "fp16": mg.M2L_FP16}[m2l]
     f = mg.Film(*w.dims, *w.d, solver=mg.SOLVER_FMM, P=P, m2l_precision=prec, leaf_width=leaf,
                 mu_0=wl.MU_0, Ms=wl.MS, Aexch=wl.AEXCH, alfa=wl.ALFA, gamma=wl.GAMMA, material=w.material)
     f.set_m(w.M)
@@ -290,7 +290,7 @@ def bench_c5(args, mg, wl):
     films = wl.c5_films()
     mat = np.concatenate([f.material for f in films])
     M0 = np.concatenate([f.M for f in films])
-    prec = {"fp32": mg.M2L_FP32, "bf16": mg.M2L_BF16, "bf16x3": mg.M2L_BF16X3}[args.m2l]
+    prec = {"fp32": mg.M2L_FP32, "bf16": mg.M2L_BF16, "bf16x3": mg.M2L_BF16X3, "fp16": mg.M2L_FP16}[args.m2l]
     b = mg.FilmBatch(films, 3, *films[0].d, solver=mg.SOLVER_FMM, P=4, m2l_precision=prec, material=mat,
                      mu_0=wl.MU_0, Ms=wl.MS, Aexch=wl.AEXCH, alfa=wl.ALFA, gamma=wl.GAMMA)
     info = b.info()

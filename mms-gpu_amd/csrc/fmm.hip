@@ -31,8 +31,15 @@
 //   FP32    v_mfma_f32_16x16x4_f32            (FP32 operands and accumulation)
 //   BF16    v_mfma_f32_16x16x32_bf16          (BF16 operands, FP32 accumulation)
 //   BF16X3  hi/lo BF16 split of both operands, C += Th Mh + Th Ml + Tl Mh
+//   FP16    v_mfma_f32_16x16x32_f16           (FP16 operands, FP32 accumulation)
+//           with exact power-of-two scaling into FP16 range: operator row j
+//           and column k carry factors a_j, b_k (host equilibration), and each
+//           workgroup scales its staged window M_k / b_k by s = 2^e so that its
+//           largest entry lies in [2^14, 2^15); the accumulators are
+//           multiplied by 1 / (a_j s) before they are stored
 // L2L is always FP32.
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <type_traits>
@@ -44,6 +51,14 @@ namespace {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x4 mfma32(f16x8 a, f16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
 
 template <int K> struct KPad { static constexpr int KP = ((K + 31) / 32) * 32; };
 template <int P> struct NC { static constexpr int K = (P + 1) * (P + 2) / 2; };
@@ -513,7 +528,8 @@ struct M2LLevels {
 template <int KP, int MODE, int NC, int ZP>
 __global__ __launch_bounds__(256 * ZP) void m2l_mfma_kernel(const float *__restrict__ Mall, float *__restrict__ Part,
                                                             const char *__restrict__ Tall, M2LLevels lv, int zd,
-                                                            long film_stride, long part_stride) {
+                                                            long film_stride, long part_stride,
+                                                            const float *__restrict__ f16scale) {
     // ZP target layers share the staged source window: waves 4 zl .. 4 zl + 3
     // take target layer zt = ZP * zg + zl
     typedef M2LCfg<KP, MODE, NC> C;
@@ -554,7 +570,8 @@ __global__ __launch_bounds__(256 * ZP) void m2l_mfma_kernel(const float *__restr
     // operator fragments of the first three offsets, in flight while the window is staged
     constexpr int NP = (MODE == MMS_M2L_BF16X3) ? 2 : 1;
     constexpr int NK = (MODE == MMS_M2L_FP32) ? G / 4 : G / 8;  // 16-byte A loads per row tile and plane
-    typedef typename std::conditional<MODE == MMS_M2L_FP32, f32x4, bf16x8>::type AV;
+    typedef typename std::conditional<MODE == MMS_M2L_FP16, f16x8, bf16x8>::type BV;  // 8 x 16-bit operand
+    typedef typename std::conditional<MODE == MMS_M2L_FP32, f32x4, BV>::type AV;
     AV abuf[3][NP][RT][NK];
     const long lo_base = (long)RT * 64 * G * 49 * (2 * zd - 1);  // bf16 lo plane offset (elements)
     auto loadA = [&](AV (&a)[NP][RT][NK], int code) {
@@ -569,7 +586,7 @@ __global__ __launch_bounds__(256 * ZP) void m2l_mfma_kernel(const float *__restr
                         a[q][r][sk] = *reinterpret_cast<const f32x4 *>(
                             reinterpret_cast<const float *>(Tfrag) + ((fbase + r) * 64 + lane) * G + 4 * sk);
                     else
-                        a[q][r][sk] = *reinterpret_cast<const bf16x8 *>(
+                        a[q][r][sk] = *reinterpret_cast<const AV *>(
                             reinterpret_cast<const unsigned short *>(Tfrag) + q * lo_base + ((fbase + r) * 64 + lane) * G + 8 * sk);
                 }
     };
@@ -578,7 +595,84 @@ __global__ __launch_bounds__(256 * ZP) void m2l_mfma_kernel(const float *__restr
 
     // stage the source window of layer zs (zero outside the grid), in
     // batches of 8 loads per thread issued before any of them is used
-    {
+    float inv_s = 1.0f;  // FP16: 1 / (window scale)
+    if constexpr (MODE == MMS_M2L_FP16) {
+        // every thread stages the same 4 slots 4q .. 4q + 3 of its boxes
+        const f32x4 *M4 = reinterpret_cast<const f32x4 *>(M);
+        constexpr int NT = 256 * ZP, NQ = KP / 4, NE = WX * WY * NQ, NB = (NE + NT - 1) / NT;
+        static_assert(NT % NQ == 0, "staging slot must be fixed per thread");
+        const int q = threadIdx.x % NQ;
+        const f32x4 cs = *reinterpret_cast<const f32x4 *>(f16scale + KP + 4 * q);
+        auto ld = [&](int k) {  // k >= NB reads nothing and returns 0
+            const int e = threadIdx.x + NT * k, box = e / NQ;
+            const int gx = x0 - 3 + box % WX, gy = y0 - 3 + box / WX;
+            f32x4 v = f32x4{0.f, 0.f, 0.f, 0.f};
+            if (e < NE && gx >= 0 && gx < nb && gy >= 0 && gy < nb)
+                v = M4[(((long)zs * nb + gy) * nb + gx) * NQ + q] * cs;
+            return v;
+        };
+        auto st = [&](int k, f32x4 v, float sc) {
+            const int e = threadIdx.x + NT * k;
+            if (e >= NE) return;
+            const int box = e / NQ;
+            unsigned short h[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) h[t] = __builtin_bit_cast(unsigned short, (_Float16)(v[t] * sc));
+            uint2 hv;
+            hv.x = h[0] | ((unsigned)h[1] << 16);
+            hv.y = h[2] | ((unsigned)h[3] << 16);
+            *reinterpret_cast<uint2 *>(reinterpret_cast<unsigned short *>(smem) + box * SB + 4 * q) = hv;
+        };
+        // the whole window is held in registers when it is at most 8 loads
+        // per thread; otherwise it is read twice (maximum, then conversion)
+        constexpr bool HOLD = NB <= 8;
+        f32x4 vb[HOLD ? NB : 8];
+        float mx = 0.0f;
+        if constexpr (HOLD) {
+#pragma unroll
+            for (int k = 0; k < NB; k++) vb[k] = ld(k);
+#pragma unroll
+            for (int k = 0; k < NB; k++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) mx = fmaxf(mx, fabsf(vb[k][t]));
+        } else {
+#pragma unroll 1
+            for (int k0 = 0; k0 < NB; k0 += 8) {
+#pragma unroll
+                for (int k = 0; k < 8; k++) vb[k] = ld(k0 + k);
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+#pragma unroll
+                    for (int t = 0; t < 4; t++) mx = fmaxf(mx, fabsf(vb[k][t]));
+            }
+        }
+        // workgroup maximum: wave reduction, then one value per wave in LDS
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+        __shared__ float wmax[NT / 64];
+        if (lane == 0) wmax[threadIdx.x >> 6] = mx;
+        __syncthreads();
+        mx = wmax[0];
+#pragma unroll
+        for (int w = 1; w < NT / 64; w++) mx = fmaxf(mx, wmax[w]);
+        // s = 2^(14 - e) for 2^e <= max < 2^(e+1), clamped so s and 1/s stay normal
+        int es = 14 - ((int)((__float_as_uint(mx) >> 23) & 255u) - 127);
+        es = es > 100 ? 100 : (es < -100 ? -100 : es);
+        const float sc = __uint_as_float((unsigned)(es + 127) << 23);
+        inv_s = __uint_as_float((unsigned)(127 - es) << 23);
+        if constexpr (HOLD) {
+#pragma unroll
+            for (int k = 0; k < NB; k++) st(k, vb[k], sc);
+        } else {
+#pragma unroll 1
+            for (int k0 = 0; k0 < NB; k0 += 8) {
+#pragma unroll
+                for (int k = 0; k < 8; k++) vb[k] = ld(k0 + k);
+#pragma unroll
+                for (int k = 0; k < 8; k++) st(k0 + k, vb[k], sc);
+            }
+        }
+    } else {
         const f32x4 *M4 = reinterpret_cast<const f32x4 *>(M);
         constexpr int NT = 256 * ZP, NE = WX * WY * (KP / 4), BATCH = 8, NIT = (NE + NT * BATCH - 1) / (NT * BATCH);
 #pragma unroll 1
@@ -649,15 +743,15 @@ __global__ __launch_bounds__(256 * ZP) void m2l_mfma_kernel(const float *__restr
                 const unsigned short *hi = reinterpret_cast<const unsigned short *>(smem);
 #pragma unroll
                 for (int sk = 0; sk < NK; sk++) {
-                    bf16x8 bh[NC];
+                    BV bh[NC];
 #pragma unroll
                     for (int c = 0; c < NC; c++)
-                        bh[c] = *reinterpret_cast<const bf16x8 *>(hi + (tbox[c] + woff) * SB + g * G + 8 * sk);
+                        bh[c] = *reinterpret_cast<const BV *>(hi + (tbox[c] + woff) * SB + g * G + 8 * sk);
 #pragma unroll
                     for (int r = 0; r < RT; r++)
 #pragma unroll
                         for (int c = 0; c < NC; c++)
-                            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(abuf[k][0][r][sk], bh[c], acc[r][c], 0, 0, 0);
+                            acc[r][c] = mfma32(abuf[k][0][r][sk], bh[c], acc[r][c]);
                     if constexpr (MODE == MMS_M2L_BF16X3) {
                         bf16x8 bl[NC];
 #pragma unroll
@@ -680,6 +774,14 @@ __global__ __launch_bounds__(256 * ZP) void m2l_mfma_kernel(const float *__restr
         }
     }
     // D tile row 4g + q (q = 0..3), column j16: slots 16r + 4g .. +3 of target j16
+    if constexpr (MODE == MMS_M2L_FP16) {
+#pragma unroll
+        for (int r = 0; r < RT; r++) {
+            const f32x4 rs = *reinterpret_cast<const f32x4 *>(f16scale + 16 * r + 4 * g) * inv_s;
+#pragma unroll
+            for (int c = 0; c < NC; c++) acc[r][c] *= rs;
+        }
+    }
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         const int t = 16 * c + j16;
@@ -1191,7 +1293,7 @@ int launch_m2l(const FmmPlan &pl, hipStream_t s) {
     MMS_CHECK(zz <= 65535, "FMM batch too large for one M2L launch (films x layers^2 > 65535)");
     hipLaunchKernelGGL((m2l_mfma_kernel<KP, MODE, NC, ZP>), dim3(tiles, 1, (unsigned)zz), dim3(256 * ZP), (C::lds_bytes), s,
                        pl.mult, pl.part, reinterpret_cast<const char *>(pl.d_m2l), lv, pl.zd, pl.film_stride,
-                       pl.film_stride * pl.nfilm);
+                       pl.film_stride * pl.nfilm, pl.d_m2l_scale);
     MMS_HIP(hipGetLastError());
     return 0;
 }
@@ -1361,6 +1463,7 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
         switch (pl.precision) {
         case MMS_M2L_BF16: st = launch_m2l_nc<KP, MMS_M2L_BF16>(pl, s); break;
         case MMS_M2L_BF16X3: st = launch_m2l_nc<KP, MMS_M2L_BF16X3>(pl, s); break;
+        case MMS_M2L_FP16: st = launch_m2l_nc<KP, MMS_M2L_FP16>(pl, s); break;
         default: st = launch_m2l_nc<KP, MMS_M2L_FP32>(pl, s);
         }
         if (st) return 1;
@@ -1470,7 +1573,7 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
     MMS_CHECK(P_ >= 1 && P_ <= 9, "FMM expansion order must be 1..9");
     MMS_CHECK(w_ == 2 || w_ == 4 || w_ == 8, "FMM leaf width must be 2, 4 or 8");
     MMS_CHECK(g.zd >= 1 && g.zd <= 16, "FMM supports 1..16 layers");
-    MMS_CHECK(precision_ >= MMS_M2L_FP32 && precision_ <= MMS_M2L_BF16X3, "unknown M2L precision");
+    MMS_CHECK(precision_ >= MMS_M2L_FP32 && precision_ <= MMS_M2L_FP16, "unknown M2L precision");
     P = P_; w = w_; precision = precision_; zd = g.zd;
     m2l_cols = (cols_ == 8) ? 8 : 4;
     K = fmm_ncoef(P);
@@ -1550,12 +1653,18 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
                                                   : nblk * blk * sizeof(unsigned short) * (precision == MMS_M2L_BF16X3 ? 2 : 1);
     std::vector<char> all(m2l_level_bytes * (Lleaf - 1));
     std::vector<float> lv(nblk * blk);
+    // fragment element i = (blk, rt, lane, u) holds Op[16 rt + lane % 16][G (lane / 16) + u]
+    auto frow = [&](size_t i) { return (int)(16 * ((i / (64 * G)) % RT) + (i / G) % 16); };
+    auto fcol = [&](size_t i) { return (int)(G * ((i / G) % 64 / 16) + i % G); };
+    std::vector<float> fp16_levels;  // FP16: every level's FP32 fragments, scaled once all are known
     for (int L = 2; L <= Lleaf; L++) {
         std::vector<double> T = ops.m2l_level(L, Lmax, zd, (double)g.ay, (double)g.az);
         for (size_t b = 0; b < nblk; b++) frag(&T[b * K * K], lv, b * blk);
         char *dst = all.data() + (size_t)(L - 2) * m2l_level_bytes;
         if (precision == MMS_M2L_FP32) {
             memcpy(dst, lv.data(), m2l_level_bytes);
+        } else if (precision == MMS_M2L_FP16) {
+            fp16_levels.insert(fp16_levels.end(), lv.begin(), lv.end());
         } else {
             unsigned short *h = reinterpret_cast<unsigned short *>(dst);
             for (size_t i = 0; i < lv.size(); i++) {
@@ -1563,6 +1672,40 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
                 if (precision == MMS_M2L_BF16X3) h[lv.size() + i] = host_bf16(lv[i] - host_bf2f(h[i]));
             }
         }
+    }
+    if (precision == MMS_M2L_FP16) {
+        // Row and column factors a_j, b_k (powers of two) that bring the
+        // largest |a_j T_jk b_k| of every row and column over all levels,
+        // offsets and layer distances near 1 (alternating square-root
+        // equilibration), so the operator sits well inside FP16 range.
+        std::vector<double> a(KP, 1.0), b(KP, 1.0), mx(KP);
+        for (int it = 0; it < 8; it++) {
+            std::fill(mx.begin(), mx.end(), 0.0);
+            for (size_t i = 0; i < fp16_levels.size(); i++)
+                mx[frow(i)] = std::max(mx[frow(i)], std::fabs(fp16_levels[i]) * a[frow(i)] * b[fcol(i)]);
+            for (int j = 0; j < KP; j++)
+                if (mx[j] > 0) a[j] /= std::sqrt(mx[j]);
+            std::fill(mx.begin(), mx.end(), 0.0);
+            for (size_t i = 0; i < fp16_levels.size(); i++)
+                mx[fcol(i)] = std::max(mx[fcol(i)], std::fabs(fp16_levels[i]) * a[frow(i)] * b[fcol(i)]);
+            for (int k = 0; k < KP; k++)
+                if (mx[k] > 0) b[k] /= std::sqrt(mx[k]);
+        }
+        std::vector<float> sc(2 * KP);
+        for (int j = 0; j < KP; j++) {
+            a[j] = std::ldexp(1.0, (int)std::lround(std::log2(a[j])));
+            b[j] = std::ldexp(1.0, (int)std::lround(std::log2(b[j])));
+            sc[j] = (float)(1.0 / a[j]);
+            sc[KP + j] = (float)(1.0 / b[j]);
+        }
+        unsigned short *h = reinterpret_cast<unsigned short *>(all.data());
+        for (size_t i = 0; i < fp16_levels.size(); i++) {
+            const float v = (float)(fp16_levels[i] * a[frow(i)] * b[fcol(i)]);
+            MMS_CHECK(std::fabs(v) < 65504.0f, "FP16 M2L operator out of range");
+            const _Float16 hv = (_Float16)v;
+            memcpy(&h[i], &hv, 2);
+        }
+        if (upload((void **)&d_m2l_scale, sc.data(), sizeof(float) * sc.size())) return 1;
     }
     if (upload((void **)&d_m2l, all.data(), all.size())) return 1;
     // near-field 1/r table over offsets dx, dy in [-(2w-1), 2w-1], dz in [-(zd-1), zd-1]
@@ -1607,11 +1750,12 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
 }
 
 void FmmPlan::release() {
-    void *bufs[] = {mult, loc, part, d_p2pA, d_p2mf, d_l2p, d_l2pf, d_m2m, d_l2l, d_m2l, d_near};
+    void *bufs[] = {mult, loc, part, d_p2pA, d_p2mf, d_l2p, d_l2pf, d_m2m, d_l2l, d_m2l, d_near, d_m2l_scale};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     mult = loc = part = d_p2mf = d_l2p = d_l2pf = d_m2m = d_l2l = d_near = nullptr;
     d_m2l = nullptr;
+    d_m2l_scale = nullptr;
     d_p2pA = nullptr;
     m2l_level_bytes = 0;
     if (d_films) (void)hipFree(d_films);

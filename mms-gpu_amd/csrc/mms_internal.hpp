@@ -106,6 +106,7 @@ struct FmmPlan {
     float *d_l2pf = nullptr; // leaf width 4: L2P as MFMA fragments [64][KP/4]
     float *d_m2m = nullptr;  // MFMA fragments [4 child positions][KP/16][64][KP/4]
     float *d_l2l = nullptr;  // MFMA fragments [4 child positions][KP/16][64][KP/4]
+    float *d_m2l_scale = nullptr;  // FP16 M2L: 1/row scale [KP], 1/column scale [KP] (powers of two)
     void *d_m2l = nullptr;   // MFMA fragments per level [2zd-1][49 offsets][KP/16][64][KP/4]
                              // (float, or bf16 hi [+ bf16 lo] planes)
     size_t m2l_level_bytes = 0;

@@ -177,18 +177,19 @@ def test_fmm_linearity_and_sign(mg):
 
 
 @pytest.mark.parametrize("leaf", [4, 8])
-@pytest.mark.parametrize("mode,tol", [("fp32", 2e-6), ("bf16x3", 2e-5), ("bf16", 3e-3)])
+@pytest.mark.parametrize("mode,tol", [("fp32", 2e-6), ("bf16x3", 2e-5), ("bf16", 3e-3), ("fp16", 4e-4)])
 def test_fmm_m2l_precision_modes(mg, mode, tol, leaf):
     """M2L on the matrix cores in each operand precision vs the FP64 FMM
     oracle (same P, leaf width, tree): FP32 to rounding; hi/lo BF16 split
-    to ~2^-16 of the far field; plain BF16 to ~2^-8 of the far field.  At
-    leaf width 8 the BF16 modes also run the near field in the hi/lo BF16
-    split (three products), which stays at the split's ~2^-16."""
+    to ~2^-16 of the far field; plain BF16 to ~2^-8 and scaled FP16 to
+    ~2^-11 of the far field.  At leaf width 8 the 16-bit modes run the near
+    field in the hi/lo BF16 split (three products), which stays at the
+    split's ~2^-16."""
     from oracle import fmm_oracle as fo
     rng = np.random.default_rng(21)
     xd, zd, P = 128, 4, 6
     q = (rng.normal(size=(zd, xd, xd)) * 1e14).astype(F)
-    prec = {"fp32": mg.M2L_FP32, "bf16": mg.M2L_BF16, "bf16x3": mg.M2L_BF16X3}[mode]
+    prec = {"fp32": mg.M2L_FP32, "bf16": mg.M2L_BF16, "bf16x3": mg.M2L_BF16X3, "fp16": mg.M2L_FP16}[mode]
     with mg.Film(xd, xd, zd, *D5, solver=mg.SOLVER_FMM, P=P, m2l_precision=prec, leaf_width=leaf) as f:
         pot = f.potential(q.ravel())
     ref = fo.FmmOracle(xd, xd, zd, P, w=leaf).potential_units(q.astype(float)).ravel() * float(D5[0]) ** 2
@@ -197,7 +198,7 @@ def test_fmm_m2l_precision_modes(mg, mode, tol, leaf):
     assert err < tol
 
 
-@pytest.mark.parametrize("leaf,prec", [(8, "bf16x3"), (8, "fp32"), (4, "fp32")])
+@pytest.mark.parametrize("leaf,prec", [(8, "bf16x3"), (8, "fp16"), (8, "fp32"), (4, "fp32")])
 def test_concurrent_near_field_matches_sequential(mg, leaf, prec):
     """The near field runs on a second stream beside the multipole passes
     (fork/join) except inside a profiling window, where every kernel runs in
@@ -205,7 +206,7 @@ def test_concurrent_near_field_matches_sequential(mg, leaf, prec):
     eager calls and in replayed step graphs."""
     from mms_gpu import workloads as wl
     w = wl.c3_domains(side=128, zd=4, domain=16)
-    p = {"fp32": mg.M2L_FP32, "bf16x3": mg.M2L_BF16X3}[prec]
+    p = {"fp32": mg.M2L_FP32, "bf16x3": mg.M2L_BF16X3, "fp16": mg.M2L_FP16}[prec]
     ch = (np.random.default_rng(3).normal(size=w.n) * 1e14).astype(F)
     with mg.Film(*w.dims, *w.d, solver=mg.SOLVER_FMM, P=6, m2l_precision=p, leaf_width=leaf,
                  material=w.material) as f:
@@ -238,7 +239,7 @@ def test_fmm_thin_films(mg, zd):
     q = (rng.normal(size=(zd, xd, xd)) * 1e14).astype(F)
     for leaf in (4, 8):
         ref = fo.FmmOracle(xd, xd, zd, 6, w=leaf).potential_units(q.astype(float)).ravel() * float(D5[0]) ** 2
-        for prec, tol in ((mg.M2L_FP32, 2e-6), (mg.M2L_BF16X3, 2e-5)):
+        for prec, tol in ((mg.M2L_FP32, 2e-6), (mg.M2L_BF16X3, 2e-5), (mg.M2L_FP16, 4e-4)):
             with mg.Film(xd, xd, zd, *D5, solver=mg.SOLVER_FMM, P=6, m2l_precision=prec, leaf_width=leaf) as f:
                 pot = f.potential(q.ravel())
             err = _rel(pot, ref)
@@ -248,36 +249,57 @@ def test_fmm_thin_films(mg, zd):
 
 @pytest.mark.parametrize("P", [7, 9])
 def test_fmm_high_order_bf16_split(mg, P):
-    """P = 7..9 (64-coefficient operator blocks) in the BF16 split form at
-    leaf widths 4 and 8 against the FP64 FMM oracle."""
+    """P = 7..9 (64-coefficient operator blocks, source window staged in
+    two reads in the FP16 form) in the BF16 split and FP16 forms at leaf
+    widths 4 and 8 against the FP64 FMM oracle."""
     from oracle import fmm_oracle as fo
     rng = np.random.default_rng(P)
     xd, zd = 64, 3
     q = (rng.normal(size=(zd, xd, xd)) * 1e14).astype(F)
     for leaf in (4, 8):
-        with mg.Film(xd, xd, zd, *D5, solver=mg.SOLVER_FMM, P=P, m2l_precision=mg.M2L_BF16X3, leaf_width=leaf) as f:
-            pot = f.potential(q.ravel())
         ref = fo.FmmOracle(xd, xd, zd, P, w=leaf).potential_units(q.astype(float)).ravel() * float(D5[0]) ** 2
-        err = _rel(pot, ref)
-        _say(f"P={P} leaf {leaf} bf16x3: rel err vs FP64 FMM {err:.2e}")
-        assert err < 2e-5
+        for prec, name, tol in ((mg.M2L_BF16X3, "bf16x3", 2e-5), (mg.M2L_FP16, "fp16", 4e-4)):
+            with mg.Film(xd, xd, zd, *D5, solver=mg.SOLVER_FMM, P=P, m2l_precision=prec, leaf_width=leaf) as f:
+                pot = f.potential(q.ravel())
+            err = _rel(pot, ref)
+            _say(f"P={P} leaf {leaf} {name}: rel err vs FP64 FMM {err:.2e}")
+            assert err < tol
+
+
+@pytest.mark.parametrize("leaf", [4, 8])
+def test_fp16_m2l_scaling_is_exact(mg, leaf):
+    """The FP16 M2L scales operators and staged windows by powers of two
+    only, and every other pass is FP32: charges scaled by 2^40 or 2^-40
+    (far outside FP16 range) give the potential scaled by exactly the same
+    factor, bit for bit."""
+    rng = np.random.default_rng(44)
+    xd, zd = 96, 3
+    q = (rng.normal(size=(zd, xd, xd)) * 1e14).astype(F)
+    with mg.Film(xd, xd, zd, *D5, solver=mg.SOLVER_FMM, P=6, m2l_precision=mg.M2L_FP16, leaf_width=leaf) as f:
+        p0 = f.potential(q.ravel())
+        for e in (40, -40):
+            pe = f.potential((q * F(2.0 ** e)).ravel())
+            assert np.array_equal(pe, p0 * F(2.0 ** e))
 
 
 def test_bf16_near_field_ragged_films(mg):
     """Leaf width 8 with the BF16 split near field on non-square,
     non-power-of-two films with zero and sign-alternating layers: matches
-    the FP32 near field to the split's precision."""
+    the FP32 near field to the split's precision (the FP16 mode runs the
+    same split near field; its M2L adds ~2^-11 of the far field)."""
     rng = np.random.default_rng(5)
     for xd, yd, zd in [(72, 40, 3), (100, 130, 4), (64, 64, 5)]:
         q = (rng.normal(size=(zd, yd, xd)) * 1e14).astype(F)
         q[1] = 0.0
         pots = []
-        for prec in (mg.M2L_FP32, mg.M2L_BF16X3):
+        for prec in (mg.M2L_FP32, mg.M2L_BF16X3, mg.M2L_FP16):
             with mg.Film(xd, yd, zd, *D5, solver=mg.SOLVER_FMM, P=6, m2l_precision=prec, leaf_width=8) as f:
                 pots.append(f.potential(q.ravel()))
         err = _rel(pots[1], pots[0])
-        _say(f"bf16 near field {xd}x{yd}x{zd}: rel diff vs FP32 {err:.2e}")
+        err16 = _rel(pots[2], pots[0])
+        _say(f"near field {xd}x{yd}x{zd}: rel diff vs FP32 bf16x3 {err:.2e} fp16 {err16:.2e}")
         assert err < 2e-5
+        assert err16 < 4e-4
 
 
 @pytest.mark.parametrize("solver", ["fmm", "direct"])

@@ -262,14 +262,20 @@ __global__ __launch_bounds__(256) void up_subtree_kernel(const float *__restrict
 
 // Downward pass of one subtree: the workgroup owns one box of level R of one
 // layer.  Each level's local expansions are the M2L partial sums over source
-// layers plus L2L from the parent (FP32 MFMA, wave w = child position w);
+// layers plus L2P from the parent (FP32 MFMA, wave w = child position w);
 // the root takes its parent from HBM (level R - 1), the levels between stay
-// in LDS and only the leaf locals are written to HBM.
-template <int KP>
+// in LDS.  W = 0: the leaf locals are written to HBM.  W = 4, 8: the leaf
+// locals stay in LDS too and the far field is added to the potential
+// (phi[cells of leaf] += scale * L2P[W*W x KP] * L, FP32 MFMA), so phi must
+// already hold the near field.
+template <int KP, int W>
 __global__ __launch_bounds__(256) void down_subtree_kernel(const float *__restrict__ part, long part_stride,
                                                            float *__restrict__ Lall, const float *__restrict__ l2lfrag,
-                                                           LevelOffsets lo, int R, int D, int zd, long film_stride) {
+                                                           LevelOffsets lo, int R, int D, int zd, long film_stride,
+                                                           float *__restrict__ phi, const float *__restrict__ l2pfrag,
+                                                           int xd, int yd, float scale, FilmSet fs) {
     constexpr int G = KP / 4, RT = KP / 16, SB = KP + 4;
+    constexpr bool FUSE = W > 0;
     extern __shared__ __attribute__((aligned(16))) float lds[];
     const int nR = 1 << R, Lleaf = R + D;
     const int rbx = blockIdx.x % nR, rby = blockIdx.x / nR;
@@ -277,7 +283,9 @@ __global__ __launch_bounds__(256) void down_subtree_kernel(const float *__restri
     const float *Pf = part + (long)f * film_stride;
     float *L = Lall + (long)f * film_stride;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j16 = lane & 15, g = lane >> 4;
-    float *pa = lds, *ch = lds + 64 * SB;
+    // two level buffers; with L2P fused the leaf level (4^D boxes) is kept too
+    const int nbuf = (FUSE && D > 3) ? (1 << (2 * D)) : 64;
+    float *pa = lds, *ch = lds + nbuf * SB;
     // root box (level R): wave 0, column 0
     if (wave == 0) {
         f32x4 acc[RT];
@@ -307,7 +315,7 @@ __global__ __launch_bounds__(256) void down_subtree_kernel(const float *__restri
                 f32x4 v = acc[r];
 #pragma unroll 4
                 for (int s = 0; s < zd; s++) v += *reinterpret_cast<const f32x4 *>(Pf + s * part_stride + e + 16 * r + 4 * g);
-                if (D == 0) *reinterpret_cast<f32x4 *>(L + e + 16 * r + 4 * g) = v;
+                if (D == 0 && !FUSE) *reinterpret_cast<f32x4 *>(L + e + 16 * r + 4 * g) = v;
                 else *reinterpret_cast<f32x4 *>(pa + 16 * r + 4 * g) = v;
             }
         }
@@ -345,7 +353,7 @@ __global__ __launch_bounds__(256) void down_subtree_kernel(const float *__restri
 #pragma unroll 4
                     for (int zs = 0; zs < zd; zs++)
                         v += *reinterpret_cast<const f32x4 *>(Pf + zs * part_stride + e + 16 * r + 4 * g);
-                    if (l == Lleaf) *reinterpret_cast<f32x4 *>(L + e + 16 * r + 4 * g) = v;
+                    if (l == Lleaf && !FUSE) *reinterpret_cast<f32x4 *>(L + e + 16 * r + 4 * g) = v;
                     else *reinterpret_cast<f32x4 *>(ch + (cy * s + cx) * SB + 16 * r + 4 * g) = v;
                 }
             }
@@ -353,6 +361,53 @@ __global__ __launch_bounds__(256) void down_subtree_kernel(const float *__restri
         __syncthreads();
         float *t = pa; pa = ch; ch = t;
         sp = s;
+    }
+    if constexpr (FUSE) {
+        // pa = the sp x sp leaf locals of this subtree; wave w takes column
+        // tiles of 16 leaves w, w + 4, ...
+        constexpr int RTW = W * W / 16;
+        float *ph = phi;
+        if (fs.desc) {
+            const FilmDesc d = fs.desc[f];
+            xd = d.xd; yd = d.yd; ph += d.off;
+        }
+        const long plane = (long)xd * yd;
+        const int nl = sp * sp;
+        for (int c = wave; c < (nl + 15) / 16; c += 4) {
+            const int p = 16 * c + j16, px = p % sp, py = p / sp;
+            const bool ok = p < nl;
+            f32x4 acc[RTW];
+#pragma unroll
+            for (int r = 0; r < RTW; r++) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int u4 = 0; u4 < G / 4; u4++) {
+                const f32x4 b = ok ? *reinterpret_cast<const f32x4 *>(pa + p * SB + g * G + 4 * u4)
+                                   : f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int r = 0; r < RTW; r++) {
+                    const f32x4 a = *reinterpret_cast<const f32x4 *>(l2pfrag + ((long)r * 64 + lane) * G + 4 * u4);
+#pragma unroll
+                    for (int u = 0; u < 4; u++) acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], acc[r], 0, 0, 0);
+                }
+            }
+            const int bx = rbx * sp + px, by = rby * sp + py;
+            // D row 4g + q of tile r = cell 16 r + 4g + q of leaf j16: 4 consecutive x cells
+#pragma unroll
+            for (int r = 0; r < RTW; r++) {
+                const int cc = 16 * r + 4 * g;
+                const int y = by * W + cc / W, xb = bx * W + cc % W;
+                if (!ok || y >= yd) continue;
+                float *dst = ph + (long)z * plane + (long)y * xd + xb;
+                if (xb + 3 < xd && (((long)z * plane + (long)y * xd + xb) & 3) == 0) {
+                    const f32x4 v = *reinterpret_cast<const f32x4 *>(dst);
+                    *reinterpret_cast<f32x4 *>(dst) = v + acc[r] * scale;
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+                        if (xb + q < xd) dst[q] += acc[r][q] * scale;
+                }
+            }
+        }
     }
 }
 
@@ -1004,8 +1059,8 @@ __global__ __launch_bounds__(256) void p2p_l2p_mfma_kernel(const float *__restri
                 }
             }
         }
-        // far field: L2P[W*W x KP] * L[KP x 16 boxes] (skipped when the far
-        // field is added by l2p_add_kernel after a concurrent FMM pass)
+        // far field: L2P[W*W x KP] * L[KP x 16 boxes] (skipped when the
+        // downward pass adds the far field itself)
         const int bxg = blockIdx.x * 4 + jx, byg = blockIdx.y * 4 + jy;
         const bool boxok = (bxg < nb && byg < nb);
         const float *Lv = Lleaf + (((long)zt * nb + (boxok ? byg : 0)) * nb + (boxok ? bxg : 0)) * KP + g * G;
@@ -1204,59 +1259,6 @@ __global__ __launch_bounds__(256) void p2p_l2p_bf16_kernel(const float *__restri
                         if (xb + q < xd) dst[q] = acc[r][c][q] * scale;
                 }
             }
-        }
-    }
-}
-
-// Far field added to the near field: phi[cells of leaf box] += scale *
-// L2P[W*W x KP] * L[KP x 16 boxes] (FP32 MFMA), for leaf widths 4 and 8.
-// Wave w of the block takes leaves (4 blockIdx.x + w) * 16 .. + 15 of leaf row
-// blockIdx.y; grid z = film * zd + layer.
-template <int K, int KP, int W>
-__global__ __launch_bounds__(256) void l2p_add_kernel(float *__restrict__ phi, const float *__restrict__ Lleaf,
-                                                      const float *__restrict__ l2pfrag, int xd, int yd, int zd,
-                                                      int nb, float scale, FilmSet fs) {
-    constexpr int G = KP / 4, RT = W * W / 16;
-    const int f = blockIdx.z / zd, z = blockIdx.z % zd;
-    if (fs.desc) {
-        const FilmDesc d = fs.desc[f];
-        xd = d.xd; yd = d.yd; phi += d.off;
-    }
-    Lleaf += (long)f * fs.stride;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j16 = lane & 15, g = lane >> 4;
-    const int bx0 = (blockIdx.x * 4 + wave) * 16, by = blockIdx.y;
-    if (bx0 * W >= xd || by * W >= yd) return;  // wave-uniform: no leaf of this wave holds film cells
-    const int bx = bx0 + j16;
-    const bool boxok = bx < nb;
-    const float *Lv = Lleaf + (((long)z * nb + by) * nb + (boxok ? bx : 0)) * KP + g * G;
-    f32x4 acc[RT];
-#pragma unroll
-    for (int r = 0; r < RT; r++) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int u4 = 0; u4 < G / 4; u4++) {
-        const f32x4 b = boxok ? *reinterpret_cast<const f32x4 *>(Lv + 4 * u4) : f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int r = 0; r < RT; r++) {
-            const f32x4 a = *reinterpret_cast<const f32x4 *>(l2pfrag + ((long)r * 64 + lane) * G + 4 * u4);
-#pragma unroll
-            for (int u = 0; u < 4; u++) acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], acc[r], 0, 0, 0);
-        }
-    }
-    const long plane = (long)xd * yd;
-    // D row 4g + q of tile r = cell 16 r + 4g + q of box j16: 4 consecutive x cells
-#pragma unroll
-    for (int r = 0; r < RT; r++) {
-        const int c = 16 * r + 4 * g;
-        const int y = by * W + c / W, xb = bx * W + c % W;
-        if (!boxok || y >= yd) continue;
-        float *dst = phi + (long)z * plane + (long)y * xd + xb;
-        if (xb + 3 < xd && (((long)z * plane + (long)y * xd + xb) & 3) == 0) {
-            f32x4 v = *reinterpret_cast<const f32x4 *>(dst);
-            *reinterpret_cast<f32x4 *>(dst) = v + acc[r] * scale;
-        } else {
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-                if (xb + q < xd) dst[q] += acc[r][q] * scale;
         }
     }
 }
@@ -1484,26 +1486,24 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
         }
         for (int L = Lc + 1; L < R; L++)
             if (launch_l2l_sum<KP>(pl, L, s)) return 1;
-        const size_t lds = sizeof(float) * 2 * 64 * SB;
+        // leaf widths 4 and 8: L2P fused into the leaf level, after the near field wrote phi
+        constexpr int WF = split ? W : 0;
+        const int nbuf = (split && D > 3) ? (1 << (2 * D)) : 64;
+        const size_t lds = sizeof(float) * 2 * (size_t)nbuf * SB;
+        MMS_CHECK(lds <= 160 * 1024, "FMM downward pass: subtree exceeds LDS");
         static int attr = 0;
         if (!attr) {
-            MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&down_subtree_kernel<KP>),
+            MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&down_subtree_kernel<KP, WF>),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
             attr = 1;
         }
-        hipLaunchKernelGGL((down_subtree_kernel<KP>), dim3(nR * nR, zd * pl.nfilm), dim3(256), lds, s, pl.part,
-                           pl.film_stride * pl.nfilm, pl.loc, pl.d_l2l, lo, R, D, zd, pl.film_stride);
+        if (conc) MMS_HIP(hipStreamWaitEvent(s, pl.ev_join, 0));
+        hipLaunchKernelGGL((down_subtree_kernel<KP, WF>), dim3(nR * nR, zd * pl.nfilm), dim3(256), lds, s, pl.part,
+                           pl.film_stride * pl.nfilm, pl.loc, pl.d_l2l, lo, R, D, zd, pl.film_stride, phi, pl.d_l2pf,
+                           g.xd, g.yd, scale, pl.films());
         MMS_HIP(hipGetLastError());
     }
-    if constexpr (split) {
-        if (conc) MMS_HIP(hipStreamWaitEvent(s, pl.ev_join, 0));
-        KTimer kt(prof, K_L2P);
-        const int nb = 1 << pl.Lleaf;
-        dim3 grid((nb + 63) / 64, nb, zd * pl.nfilm);
-        hipLaunchKernelGGL((l2p_add_kernel<K, KP, W>), grid, dim3(256), 0, s, phi, pl.loc + pl.lvl_off[pl.Lleaf],
-                           pl.d_l2pf, g.xd, g.yd, zd, nb, scale, pl.films());
-        MMS_HIP(hipGetLastError());
-    } else {
+    if constexpr (!split) {
         KTimer kt(prof, K_P2P_L2P);
         if (launch_near<K, KP, W, true>(g, pl, rho, phi, scale, false, s)) return 1;
     }

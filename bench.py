@@ -65,14 +65,15 @@ def p2p_pairs_per_eval(side, zd, w):
 
 
 def p2p_flops_per_eval(side, zd, w, m2l):
-    """MFMA work of the near-field kernel (L2P runs in its own kernel).
-    FP32 form: one MAC per (target, window source) pair -- the 1/r table
-    lookup replaces the distance arithmetic.  BF16 split form (leaf 8): the
-    24 x 24 source window of a target is packed into 18 k steps of 32 with no
-    padding, three BF16 products each.  Returns (flops, peak TFLOP/s)."""
+    """MFMA work of the near-field kernel (L2P is added by the downward
+    pass).  FP32 form: one MAC per (target, window source) pair -- the 1/r
+    table lookup replaces the distance arithmetic.  16-bit forms (leaf 8):
+    the 24 x 24 source window of a target is packed into 18 k steps of 32
+    with no padding, three BF16 products each in the BF16 split, one FP16
+    product in the FP16 form.  Returns (flops, peak TFLOP/s)."""
     cells = side * side * zd
     if w == 8 and m2l != "fp32":
-        return 2 * cells * zd * (3 * w) * (3 * w) * 3, BF16_PEAK_TFLOPS
+        return 2 * cells * zd * (3 * w) * (3 * w) * (1 if m2l == "fp16" else 3), BF16_PEAK_TFLOPS
     return 2 * cells * zd * 9 * w * w, FP32_PEAK_TFLOPS
 
 
@@ -374,7 +375,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--m2l", default="bf16x3", choices=PREC_NAMES)
+    ap.add_argument("--m2l", default="fp16", choices=PREC_NAMES)
     ap.add_argument("--leaf", type=int, default=8)
     ap.add_argument("--P", type=int, default=6)
     ap.add_argument("--side", type=int, default=1024)

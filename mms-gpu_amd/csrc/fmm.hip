@@ -1105,8 +1105,12 @@ __global__ __launch_bounds__(256) void p2p_l2p_mfma_kernel(const float *__restri
 // v_mfma_f32_16x16x32_bf16 (Ah Bh + Ah Bl + Al Bh).  Walking sy with a
 // stride of 2, three of the four operators of the next row are the previous
 // row's: a four-deep register window, one new operator per row.
+// F16: one v_mfma_f32_16x16x32_f16 product per step instead of three; the
+// window is scaled by s = 2^e (its largest |charge| in [2^14, 2^15)) before
+// the FP16 conversion and the sums by 1 / s afterwards (exact), the 1/r
+// operator (values in (0, 1]) is stored as plain FP16.
 // L2P is fused as in the FP32 kernel.
-template <int K, int KP, int NCB, bool FAR>
+template <int K, int KP, int NCB, bool FAR, bool F16>
 __global__ __launch_bounds__(256) void p2p_l2p_bf16_kernel(const float *__restrict__ rho, float *__restrict__ phi,
                                                            const float *__restrict__ Lleaf,
                                                            const float *__restrict__ l2pfrag,
@@ -1122,12 +1126,73 @@ __global__ __launch_bounds__(256) void p2p_l2p_bf16_kernel(const float *__restri
     Lleaf += (long)blockIdx.z * fs.stride;
     constexpr int G = KP / 4, RT = 4;
     extern __shared__ __attribute__((aligned(16))) unsigned short sh[];
-    unsigned short *qh = sh, *ql = sh + zd * SWY * SP;  // [zd][SWY][SP] hi, lo
+    unsigned short *qh = sh, *ql = sh + zd * SWY * SP;  // [zd][SWY][SP] hi, lo (F16: hi only)
+    typedef typename std::conditional<F16, f16x8, bf16x8>::type BV;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j16 = lane & 15, g = lane >> 4;
     const int x0 = blockIdx.x * TWX, y0 = blockIdx.y * TWY;
     const long plane = (long)xd * yd;
-    {   // charges of the window, 16 loads per thread in flight per batch
+    float inv_s = 1.0f;
+    if constexpr (F16) {
+        // The window's largest |charge| fixes the scale: up to 4 layers
+        // (64 charges per thread) the window is loaded once into registers,
+        // deeper films read it twice (maximum, then the scaled values).
+        constexpr int BATCH = 16, HOLD = 64;
+        const int ne = zd * SWY * SWX;
+        auto ld = [&](int e) {  // e >= ne reads nothing and returns 0
+            const int zs = e / (SWY * SWX), r = e % (SWY * SWX), sy = r / SWX, sx = r % SWX;
+            const int gx = x0 - W + sx, gy = y0 - W + sy;
+            return (e < ne && gx >= 0 && gx < xd && gy >= 0 && gy < yd) ? rho[(long)zs * plane + (long)gy * xd + gx]
+                                                                      : 0.0f;
+        };
+        auto st = [&](int e, float v, float sc) {
+            if (e >= ne) return;
+            const int zs = e / (SWY * SWX), r = e % (SWY * SWX), sy = r / SWX, sx = r % SWX;
+            qh[(zs * SWY + sy) * SP + sx] = __builtin_bit_cast(unsigned short, (_Float16)(v * sc));
+        };
+        const bool hold = ne <= 256 * HOLD;  // uniform
+        float vh[HOLD];
+        float mx = 0.0f;
+        if (hold) {
+#pragma unroll
+            for (int k = 0; k < HOLD; k++) vh[k] = ld(threadIdx.x + 256 * k);
+#pragma unroll
+            for (int k = 0; k < HOLD; k++) mx = fmaxf(mx, fabsf(vh[k]));
+        } else {
+#pragma unroll 1
+            for (int e0 = 0; e0 < ne; e0 += 256 * BATCH) {
+                float vb[BATCH];
+#pragma unroll
+                for (int k = 0; k < BATCH; k++) vb[k] = ld(e0 + threadIdx.x + 256 * k);
+#pragma unroll
+                for (int k = 0; k < BATCH; k++) mx = fmaxf(mx, fabsf(vb[k]));
+            }
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+        __shared__ float wmax[4];
+        if (lane == 0) wmax[wave] = mx;
+        __syncthreads();
+        mx = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+        // s = 2^(14 - e) for 2^e <= max < 2^(e+1), clamped so s and 1/s stay normal
+        int es = 14 - ((int)((__float_as_uint(mx) >> 23) & 255u) - 127);
+        es = es > 100 ? 100 : (es < -100 ? -100 : es);
+        const float sc = __uint_as_float((unsigned)(es + 127) << 23);
+        inv_s = __uint_as_float((unsigned)(127 - es) << 23);
+        if (hold) {
+#pragma unroll
+            for (int k = 0; k < HOLD; k++) st(threadIdx.x + 256 * k, vh[k], sc);
+        } else {
+#pragma unroll 1
+            for (int e0 = 0; e0 < ne; e0 += 256 * BATCH) {
+                float vb[BATCH];
+#pragma unroll
+                for (int k = 0; k < BATCH; k++) vb[k] = ld(e0 + threadIdx.x + 256 * k);
+#pragma unroll
+                for (int k = 0; k < BATCH; k++) st(e0 + threadIdx.x + 256 * k, vb[k], sc);
+            }
+        }
+    } else {   // charges of the window, 16 loads per thread in flight per batch
         constexpr int BATCH = 16;
         const int ne = zd * SWY * SWX;
 #pragma unroll 1
@@ -1172,9 +1237,9 @@ __global__ __launch_bounds__(256) void p2p_l2p_bf16_kernel(const float *__restri
                 for (int c = 0; c < NCB; c++) accz[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
             const int dzi = (zs > zt ? zs - zt : zt - zs) + zd - 1;  // 1/r is even in dz
             const unsigned short *Ab = afrag + (long)dzi * 42 * 2 * 64 * 8 + lane * 8;
-            auto loadA = [&](bf16x8 &h, bf16x8 &l, int op) {
-                h = *reinterpret_cast<const bf16x8 *>(Ab + (long)(op * 2) * 64 * 8);
-                l = *reinterpret_cast<const bf16x8 *>(Ab + (long)(op * 2 + 1) * 64 * 8);
+            auto loadA = [&](BV &h, BV &l, int op) {
+                h = *reinterpret_cast<const BV *>(Ab + (long)(op * 2) * 64 * 8);
+                if constexpr (!F16) l = *reinterpret_cast<const BV *>(Ab + (long)(op * 2 + 1) * 64 * 8);
             };
             const int bbase = (zs * SWY + W * jy) * SP;  // row 0 of this lane's box window in layer zs
 #pragma unroll
@@ -1184,7 +1249,7 @@ __global__ __launch_bounds__(256) void p2p_l2p_bf16_kernel(const float *__restri
                 const int fl = 32 * ph + 8 * g, srow = fl / 24, scol = fl % 24;
                 // window: A[p] = operator (ph, d = 4 a - 2 p), index 14 ph + 2 a - p + 3;
                 // An0 / An1 = pairs 0 / 1 of step a + 1
-                bf16x8 Ah[4], Al[4], An0h, An0l, An1h, An1l;
+                BV Ah[4], Al[4], An0h, An0l, An1h, An1l;
 #pragma unroll
                 for (int p = 0; p < 4; p++) loadA(Ah[p], Al[p], 14 * ph + 3 - p);
                 loadA(An0h, An0l, 14 * ph + 5);
@@ -1192,20 +1257,22 @@ __global__ __launch_bounds__(256) void p2p_l2p_bf16_kernel(const float *__restri
 #pragma unroll
                 for (int aa = 0; aa < 6; aa++) {
                     const int ro = bbase + (4 * aa + srow) * SP + scol;
-                    bf16x8 bh[NCB], bl[NCB];
+                    BV bh[NCB], bl[NCB];
 #pragma unroll
                     for (int c = 0; c < NCB; c++) {
                         const int jx = 4 * c + (j16 & 3);
-                        bh[c] = *reinterpret_cast<const bf16x8 *>(qh + ro + W * jx);
-                        bl[c] = *reinterpret_cast<const bf16x8 *>(ql + ro + W * jx);
+                        bh[c] = *reinterpret_cast<const BV *>(qh + ro + W * jx);
+                        if constexpr (!F16) bl[c] = *reinterpret_cast<const BV *>(ql + ro + W * jx);
                     }
 #pragma unroll
                     for (int p = 0; p < 4; p++)
 #pragma unroll
                         for (int c = 0; c < NCB; c++) {
-                            accz[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[p], bh[c], accz[p][c], 0, 0, 0);
-                            accz[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[p], bl[c], accz[p][c], 0, 0, 0);
-                            accz[p][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al[p], bh[c], accz[p][c], 0, 0, 0);
+                            accz[p][c] = mfma32(Ah[p], bh[c], accz[p][c]);
+                            if constexpr (!F16) {
+                                accz[p][c] = mfma32(Ah[p], bl[c], accz[p][c]);
+                                accz[p][c] = mfma32(Al[p], bh[c], accz[p][c]);
+                            }
                         }
                     if (aa < 5) {  // slide: pairs 2, 3 of step a + 1 use pairs 0, 1 of step a
                         Ah[3] = Ah[1]; Al[3] = Al[1];
@@ -1223,6 +1290,12 @@ __global__ __launch_bounds__(256) void p2p_l2p_bf16_kernel(const float *__restri
             for (int r = 0; r < RT; r++)
 #pragma unroll
                 for (int c = 0; c < NCB; c++) acc[r][c] += accz[r][c];
+        }
+        if constexpr (F16) {
+#pragma unroll
+            for (int r = 0; r < RT; r++)
+#pragma unroll
+                for (int c = 0; c < NCB; c++) acc[r][c] *= inv_s;
         }
 #pragma unroll
         for (int c = 0; c < NCB; c++) {
@@ -1325,21 +1398,22 @@ int launch_l2l_sum(const FmmPlan &pl, int L, hipStream_t s) {
 // (77 KB) share a CU instead of two near-field workgroups filling it.
 constexpr size_t kNearLdsShared = 82 * 1024;
 
-template <int K, int KP, int NCB, bool FAR>
+template <int K, int KP, int NCB, bool FAR, bool F16>
 int launch_p2p_bf16(const Grid &g, const FmmPlan &pl, const float *rho, float *phi, float scale, bool shared,
                     hipStream_t s) {
     constexpr int SP = 32 * NCB + 16 + 8;
-    size_t lds = sizeof(unsigned short) * 2 * (size_t)g.zd * 48 * SP;
+    size_t lds = sizeof(unsigned short) * (F16 ? 1 : 2) * (size_t)g.zd * 48 * SP;
     if (shared && lds < kNearLdsShared) lds = kNearLdsShared;
-    MMS_CHECK(lds <= 160 * 1024, "FMM near field: too many layers for the LDS tile");
+    constexpr int kMaxDyn = 160 * 1024 - 256;  // the FP16 form also has a few bytes of static LDS
+    MMS_CHECK(lds <= (size_t)kMaxDyn, "FMM near field: too many layers for the LDS tile");
     static int attr = 0;
     if (!attr) {
-        MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&p2p_l2p_bf16_kernel<K, KP, NCB, FAR>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&p2p_l2p_bf16_kernel<K, KP, NCB, FAR, F16>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDyn));
         attr = 1;
     }
     dim3 grid((g.xd + 32 * NCB - 1) / (32 * NCB), (g.yd + 31) / 32, pl.nfilm);
-    hipLaunchKernelGGL((p2p_l2p_bf16_kernel<K, KP, NCB, FAR>), grid, dim3(256), lds, s, rho, phi,
+    hipLaunchKernelGGL((p2p_l2p_bf16_kernel<K, KP, NCB, FAR, F16>), grid, dim3(256), lds, s, rho, phi,
                        pl.loc + pl.lvl_off[pl.Lleaf], pl.d_l2pf, pl.d_p2pA, g.xd, g.yd, g.zd, 1 << pl.Lleaf, scale,
                        pl.films());
     MMS_HIP(hipGetLastError());
@@ -1354,7 +1428,8 @@ int launch_near(const Grid &g, const FmmPlan &pl, const float *rho, float *phi, 
                 hipStream_t s) {
     const int zd = g.zd;
     if constexpr (W == 8) {
-        if (pl.precision != MMS_M2L_FP32) return launch_p2p_bf16<K, KP, 2, FAR>(g, pl, rho, phi, scale, shared, s);
+        if (pl.precision == MMS_M2L_FP16) return launch_p2p_bf16<K, KP, 2, FAR, true>(g, pl, rho, phi, scale, shared, s);
+        if (pl.precision != MMS_M2L_FP32) return launch_p2p_bf16<K, KP, 2, FAR, false>(g, pl, rho, phi, scale, shared, s);
     }
     if constexpr (W == 4 || W == 8) {
         constexpr int TW = 4 * W, SW = TW + 2 * W, TS = 4 * W - 1;
@@ -1721,7 +1796,8 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
             }
     if (upload((void **)&d_near, tab.data(), sizeof(float) * tab.size())) return 1;
     if (w == 8) {
-        // BF16 split near-field operators [2zd-1][phase 0..2][d index 0..13][hi, lo][64 lanes][8].
+        // BF16 split (FP16: plain FP16 in the hi plane) near-field operators
+        // [2zd-1][phase 0..2][d index 0..13][hi, lo][64 lanes][8].
         // The 24 x 24 source window of a leaf is read as rows packed back to
         // back, 32 sources per k step: step (a, ph) holds flat sources
         // 96 a + 32 ph + k, i.e. row 4 a + (32 ph + k) / 24, column (32 ph + k) % 24.
@@ -1738,10 +1814,15 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
                         for (int t = 0; t < 8; t++) {
                             const int fl = 32 * ph + 8 * gg + t, row = fl / 24 + d - r + 7, col = fl % 24 - ix + 7;
                             const float v = tab[((size_t)dzi * ts + row) * ts + col];
-                            const unsigned short h = host_bf16(v);
                             const size_t o = (((size_t)((dzi * 3 + ph) * 14 + di) * 2) * 64 + lane) * 8 + t;
-                            af[o] = h;
-                            af[o + 64 * 8] = host_bf16(v - host_bf2f(h));
+                            if (precision == MMS_M2L_FP16) {  // plain FP16, lo plane unused
+                                const _Float16 hv = (_Float16)v;
+                                memcpy(&af[o], &hv, 2);
+                            } else {
+                                const unsigned short h = host_bf16(v);
+                                af[o] = h;
+                                af[o + 64 * 8] = host_bf16(v - host_bf2f(h));
+                            }
                         }
                     }
         if (upload((void **)&d_p2pA, af.data(), sizeof(unsigned short) * af.size())) return 1;

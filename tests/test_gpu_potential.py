@@ -177,14 +177,14 @@ def test_fmm_linearity_and_sign(mg):
 
 
 @pytest.mark.parametrize("leaf", [4, 8])
-@pytest.mark.parametrize("mode,tol", [("fp32", 2e-6), ("bf16x3", 2e-5), ("bf16", 3e-3), ("fp16", 4e-4)])
+@pytest.mark.parametrize("mode,tol", [("fp32", 2e-6), ("bf16x3", 2e-5), ("bf16", 3e-3), ("fp16", 8e-4)])
 def test_fmm_m2l_precision_modes(mg, mode, tol, leaf):
     """M2L on the matrix cores in each operand precision vs the FP64 FMM
     oracle (same P, leaf width, tree): FP32 to rounding; hi/lo BF16 split
     to ~2^-16 of the far field; plain BF16 to ~2^-8 and scaled FP16 to
-    ~2^-11 of the far field.  At leaf width 8 the 16-bit modes run the near
+    ~2^-11 of the far field.  At leaf width 8 the BF16 modes run the near
     field in the hi/lo BF16 split (three products), which stays at the
-    split's ~2^-16."""
+    split's ~2^-16, and the FP16 mode runs it in scaled FP16 (~2^-11)."""
     from oracle import fmm_oracle as fo
     rng = np.random.default_rng(21)
     xd, zd, P = 128, 4, 6
@@ -239,7 +239,7 @@ def test_fmm_thin_films(mg, zd):
     q = (rng.normal(size=(zd, xd, xd)) * 1e14).astype(F)
     for leaf in (4, 8):
         ref = fo.FmmOracle(xd, xd, zd, 6, w=leaf).potential_units(q.astype(float)).ravel() * float(D5[0]) ** 2
-        for prec, tol in ((mg.M2L_FP32, 2e-6), (mg.M2L_BF16X3, 2e-5), (mg.M2L_FP16, 4e-4)):
+        for prec, tol in ((mg.M2L_FP32, 2e-6), (mg.M2L_BF16X3, 2e-5), (mg.M2L_FP16, 8e-4)):
             with mg.Film(xd, xd, zd, *D5, solver=mg.SOLVER_FMM, P=6, m2l_precision=prec, leaf_width=leaf) as f:
                 pot = f.potential(q.ravel())
             err = _rel(pot, ref)
@@ -258,7 +258,7 @@ def test_fmm_high_order_bf16_split(mg, P):
     q = (rng.normal(size=(zd, xd, xd)) * 1e14).astype(F)
     for leaf in (4, 8):
         ref = fo.FmmOracle(xd, xd, zd, P, w=leaf).potential_units(q.astype(float)).ravel() * float(D5[0]) ** 2
-        for prec, name, tol in ((mg.M2L_BF16X3, "bf16x3", 2e-5), (mg.M2L_FP16, "fp16", 4e-4)):
+        for prec, name, tol in ((mg.M2L_BF16X3, "bf16x3", 2e-5), (mg.M2L_FP16, "fp16", 8e-4)):
             with mg.Film(xd, xd, zd, *D5, solver=mg.SOLVER_FMM, P=P, m2l_precision=prec, leaf_width=leaf) as f:
                 pot = f.potential(q.ravel())
             err = _rel(pot, ref)
@@ -283,10 +283,10 @@ def test_fp16_m2l_scaling_is_exact(mg, leaf):
 
 
 def test_bf16_near_field_ragged_films(mg):
-    """Leaf width 8 with the BF16 split near field on non-square,
-    non-power-of-two films with zero and sign-alternating layers: matches
-    the FP32 near field to the split's precision (the FP16 mode runs the
-    same split near field; its M2L adds ~2^-11 of the far field)."""
+    """Leaf width 8 with the BF16 split and the scaled FP16 near field on
+    non-square, non-power-of-two films with zero and sign-alternating
+    layers: matches the FP32 near field to the form's precision (5 layers
+    take the FP16 form's two-read window staging, 3 and 4 the one-read)."""
     rng = np.random.default_rng(5)
     for xd, yd, zd in [(72, 40, 3), (100, 130, 4), (64, 64, 5)]:
         q = (rng.normal(size=(zd, yd, xd)) * 1e14).astype(F)
@@ -299,7 +299,7 @@ def test_bf16_near_field_ragged_films(mg):
         err16 = _rel(pots[2], pots[0])
         _say(f"near field {xd}x{yd}x{zd}: rel diff vs FP32 bf16x3 {err:.2e} fp16 {err16:.2e}")
         assert err < 2e-5
-        assert err16 < 4e-4
+        assert err16 < 8e-4
 
 
 @pytest.mark.parametrize("solver", ["fmm", "direct"])

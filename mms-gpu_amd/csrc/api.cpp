@@ -77,7 +77,7 @@ static int ctx_alloc(mms_ctx *c, const unsigned char *material) {
     for (int i = 0; i < 3; i++) MMS_HIP(hipMalloc(&c->m32[i], sizeof(float) * 3 * n));
     MMS_HIP(hipMalloc(&c->mn64, sizeof(double) * 3 * n));
     MMS_HIP(hipMalloc(&c->mnew64, sizeof(double) * 3 * n));
-    MMS_HIP(hipMalloc(&c->acc64, sizeof(double) * 3 * n));
+    MMS_HIP(hipMalloc(&c->acc32, sizeof(float) * 3 * n));
     MMS_HIP(hipMalloc(&c->rho, sizeof(float) * n));
     MMS_HIP(hipMalloc(&c->phi, sizeof(float) * n));
     MMS_HIP(hipMalloc(&c->phi_n, sizeof(float) * n));
@@ -91,7 +91,7 @@ static int ctx_alloc(mms_ctx *c, const unsigned char *material) {
     MMS_HIP(hipMemsetAsync(c->m32[1], 0, sizeof(float) * 3 * n, c->stream));
     MMS_HIP(hipMemsetAsync(c->m32[2], 0, sizeof(float) * 3 * n, c->stream));
     MMS_HIP(hipMemsetAsync(c->mn64, 0, sizeof(double) * 3 * n, c->stream));
-    MMS_HIP(hipMemsetAsync(c->acc64, 0, sizeof(double) * 3 * n, c->stream));
+    MMS_HIP(hipMemsetAsync(c->acc32, 0, sizeof(float) * 3 * n, c->stream));
     MMS_HIP(hipMemsetAsync(c->t_dev, 0, sizeof(double), c->stream));
     if (material) {
         MMS_HIP(hipMemcpyAsync(c->matl, material, n, hipMemcpyHostToDevice, c->stream));
@@ -113,7 +113,7 @@ static int ctx_alloc(mms_ctx *c, const unsigned char *material) {
 static void ctx_free(mms_ctx *c) {
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
-    void *bufs[] = {c->m32[0], c->m32[1], c->m32[2], c->mn64, c->mnew64, c->acc64, c->rho, c->phi,
+    void *bufs[] = {c->m32[0], c->m32[1], c->m32[2], c->mn64, c->mnew64, c->acc32, c->rho, c->phi,
                     c->phi_n, c->matl, c->t_dev, c->h_aos, c->aux_aos, c->red};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -140,7 +140,7 @@ int mms_ctx::one_step(float dt) {
             if (potential(rho, phi)) return 1;
         }
         // an all-magnetic film skips the per-cell material byte
-        if (launch_stage(g, mat, s, (double)dt, phi, in[s], out[s], mn64, acc64, all_magnetic ? nullptr : matl,
+        if (launch_stage(g, mat, s, (double)dt, phi, in[s], out[s], mn64, acc32, all_magnetic ? nullptr : matl,
                          t_dev, stream, &prof))
             return 1;
     }
@@ -637,7 +637,7 @@ extern "C" int rk4_step(float t, float dt, float *t2, const unsigned char *mater
         if (st > 0 && demag) {
             if (launch_div(c->g, in[st], c->rho, s, &c->prof) || c->potential(c->rho, c->phi)) return EXIT_FAILURE;
         }
-        if (launch_stage_ex(c->g, c->mat, st, (double)dt, c->phi, in[st], outp[st], c->mn64, c->acc64, c->matl,
+        if (launch_stage_ex(c->g, c->mat, st, (double)dt, c->phi, in[st], outp[st], c->mn64, c->acc32, c->matl,
                             c->t_dev, ex, s, &c->prof))
             return EXIT_FAILURE;
     }
@@ -747,7 +747,7 @@ extern "C" int time_marching(unsigned char *material, float *M, float finaltime,
             if (st > 0 && demag) {
                 if (launch_div(c->g, in[st], c->rho, s, &c->prof) || c->potential(c->rho, c->phi)) { status = 1; break; }
             }
-            if (launch_stage_ex(c->g, c->mat, st, dt, phi_use, in[st], outp[st], c->mn64, c->acc64, c->matl, c->t_dev,
+            if (launch_stage_ex(c->g, c->mat, st, dt, phi_use, in[st], outp[st], c->mn64, c->acc32, c->matl, c->t_dev,
                                 ex, s, &c->prof))
                 status = 1;
         }

@@ -30,7 +30,8 @@ struct mms_batch {
     std::vector<int> solver;  // per film: 0 direct, 1 fmm
     hipStream_t stream = nullptr;
     float *m32[2] = {nullptr, nullptr};
-    double *mn64 = nullptr, *acc64 = nullptr;
+    double *mn64 = nullptr;
+    float *acc32 = nullptr;  // FP32 slope accumulator
     float *rho = nullptr, *phi = nullptr, *aos = nullptr;
     uint8_t *matl = nullptr;
     BatchFilm *d_films = nullptr;
@@ -75,7 +76,7 @@ int mms_batch::one_step(float dt) {
                 launches += 2 * fmm[k]->Lleaf - 1;
             }
         }
-        if (launch_stage_batch(gcommon, mat, s, (double)dt, phi, in[s], out[s], mn64, acc64, matl, d_tiles, ntiles,
+        if (launch_stage_batch(gcommon, mat, s, (double)dt, phi, in[s], out[s], mn64, acc32, matl, d_tiles, ntiles,
                                d_films, ntot, stream, &prof))
             return 1;
         launches++;
@@ -86,7 +87,7 @@ int mms_batch::one_step(float dt) {
 
 static void batch_free(mms_batch *b) {
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    void *bufs[] = {b->m32[0], b->m32[1], b->mn64, b->acc64, b->rho, b->phi, b->aos, b->matl,
+    void *bufs[] = {b->m32[0], b->m32[1], b->mn64, b->acc32, b->rho, b->phi, b->aos, b->matl,
                     b->d_films, b->d_tiles, b->d_direct};
     for (void *x : bufs)
         if (x) (void)hipFree(x);
@@ -130,7 +131,7 @@ static int batch_build(mms_batch *b, const mms_film_spec *specs, const unsigned 
     b->prof.stream = b->stream;
     for (int i = 0; i < 2; i++) MMS_HIP(hipMalloc(&b->m32[i], sizeof(float) * 3 * N));
     MMS_HIP(hipMalloc(&b->mn64, sizeof(double) * 3 * N));
-    MMS_HIP(hipMalloc(&b->acc64, sizeof(double) * 3 * N));
+    MMS_HIP(hipMalloc(&b->acc32, sizeof(float) * 3 * N));
     MMS_HIP(hipMalloc(&b->rho, sizeof(float) * N));
     MMS_HIP(hipMalloc(&b->phi, sizeof(float) * N));
     MMS_HIP(hipMalloc(&b->aos, sizeof(float) * 3 * N));
@@ -138,7 +139,7 @@ static int batch_build(mms_batch *b, const mms_film_spec *specs, const unsigned 
     MMS_HIP(hipMemset(b->m32[0], 0, sizeof(float) * 3 * N));
     MMS_HIP(hipMemset(b->m32[1], 0, sizeof(float) * 3 * N));
     MMS_HIP(hipMemset(b->mn64, 0, sizeof(double) * 3 * N));
-    MMS_HIP(hipMemset(b->acc64, 0, sizeof(double) * 3 * N));
+    MMS_HIP(hipMemset(b->acc32, 0, sizeof(float) * 3 * N));
     MMS_HIP(hipMemset(b->phi, 0, sizeof(float) * N));
     if (material) {
         MMS_HIP(hipMemcpy(b->matl, material, N, hipMemcpyHostToDevice));

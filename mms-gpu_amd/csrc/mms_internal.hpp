@@ -148,9 +148,10 @@ struct Material {
     float hext[3];
 };
 int launch_div(const Grid &g, const float *m, float *rho, hipStream_t s, Profiler *prof);
-// stage in 0..3; m_in/m_out FP32 SoA [3][n]; mn/acc FP64 SoA [3][n]; t_dev: device double
+// stage in 0..3; m_in/m_out FP32 SoA [3][n]; mn FP64 SoA [3][n]; acc (slope) FP32 SoA [3][n];
+// t_dev: device double
 int launch_stage(const Grid &g, const Material &mat, int stage, double dt, const float *phi,
-                 const float *m_in, float *m_out, double *mn, double *acc,
+                 const float *m_in, float *m_out, double *mn, float *acc,
                  const uint8_t *matl, const double *t_dev, hipStream_t s, Profiler *prof);
 // H (AoS float) at state m (SoA float); phi may be null if !demag
 int launch_hfield(const Grid &g, const Material &mat, const float *phi, const float *m,
@@ -164,7 +165,7 @@ int launch_energy_h(const float *m, const float *h_aos, const uint8_t *matl, lon
 // dM/dt and the float slope sum.  Null pointers disable each.
 struct StageExtra { const float *h_given; float *h_out; float *k_out; float *slope_out; int normalize; double *mn_out; };
 int launch_stage_ex(const Grid &g, const Material &mat, int stage, double dt, const float *phi,
-                    const float *m_in, float *m_out, double *mn, double *acc, const uint8_t *matl,
+                    const float *m_in, float *m_out, double *mn, float *acc, const uint8_t *matl,
                     const double *t_dev, const StageExtra &ex, hipStream_t s, Profiler *prof);
 // Block partials [nblk][5] of (sum Mx, My, Mz, |M|, count) over material cells
 int launch_mavg(const float *m, const uint8_t *matl, long n, double *partial, int nblk, hipStream_t s);
@@ -175,7 +176,7 @@ constexpr int STENCIL_TILE_X = 64, STENCIL_TILE_Y = 4;
 int launch_div_batch(const float *m, float *rho, const int4 *tiles, int ntiles, const BatchFilm *films, long pstride,
                      float dx, float dy, float dz, int zd, hipStream_t s, Profiler *prof);
 int launch_stage_batch(const Grid &g, const Material &mat, int stage, double dt, const float *phi, const float *m_in,
-                       float *m_out, double *mn, double *acc, const uint8_t *matl, const int4 *tiles, int ntiles,
+                       float *m_out, double *mn, float *acc, const uint8_t *matl, const int4 *tiles, int ntiles,
                        const BatchFilm *films, long pstride, hipStream_t s, Profiler *prof);
 int direct_batch_potential(const float *rho, float *phi, const int2 *items, int nitems, const BatchFilm *films,
                            float ay, float az, double scale, hipStream_t s, Profiler *prof);
@@ -196,7 +197,7 @@ struct mms_ctx {
     float *m32[3] = {nullptr, nullptr, nullptr};  // FP32 stage magnetisation slots
     double *mn64 = nullptr;    // FP64 base M_n
     double *mnew64 = nullptr;  // FP64 proposal M_{n+1} (adaptive time loop)
-    double *acc64 = nullptr;   // FP64 slope accumulator
+    float *acc32 = nullptr;    // FP32 slope accumulator (k1 + 2 k2 + 2 k3 + k4)
     float *rho = nullptr, *phi = nullptr;
     float *phi_n = nullptr;    // potential at M_n (adaptive time loop)
     uint8_t *matl = nullptr;

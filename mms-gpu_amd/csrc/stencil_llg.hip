@@ -87,7 +87,7 @@ struct StageArgs {
     const float *m_in;     // FP32 SoA stage magnetisation
     float *m_out;          // FP32 SoA next stage magnetisation
     double *mn;            // FP64 SoA base M_n (updated at stage 3)
-    double *acc;           // FP64 SoA slope accumulator
+    float *acc;            // FP32 SoA slope accumulator
     double *mn_out;        // FP64 SoA destination of M_{n+1} at stage 3 (null = mn)
     const uint8_t *matl;   // material flags (null = all magnetic)
     const double *t_dev;   // simulation time (device)
@@ -137,7 +137,7 @@ __device__ __forceinline__ void stage_column(const StageArgs &A, int x, int y, i
     const float *m_in_ = A.m_in + off;
     float *m_out_ = A.m_out + off;
     double *mn_ = A.mn + off;
-    double *acc_ = A.acc + off;
+    float *acc_ = A.acc + off;
     double *mn_out_ = A.mn_out ? A.mn_out + off : nullptr;
     const float *phi_ = A.phi ? A.phi + off : nullptr;
     const uint8_t *matl_ = A.matl ? A.matl + off : nullptr;
@@ -206,19 +206,23 @@ __device__ __forceinline__ void stage_column(const StageArgs &A, int x, int y, i
             float kx, ky, kz;
             llg_rhs(mxc, myc, mzc, Hx, Hy, Hz, A.alfa, A.gamma, A.Ms, kx, ky, kz);
             if (A.k_out) { A.k_out[3 * c] = kx; A.k_out[3 * c + 1] = ky; A.k_out[3 * c + 2] = kz; }
-            // the FP64 base and slope accumulator are touched once per stage:
-            // non-temporal loads and stores keep them from evicting the
-            // neighbour-reused FP32 m and phi from the caches
+            // the FP64 base and the slope accumulator are touched once per
+            // stage: non-temporal loads and stores keep them from evicting
+            // the neighbour-reused FP32 m and phi from the caches.  The slope
+            // k1 + 2 k2 + 2 k3 + k4 is summed in FP32 as in the reference
+            // (ode_functions.cpp:105-140); the combine M + dt/6 slope and the
+            // renormalisation are FP64.
             double *bx = mn_, *by = mn_ + n, *bz = mn_ + 2 * n;
-            double *ax = acc_, *ay = acc_ + n, *az = acc_ + 2 * n;
+            float *ax = acc_, *ay = acc_ + n, *az = acc_ + 2 * n;
             const double b0 = __builtin_nontemporal_load(&bx[c]), b1 = __builtin_nontemporal_load(&by[c]),
                          b2 = __builtin_nontemporal_load(&bz[c]);
-            double s0, s1, s2;
+            float s0, s1, s2;
             if (A.stage == 0) { s0 = kx; s1 = ky; s2 = kz; }
-            else { s0 = __builtin_nontemporal_load(&ax[c]) + (A.stage == 3 ? 1.0 : 2.0) * kx;
-                   s1 = __builtin_nontemporal_load(&ay[c]) + (A.stage == 3 ? 1.0 : 2.0) * ky;
-                   s2 = __builtin_nontemporal_load(&az[c]) + (A.stage == 3 ? 1.0 : 2.0) * kz; }
-            if (A.slope_out) { A.slope_out[3 * c] = (float)s0; A.slope_out[3 * c + 1] = (float)s1; A.slope_out[3 * c + 2] = (float)s2; }
+            else { const float wk = (A.stage == 3) ? 1.0f : 2.0f;  // w k is exact
+                   s0 = __builtin_nontemporal_load(&ax[c]) + wk * kx;
+                   s1 = __builtin_nontemporal_load(&ay[c]) + wk * ky;
+                   s2 = __builtin_nontemporal_load(&az[c]) + wk * kz; }
+            if (A.slope_out) { A.slope_out[3 * c] = s0; A.slope_out[3 * c + 1] = s1; A.slope_out[3 * c + 2] = s2; }
             double n0, n1, n2;
             if (A.stage < 3) {
                 const double h = (A.stage < 2 ? 0.5 : 1.0) * A.dt;
@@ -228,7 +232,7 @@ __device__ __forceinline__ void stage_column(const StageArgs &A, int x, int y, i
                 __builtin_nontemporal_store(s2, &az[c]);
             } else {
                 const double h = A.dt / 6.0;
-                n0 = b0 + h * s0; n1 = b1 + h * s1; n2 = b2 + h * s2;
+                n0 = b0 + h * (double)s0; n1 = b1 + h * (double)s1; n2 = b2 + h * (double)s2;
                 if (A.normalize) {
                     const double f = (double)A.Ms / sqrt(n0 * n0 + n1 * n1 + n2 * n2);
                     n0 *= f; n1 *= f; n2 *= f;
@@ -420,7 +424,7 @@ int launch_div(const Grid &g, const float *m, float *rho, hipStream_t s, Profile
 }
 
 int launch_stage_ex(const Grid &g, const Material &mat, int stage, double dt, const float *phi,
-                    const float *m_in, float *m_out, double *mn, double *acc, const uint8_t *matl,
+                    const float *m_in, float *m_out, double *mn, float *acc, const uint8_t *matl,
                     const double *t_dev, const StageExtra &ex, hipStream_t s, Profiler *prof) {
     KTimer kt(prof, K_STAGE);
     StageArgs A = make_args(g, mat);
@@ -439,7 +443,7 @@ int launch_stage_ex(const Grid &g, const Material &mat, int stage, double dt, co
 }
 
 int launch_stage(const Grid &g, const Material &mat, int stage, double dt, const float *phi,
-                 const float *m_in, float *m_out, double *mn, double *acc, const uint8_t *matl,
+                 const float *m_in, float *m_out, double *mn, float *acc, const uint8_t *matl,
                  const double *t_dev, hipStream_t s, Profiler *prof) {
     StageExtra ex{nullptr, nullptr, nullptr, nullptr, 1, nullptr};
     return launch_stage_ex(g, mat, stage, dt, phi, m_in, m_out, mn, acc, matl, t_dev, ex, s, prof);
@@ -469,7 +473,7 @@ int launch_div_batch(const float *m, float *rho, const int4 *tiles, int ntiles, 
 }
 
 int launch_stage_batch(const Grid &g, const Material &mat, int stage, double dt, const float *phi, const float *m_in,
-                       float *m_out, double *mn, double *acc, const uint8_t *matl, const int4 *tiles, int ntiles,
+                       float *m_out, double *mn, float *acc, const uint8_t *matl, const int4 *tiles, int ntiles,
                        const BatchFilm *films, long pstride, hipStream_t s, Profiler *prof) {
     KTimer kt(prof, K_STAGE);
     if (ntiles <= 0) return 0;

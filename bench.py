@@ -37,11 +37,12 @@ def log(*a):
 # ----------------------------------------------------- algorithmic work --
 def stage_bytes_per_cell(stage, all_magnetic=False):
     """Compulsory HBM bytes of the fused field+LLG stage kernel per cell:
-    phi 4 + m_s 12 + FP64 base 24 + material 1 + FP32 m_{s+1} 12, plus the
-    FP32 slope accumulator (write 12 at stages 1-3, read 12 at stages 2-4)
-    and the FP64 base write 24 at stage 4: 65, 77, 77, 89 bytes.  An
-    all-magnetic film does not read the material byte."""
-    return (65, 77, 77, 89)[stage] - (1 if all_magnetic else 0)
+    phi 4 + m_s 12 + material 1 + FP32 m_{s+1} 12, the FP32 slope
+    accumulator (write 12 at stages 1-3, read 12 at stages 2-4), and M_n:
+    none at stage 1 (its input is M_n), the FP32 copy (12) at stages 2-3,
+    the FP64 base read and write (24 + 24) at stage 4: 41, 65, 65, 89 bytes.
+    An all-magnetic film does not read the material byte."""
+    return (41, 65, 65, 89)[stage] - (1 if all_magnetic else 0)
 
 
 def m2l_flops_per_eval(side, zd, P, w, m2l):

@@ -129,18 +129,20 @@ int mms_ctx::potential(const float *rho_d, float *phi_d) {
 }
 
 // one fixed-dt RK4 step on the device: four (div, potential, stage) passes;
-// the stage-0 field is evaluated at M_n.  State returns to m32[cur].
+// the stage-0 field is evaluated at M_n.  State returns to m32[cur], which
+// keeps the FP32 copy of M_n through stages 0-2 (the other two slots carry
+// the stage values).
 int mms_ctx::one_step(float dt) {
-    const int a = cur, b = 1 - cur;  // ping-pong between m32[0] and m32[1]
-    const float *in[4] = {m32[a], m32[b], m32[a], m32[b]};
-    float *out[4] = {m32[b], m32[a], m32[b], m32[a]};
+    const int a = cur, b = (cur + 1) % 3, d = (cur + 2) % 3;
+    const float *in[4] = {m32[a], m32[b], m32[d], m32[b]};
+    float *out[4] = {m32[b], m32[d], m32[b], m32[a]};
     for (int s = 0; s < 4; s++) {
         if (p.demag) {
             if (launch_div(g, in[s], rho, stream, &prof)) return 1;
             if (potential(rho, phi)) return 1;
         }
         // an all-magnetic film skips the per-cell material byte
-        if (launch_stage(g, mat, s, (double)dt, phi, in[s], out[s], mn64, acc32, all_magnetic ? nullptr : matl,
+        if (launch_stage(g, mat, s, (double)dt, phi, in[s], out[s], mn64, m32[a], acc32, all_magnetic ? nullptr : matl,
                          t_dev, stream, &prof))
             return 1;
     }
@@ -626,7 +628,7 @@ extern "C" int rk4_step(float t, float dt, float *t2, const unsigned char *mater
         set_error("rk4_step: upload failed");
         return EXIT_FAILURE;
     }
-    StageExtra ex{nullptr, nullptr, nullptr, nullptr, normalize, nullptr};
+    StageExtra ex{nullptr, nullptr, nullptr, nullptr, normalize, nullptr, c->m32[0]};
     const float *in[4] = {c->m32[0], c->m32[1], c->m32[2], c->m32[1]};
     float *outp[4] = {c->m32[1], c->m32[2], c->m32[1], c->m32[2]};
     for (int st = 0; st < 4; st++) {
@@ -741,7 +743,7 @@ extern "C" int time_marching(unsigned char *material, float *M, float finaltime,
         const int a = c->cur, b = (c->cur + 1) % 3, d = (c->cur + 2) % 3;
         const float *in[4] = {c->m32[a], c->m32[b], c->m32[d], c->m32[b]};
         float *outp[4] = {c->m32[b], c->m32[d], c->m32[b], c->m32[d]};
-        StageExtra ex{nullptr, nullptr, nullptr, nullptr, 1, c->mnew64};
+        StageExtra ex{nullptr, nullptr, nullptr, nullptr, 1, c->mnew64, c->m32[a]};
         for (int st = 0; st < 4 && !status; st++) {
             const float *phi_use = (st == 0) ? c->phi_n : c->phi;
             if (st > 0 && demag) {

@@ -29,7 +29,7 @@ struct mms_batch {
     std::vector<BatchFilm> films;
     std::vector<int> solver;  // per film: 0 direct, 1 fmm
     hipStream_t stream = nullptr;
-    float *m32[2] = {nullptr, nullptr};
+    float *m32[3] = {nullptr, nullptr, nullptr};  // state slot cur + two stage slots
     double *mn64 = nullptr;
     float *acc32 = nullptr;  // FP32 slope accumulator
     float *rho = nullptr, *phi = nullptr, *aos = nullptr;
@@ -54,9 +54,10 @@ struct mms_batch {
 static const long kDirectMaxCells = 4096;
 
 int mms_batch::one_step(float dt) {
-    const int a = cur, b = 1 - cur;
-    const float *in[4] = {m32[a], m32[b], m32[a], m32[b]};
-    float *out[4] = {m32[b], m32[a], m32[b], m32[a]};
+    // m32[a] keeps the FP32 copy of M_n through stages 0-2; stage 3 writes M_{n+1} there
+    const int a = cur, b = (cur + 1) % 3, d = (cur + 2) % 3;
+    const float *in[4] = {m32[a], m32[b], m32[d], m32[b]};
+    float *out[4] = {m32[b], m32[d], m32[b], m32[a]};
     const double scale = (double)(gcommon.dx * gcommon.dy * gcommon.dz) / (double)gcommon.dx;
     int launches = 0;
     for (int s = 0; s < 4; s++) {
@@ -76,7 +77,7 @@ int mms_batch::one_step(float dt) {
                 launches += 2 * fmm[k]->Lleaf - 1;
             }
         }
-        if (launch_stage_batch(gcommon, mat, s, (double)dt, phi, in[s], out[s], mn64, acc32, matl, d_tiles, ntiles,
+        if (launch_stage_batch(gcommon, mat, s, (double)dt, phi, in[s], out[s], mn64, m32[a], acc32, matl, d_tiles, ntiles,
                                d_films, ntot, stream, &prof))
             return 1;
         launches++;
@@ -87,7 +88,7 @@ int mms_batch::one_step(float dt) {
 
 static void batch_free(mms_batch *b) {
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    void *bufs[] = {b->m32[0], b->m32[1], b->mn64, b->acc32, b->rho, b->phi, b->aos, b->matl,
+    void *bufs[] = {b->m32[0], b->m32[1], b->m32[2], b->mn64, b->acc32, b->rho, b->phi, b->aos, b->matl,
                     b->d_films, b->d_tiles, b->d_direct};
     for (void *x : bufs)
         if (x) (void)hipFree(x);
@@ -129,7 +130,7 @@ static int batch_build(mms_batch *b, const mms_film_spec *specs, const unsigned 
     const long N = b->ntot;
     MMS_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     b->prof.stream = b->stream;
-    for (int i = 0; i < 2; i++) MMS_HIP(hipMalloc(&b->m32[i], sizeof(float) * 3 * N));
+    for (int i = 0; i < 3; i++) MMS_HIP(hipMalloc(&b->m32[i], sizeof(float) * 3 * N));
     MMS_HIP(hipMalloc(&b->mn64, sizeof(double) * 3 * N));
     MMS_HIP(hipMalloc(&b->acc32, sizeof(float) * 3 * N));
     MMS_HIP(hipMalloc(&b->rho, sizeof(float) * N));
@@ -138,6 +139,7 @@ static int batch_build(mms_batch *b, const mms_film_spec *specs, const unsigned 
     MMS_HIP(hipMalloc(&b->matl, N));
     MMS_HIP(hipMemset(b->m32[0], 0, sizeof(float) * 3 * N));
     MMS_HIP(hipMemset(b->m32[1], 0, sizeof(float) * 3 * N));
+    MMS_HIP(hipMemset(b->m32[2], 0, sizeof(float) * 3 * N));
     MMS_HIP(hipMemset(b->mn64, 0, sizeof(double) * 3 * N));
     MMS_HIP(hipMemset(b->acc32, 0, sizeof(float) * 3 * N));
     MMS_HIP(hipMemset(b->phi, 0, sizeof(float) * N));

@@ -151,7 +151,7 @@ int launch_div(const Grid &g, const float *m, float *rho, hipStream_t s, Profile
 // stage in 0..3; m_in/m_out FP32 SoA [3][n]; mn FP64 SoA [3][n]; acc (slope) FP32 SoA [3][n];
 // t_dev: device double
 int launch_stage(const Grid &g, const Material &mat, int stage, double dt, const float *phi,
-                 const float *m_in, float *m_out, double *mn, float *acc,
+                 const float *m_in, float *m_out, double *mn, const float *m_base, float *acc,
                  const uint8_t *matl, const double *t_dev, hipStream_t s, Profiler *prof);
 // H (AoS float) at state m (SoA float); phi may be null if !demag
 int launch_hfield(const Grid &g, const Material &mat, const float *phi, const float *m,
@@ -163,7 +163,10 @@ int launch_energy_h(const float *m, const float *h_aos, const uint8_t *matl, lon
 // Stage kernel with the rk4_step drop-in's extra inputs/outputs (AoS):
 // h_given replaces the computed field; h_out/k_out/slope_out receive the field,
 // dM/dt and the float slope sum.  Null pointers disable each.
-struct StageExtra { const float *h_given; float *h_out; float *k_out; float *slope_out; int normalize; double *mn_out; };
+// m_base: FP32 copy of M_n that stays intact through stages 0-2 (the stage-0
+// input); null makes every stage read the FP64 base
+struct StageExtra { const float *h_given; float *h_out; float *k_out; float *slope_out; int normalize; double *mn_out;
+                    const float *m_base; };
 int launch_stage_ex(const Grid &g, const Material &mat, int stage, double dt, const float *phi,
                     const float *m_in, float *m_out, double *mn, float *acc, const uint8_t *matl,
                     const double *t_dev, const StageExtra &ex, hipStream_t s, Profiler *prof);
@@ -176,8 +179,9 @@ constexpr int STENCIL_TILE_X = 64, STENCIL_TILE_Y = 4;
 int launch_div_batch(const float *m, float *rho, const int4 *tiles, int ntiles, const BatchFilm *films, long pstride,
                      float dx, float dy, float dz, int zd, hipStream_t s, Profiler *prof);
 int launch_stage_batch(const Grid &g, const Material &mat, int stage, double dt, const float *phi, const float *m_in,
-                       float *m_out, double *mn, float *acc, const uint8_t *matl, const int4 *tiles, int ntiles,
-                       const BatchFilm *films, long pstride, hipStream_t s, Profiler *prof);
+                       float *m_out, double *mn, const float *m_base, float *acc, const uint8_t *matl,
+                       const int4 *tiles, int ntiles, const BatchFilm *films, long pstride, hipStream_t s,
+                       Profiler *prof);
 int direct_batch_potential(const float *rho, float *phi, const int2 *items, int nitems, const BatchFilm *films,
                            float ay, float az, double scale, hipStream_t s, Profiler *prof);
 int launch_aos_from_soa(const float *soa32, float *aos, long n, hipStream_t s);

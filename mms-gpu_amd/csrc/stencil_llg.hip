@@ -87,6 +87,8 @@ struct StageArgs {
     const float *m_in;     // FP32 SoA stage magnetisation
     float *m_out;          // FP32 SoA next stage magnetisation
     double *mn;            // FP64 SoA base M_n (updated at stage 3)
+    const float *m_base;   // FP32 SoA copy of M_n for the stage values of stages 1, 2
+                           // (stage 0: the input itself; null: the FP64 base)
     float *acc;            // FP32 SoA slope accumulator
     double *mn_out;        // FP64 SoA destination of M_{n+1} at stage 3 (null = mn)
     const uint8_t *matl;   // material flags (null = all magnetic)
@@ -212,10 +214,22 @@ __device__ __forceinline__ void stage_column(const StageArgs &A, int x, int y, i
             // k1 + 2 k2 + 2 k3 + k4 is summed in FP32 as in the reference
             // (ode_functions.cpp:105-140); the combine M + dt/6 slope and the
             // renormalisation are FP64.
+            // Stages 1-3 round their stage value M_n + h k to FP32, so they
+            // take M_n from its FP32 copy (the stage-0 input, 12 bytes
+            // instead of 24); the final combine at stage 4 reads the FP64 base.
             double *bx = mn_, *by = mn_ + n, *bz = mn_ + 2 * n;
             float *ax = acc_, *ay = acc_ + n, *az = acc_ + 2 * n;
-            const double b0 = __builtin_nontemporal_load(&bx[c]), b1 = __builtin_nontemporal_load(&by[c]),
-                         b2 = __builtin_nontemporal_load(&bz[c]);
+            double b0, b1, b2;
+            if (A.stage == 3 || !A.m_base) {
+                b0 = __builtin_nontemporal_load(&bx[c]); b1 = __builtin_nontemporal_load(&by[c]);
+                b2 = __builtin_nontemporal_load(&bz[c]);
+            } else if (A.stage == 0) {
+                b0 = mxc; b1 = myc; b2 = mzc;
+            } else {
+                const float *mb = A.m_base + off;
+                b0 = __builtin_nontemporal_load(&mb[c]); b1 = __builtin_nontemporal_load(&mb[n + c]);
+                b2 = __builtin_nontemporal_load(&mb[2 * n + c]);
+            }
             float s0, s1, s2;
             if (A.stage == 0) { s0 = kx; s1 = ky; s2 = kz; }
             else { const float wk = (A.stage == 3) ? 1.0f : 2.0f;  // w k is exact
@@ -431,7 +445,8 @@ int launch_stage_ex(const Grid &g, const Material &mat, int stage, double dt, co
     A.phi = phi; A.m_in = m_in; A.m_out = m_out; A.mn = mn; A.acc = acc; A.matl = matl;
     A.t_dev = t_dev; A.stage = stage; A.dt = dt;
     A.h_given = ex.h_given; A.h_out = ex.h_out; A.k_out = ex.k_out; A.slope_out = ex.slope_out;
-    A.normalize = ex.normalize; A.mn_out = ex.mn_out;
+    A.normalize = ex.normalize; A.mn_out = ex.mn_out; A.m_base = ex.m_base;
+    MMS_CHECK(stage != 0 || !ex.m_base || ex.m_base == m_in, "stage 0 takes M_n from its own input");
     dim3 grid(xcd_grid(g)), block(BX, BY);
     switch (g.zd) {
     case 3: hipLaunchKernelGGL(stage_kernel<3>, grid, block, 0, s, A); break;
@@ -443,9 +458,9 @@ int launch_stage_ex(const Grid &g, const Material &mat, int stage, double dt, co
 }
 
 int launch_stage(const Grid &g, const Material &mat, int stage, double dt, const float *phi,
-                 const float *m_in, float *m_out, double *mn, float *acc, const uint8_t *matl,
+                 const float *m_in, float *m_out, double *mn, const float *m_base, float *acc, const uint8_t *matl,
                  const double *t_dev, hipStream_t s, Profiler *prof) {
-    StageExtra ex{nullptr, nullptr, nullptr, nullptr, 1, nullptr};
+    StageExtra ex{nullptr, nullptr, nullptr, nullptr, 1, nullptr, m_base};
     return launch_stage_ex(g, mat, stage, dt, phi, m_in, m_out, mn, acc, matl, t_dev, ex, s, prof);
 }
 
@@ -473,12 +488,14 @@ int launch_div_batch(const float *m, float *rho, const int4 *tiles, int ntiles, 
 }
 
 int launch_stage_batch(const Grid &g, const Material &mat, int stage, double dt, const float *phi, const float *m_in,
-                       float *m_out, double *mn, float *acc, const uint8_t *matl, const int4 *tiles, int ntiles,
-                       const BatchFilm *films, long pstride, hipStream_t s, Profiler *prof) {
+                       float *m_out, double *mn, const float *m_base, float *acc, const uint8_t *matl,
+                       const int4 *tiles, int ntiles, const BatchFilm *films, long pstride, hipStream_t s,
+                       Profiler *prof) {
     KTimer kt(prof, K_STAGE);
     if (ntiles <= 0) return 0;
+    MMS_CHECK(stage != 0 || !m_base || m_base == m_in, "stage 0 takes M_n from its own input");
     StageArgs A = make_args(g, mat);
-    A.phi = phi; A.m_in = m_in; A.m_out = m_out; A.mn = mn; A.acc = acc; A.matl = matl;
+    A.phi = phi; A.m_in = m_in; A.m_out = m_out; A.mn = mn; A.m_base = m_base; A.acc = acc; A.matl = matl;
     A.t_dev = nullptr; A.stage = stage; A.dt = dt; A.external = 2;
     if (g.zd == 3)
         hipLaunchKernelGGL(stage_batch_kernel<3>, dim3(ntiles), dim3(BX, BY), 0, s, A, tiles, films, pstride);

@@ -1394,16 +1394,19 @@ int launch_l2l_sum(const FmmPlan &pl, int L, hipStream_t s) {
 }
 
 // Dynamic LDS asked by a near-field launch that runs beside the multipole
-// passes: at least 82 KB, so one near-field workgroup and one M2L workgroup
-// (77 KB) share a CU instead of two near-field workgroups filling it.
-constexpr size_t kNearLdsShared = 82 * 1024;
+// passes.  BF16 split and FP32: at least 82 KB, so one near-field workgroup
+// and one M2L workgroup (77 KB) share a CU instead of two near-field
+// workgroups filling it.  FP16: at least 40 KB (the window needs 34 KB),
+// room for three 39 KB M2L workgroups beside it (measured: 1313 vs 1265
+// steps/s with 82 KB on the 1024 x 1024 x 4 film).
+size_t near_lds_shared(int precision) { return (precision == MMS_M2L_FP16 ? 40 : 82) * 1024; }
 
 template <int K, int KP, int NCB, bool FAR, bool F16>
 int launch_p2p_bf16(const Grid &g, const FmmPlan &pl, const float *rho, float *phi, float scale, bool shared,
                     hipStream_t s) {
     constexpr int SP = 32 * NCB + 16 + 8;
     size_t lds = sizeof(unsigned short) * (F16 ? 1 : 2) * (size_t)g.zd * 48 * SP;
-    if (shared && lds < kNearLdsShared) lds = kNearLdsShared;
+    if (shared && lds < near_lds_shared(pl.precision)) lds = near_lds_shared(pl.precision);
     constexpr int kMaxDyn = 160 * 1024 - 256;  // the FP16 form also has a few bytes of static LDS
     MMS_CHECK(lds <= (size_t)kMaxDyn, "FMM near field: too many layers for the LDS tile");
     static int attr = 0;
@@ -1434,7 +1437,7 @@ int launch_near(const Grid &g, const FmmPlan &pl, const float *rho, float *phi, 
     if constexpr (W == 4 || W == 8) {
         constexpr int TW = 4 * W, SW = TW + 2 * W, TS = 4 * W - 1;
         size_t lds = sizeof(float) * ((size_t)zd * SW * SW + (size_t)(2 * zd - 1) * TS * TS);
-        if (shared && lds < kNearLdsShared) lds = kNearLdsShared;
+        if (shared && lds < near_lds_shared(pl.precision)) lds = near_lds_shared(pl.precision);
         static int attr = 0;
         if (!attr) {
             MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&p2p_l2p_mfma_kernel<K, KP, W, FAR>),

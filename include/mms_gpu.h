@@ -115,9 +115,11 @@ enum mms_m2l_precision {
     MMS_M2L_FP32 = 0,   /* FP32 operands, FP32 accumulation                   */
     MMS_M2L_BF16 = 1,   /* BF16 operands, FP32 accumulation (MFMA)            */
     MMS_M2L_BF16X3 = 2, /* hi/lo BF16 split of both operands, 3 MFMA products */
-    MMS_M2L_FP16 = 3    /* FP16 operands, FP32 accumulation (MFMA): operator
+    MMS_M2L_FP16 = 3,   /* FP16 operands, FP32 accumulation (MFMA): operator
                            rows/columns and each workgroup's source window are
                            scaled by powers of two into FP16 range          */
+    MMS_M2L_BF16X3_FP16NEAR = 4 /* BF16X3 M2L with the near field (leaf
+                           width 8) in scaled FP16, one MFMA product     */
 };
 enum mms_external {
     MMS_EXT_NONE = 0,

@@ -1394,19 +1394,20 @@ int launch_l2l_sum(const FmmPlan &pl, int L, hipStream_t s) {
 }
 
 // Dynamic LDS asked by a near-field launch that runs beside the multipole
-// passes.  BF16 split and FP32: at least 82 KB, so one near-field workgroup
-// and one M2L workgroup (77 KB) share a CU instead of two near-field
-// workgroups filling it.  FP16: at least 40 KB (the window needs 34 KB),
-// room for three 39 KB M2L workgroups beside it (measured: 1313 vs 1265
-// steps/s with 82 KB on the 1024 x 1024 x 4 film).
-size_t near_lds_shared(int precision) { return (precision == MMS_M2L_FP16 ? 40 : 82) * 1024; }
+// passes.  BF16 split and FP32 near field: at least 82 KB, so one near-field
+// workgroup and one BF16 split M2L workgroup (77 KB) share a CU instead of
+// two near-field workgroups filling it.  FP16 near field: at least 40 KB
+// (its window needs 34 KB), beside one 77 KB or three 39 KB M2L workgroups
+// (measured on the 1024 x 1024 x 4 film against 82 KB: 1210 vs 1162 steps/s
+// with the BF16 split M2L, 1313 vs 1265 with the FP16 M2L).
+size_t near_lds_shared(const FmmPlan &pl) { return (pl.near_f16 ? 40 : 82) * 1024; }
 
 template <int K, int KP, int NCB, bool FAR, bool F16>
 int launch_p2p_bf16(const Grid &g, const FmmPlan &pl, const float *rho, float *phi, float scale, bool shared,
                     hipStream_t s) {
     constexpr int SP = 32 * NCB + 16 + 8;
     size_t lds = sizeof(unsigned short) * (F16 ? 1 : 2) * (size_t)g.zd * 48 * SP;
-    if (shared && lds < near_lds_shared(pl.precision)) lds = near_lds_shared(pl.precision);
+    if (shared && lds < near_lds_shared(pl)) lds = near_lds_shared(pl);
     constexpr int kMaxDyn = 160 * 1024 - 256;  // the FP16 form also has a few bytes of static LDS
     MMS_CHECK(lds <= (size_t)kMaxDyn, "FMM near field: too many layers for the LDS tile");
     static int attr = 0;
@@ -1431,13 +1432,13 @@ int launch_near(const Grid &g, const FmmPlan &pl, const float *rho, float *phi, 
                 hipStream_t s) {
     const int zd = g.zd;
     if constexpr (W == 8) {
-        if (pl.precision == MMS_M2L_FP16) return launch_p2p_bf16<K, KP, 2, FAR, true>(g, pl, rho, phi, scale, shared, s);
+        if (pl.near_f16) return launch_p2p_bf16<K, KP, 2, FAR, true>(g, pl, rho, phi, scale, shared, s);
         if (pl.precision != MMS_M2L_FP32) return launch_p2p_bf16<K, KP, 2, FAR, false>(g, pl, rho, phi, scale, shared, s);
     }
     if constexpr (W == 4 || W == 8) {
         constexpr int TW = 4 * W, SW = TW + 2 * W, TS = 4 * W - 1;
         size_t lds = sizeof(float) * ((size_t)zd * SW * SW + (size_t)(2 * zd - 1) * TS * TS);
-        if (shared && lds < near_lds_shared(pl.precision)) lds = near_lds_shared(pl.precision);
+        if (shared && lds < near_lds_shared(pl)) lds = near_lds_shared(pl);
         static int attr = 0;
         if (!attr) {
             MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&p2p_l2p_mfma_kernel<K, KP, W, FAR>),
@@ -1540,7 +1541,7 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
     {
         KTimer kt(prof, K_M2L);
         int st;
-        switch (pl.precision) {
+        switch (pl.m2l_mode) {
         case MMS_M2L_BF16: st = launch_m2l_nc<KP, MMS_M2L_BF16>(pl, s); break;
         case MMS_M2L_BF16X3: st = launch_m2l_nc<KP, MMS_M2L_BF16X3>(pl, s); break;
         case MMS_M2L_FP16: st = launch_m2l_nc<KP, MMS_M2L_FP16>(pl, s); break;
@@ -1651,8 +1652,10 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
     MMS_CHECK(P_ >= 1 && P_ <= 9, "FMM expansion order must be 1..9");
     MMS_CHECK(w_ == 2 || w_ == 4 || w_ == 8, "FMM leaf width must be 2, 4 or 8");
     MMS_CHECK(g.zd >= 1 && g.zd <= 16, "FMM supports 1..16 layers");
-    MMS_CHECK(precision_ >= MMS_M2L_FP32 && precision_ <= MMS_M2L_FP16, "unknown M2L precision");
+    MMS_CHECK(precision_ >= MMS_M2L_FP32 && precision_ <= MMS_M2L_BF16X3_FP16NEAR, "unknown M2L precision");
     P = P_; w = w_; precision = precision_; zd = g.zd;
+    m2l_mode = (precision == MMS_M2L_BF16X3_FP16NEAR) ? MMS_M2L_BF16X3 : precision;
+    near_f16 = (precision == MMS_M2L_FP16 || precision == MMS_M2L_BF16X3_FP16NEAR);
     m2l_cols = (cols_ == 8) ? 8 : 4;
     K = fmm_ncoef(P);
     KP = ((K + 31) / 32) * 32;
@@ -1727,8 +1730,8 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
     }
     // M2L fragments per level: [dz][49][blk]
     const size_t nblk = (size_t)(2 * zd - 1) * 49;
-    m2l_level_bytes = (precision == MMS_M2L_FP32) ? nblk * blk * sizeof(float)
-                                                  : nblk * blk * sizeof(unsigned short) * (precision == MMS_M2L_BF16X3 ? 2 : 1);
+    m2l_level_bytes = (m2l_mode == MMS_M2L_FP32) ? nblk * blk * sizeof(float)
+                                                 : nblk * blk * sizeof(unsigned short) * (m2l_mode == MMS_M2L_BF16X3 ? 2 : 1);
     std::vector<char> all(m2l_level_bytes * (Lleaf - 1));
     std::vector<float> lv(nblk * blk);
     // fragment element i = (blk, rt, lane, u) holds Op[16 rt + lane % 16][G (lane / 16) + u]
@@ -1739,19 +1742,19 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
         std::vector<double> T = ops.m2l_level(L, Lmax, zd, (double)g.ay, (double)g.az);
         for (size_t b = 0; b < nblk; b++) frag(&T[b * K * K], lv, b * blk);
         char *dst = all.data() + (size_t)(L - 2) * m2l_level_bytes;
-        if (precision == MMS_M2L_FP32) {
+        if (m2l_mode == MMS_M2L_FP32) {
             memcpy(dst, lv.data(), m2l_level_bytes);
-        } else if (precision == MMS_M2L_FP16) {
+        } else if (m2l_mode == MMS_M2L_FP16) {
             fp16_levels.insert(fp16_levels.end(), lv.begin(), lv.end());
         } else {
             unsigned short *h = reinterpret_cast<unsigned short *>(dst);
             for (size_t i = 0; i < lv.size(); i++) {
                 h[i] = host_bf16(lv[i]);
-                if (precision == MMS_M2L_BF16X3) h[lv.size() + i] = host_bf16(lv[i] - host_bf2f(h[i]));
+                if (m2l_mode == MMS_M2L_BF16X3) h[lv.size() + i] = host_bf16(lv[i] - host_bf2f(h[i]));
             }
         }
     }
-    if (precision == MMS_M2L_FP16) {
+    if (m2l_mode == MMS_M2L_FP16) {
         // Row and column factors a_j, b_k (powers of two) that bring the
         // largest |a_j T_jk b_k| of every row and column over all levels,
         // offsets and layer distances near 1 (alternating square-root
@@ -1818,7 +1821,7 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
                             const int fl = 32 * ph + 8 * gg + t, row = fl / 24 + d - r + 7, col = fl % 24 - ix + 7;
                             const float v = tab[((size_t)dzi * ts + row) * ts + col];
                             const size_t o = (((size_t)((dzi * 3 + ph) * 14 + di) * 2) * 64 + lane) * 8 + t;
-                            if (precision == MMS_M2L_FP16) {  // plain FP16, lo plane unused
+                            if (near_f16) {  // plain FP16, lo plane unused
                                 const _Float16 hv = (_Float16)v;
                                 memcpy(&af[o], &hv, 2);
                             } else {

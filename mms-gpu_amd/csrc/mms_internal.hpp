@@ -94,7 +94,9 @@ struct FilmDesc { int xd, yd; long off; };
 struct FilmSet { const FilmDesc *desc; long stride; int zd; };  // desc null: single film
 struct FmmPlan {
     int P = 0, K = 0, KP = 0, w = 4, Lmax = 0, Lleaf = 0, zd = 0;
-    int precision = MMS_M2L_FP32;
+    int precision = MMS_M2L_FP32;  // requested form (mms_m2l_precision)
+    int m2l_mode = MMS_M2L_FP32;   // M2L operands: FP32, BF16, BF16X3 or FP16
+    bool near_f16 = false;         // leaf-8 near field in scaled FP16 (else BF16 split / FP32)
     int m2l_cols = 4;        // MFMA column tiles (16 targets each) per wave in M2L: 4 or 8
     // device buffers; box vectors have KP slots (K rounded up to 32, zero padded)
     float *mult = nullptr;   // all levels, [level][z][by][bx][KP]

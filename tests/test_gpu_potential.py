@@ -177,19 +177,22 @@ def test_fmm_linearity_and_sign(mg):
 
 
 @pytest.mark.parametrize("leaf", [4, 8])
-@pytest.mark.parametrize("mode,tol", [("fp32", 2e-6), ("bf16x3", 2e-5), ("bf16", 3e-3), ("fp16", 8e-4)])
+@pytest.mark.parametrize("mode,tol", [("fp32", 2e-6), ("bf16x3", 2e-5), ("bf16", 3e-3), ("fp16", 8e-4),
+                                      ("bf16x3_f16near", 8e-4)])
 def test_fmm_m2l_precision_modes(mg, mode, tol, leaf):
     """M2L on the matrix cores in each operand precision vs the FP64 FMM
     oracle (same P, leaf width, tree): FP32 to rounding; hi/lo BF16 split
     to ~2^-16 of the far field; plain BF16 to ~2^-8 and scaled FP16 to
     ~2^-11 of the far field.  At leaf width 8 the BF16 modes run the near
     field in the hi/lo BF16 split (three products), which stays at the
-    split's ~2^-16, and the FP16 mode runs it in scaled FP16 (~2^-11)."""
+    split's ~2^-16; the FP16 mode and the BF16 split M2L with FP16 near
+    field run it in scaled FP16 (~2^-11)."""
     from oracle import fmm_oracle as fo
     rng = np.random.default_rng(21)
     xd, zd, P = 128, 4, 6
     q = (rng.normal(size=(zd, xd, xd)) * 1e14).astype(F)
-    prec = {"fp32": mg.M2L_FP32, "bf16": mg.M2L_BF16, "bf16x3": mg.M2L_BF16X3, "fp16": mg.M2L_FP16}[mode]
+    prec = {"fp32": mg.M2L_FP32, "bf16": mg.M2L_BF16, "bf16x3": mg.M2L_BF16X3, "fp16": mg.M2L_FP16,
+            "bf16x3_f16near": mg.M2L_BF16X3_FP16NEAR}[mode]
     with mg.Film(xd, xd, zd, *D5, solver=mg.SOLVER_FMM, P=P, m2l_precision=prec, leaf_width=leaf) as f:
         pot = f.potential(q.ravel())
     ref = fo.FmmOracle(xd, xd, zd, P, w=leaf).potential_units(q.astype(float)).ravel() * float(D5[0]) ** 2
@@ -198,7 +201,8 @@ def test_fmm_m2l_precision_modes(mg, mode, tol, leaf):
     assert err < tol
 
 
-@pytest.mark.parametrize("leaf,prec", [(8, "bf16x3"), (8, "fp16"), (8, "fp32"), (4, "fp32")])
+@pytest.mark.parametrize("leaf,prec", [(8, "bf16x3"), (8, "fp16"), (8, "bf16x3_f16near"), (8, "fp32"),
+                                       (4, "fp32")])
 def test_concurrent_near_field_matches_sequential(mg, leaf, prec):
     """The near field runs on a second stream beside the multipole passes
     (fork/join) except inside a profiling window, where every kernel runs in
@@ -206,7 +210,8 @@ def test_concurrent_near_field_matches_sequential(mg, leaf, prec):
     eager calls and in replayed step graphs."""
     from mms_gpu import workloads as wl
     w = wl.c3_domains(side=128, zd=4, domain=16)
-    p = {"fp32": mg.M2L_FP32, "bf16x3": mg.M2L_BF16X3, "fp16": mg.M2L_FP16}[prec]
+    p = {"fp32": mg.M2L_FP32, "bf16x3": mg.M2L_BF16X3, "fp16": mg.M2L_FP16,
+         "bf16x3_f16near": mg.M2L_BF16X3_FP16NEAR}[prec]
     ch = (np.random.default_rng(3).normal(size=w.n) * 1e14).astype(F)
     with mg.Film(*w.dims, *w.d, solver=mg.SOLVER_FMM, P=6, m2l_precision=p, leaf_width=leaf,
                  material=w.material) as f:

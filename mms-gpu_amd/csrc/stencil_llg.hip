@@ -128,28 +128,37 @@ __device__ inline void llg_rhs(float Mx, float My, float Mz, float Hx, float Hy,
     kz = cz * (-gamma) - ddz * a;
 }
 
-// One (x, y) column of one film: film extent xd x yd x zd, its first cell at
-// `off` of the arrays, component planes `n` cells apart.
-template <int ZD>
-__device__ __forceinline__ void stage_column(const StageArgs &A, int x, int y, int xd, int yd, int zd_rt, long off,
-                                             long n, float hx, float hy, float hz) {
+// One (x, y) column of one film: film extent xd x yd x zd, component planes
+// `n` cells apart; the arrays start at the film's first cell.  The arrays are
+// distinct buffers (__restrict__), so the loads of the next z level can be
+// issued ahead of the current level's stores.  ST >= 0: the plain resident
+// step at stage ST (field computed here, no extra outputs, M_{n+1} written
+// over the base), compiled without the per-stage branches; ST = -1: every
+// option at run time.
+template <int ZD, int ST>
+__device__ __forceinline__ void stage_column_r(const StageArgs &A, const float *__restrict__ m_in_,
+                                               float *__restrict__ m_out_, double *__restrict__ mn_,
+                                               float *__restrict__ acc_, double *__restrict__ mn_out_,
+                                               const float *__restrict__ phi_, const uint8_t *__restrict__ matl_,
+                                               const float *__restrict__ mb_, const float *__restrict__ h_given_,
+                                               float *__restrict__ h_out_, float *__restrict__ k_out_,
+                                               float *__restrict__ slope_out_, int x, int y, int xd, int yd,
+                                               int zd_rt, long n, float hx, float hy, float hz) {
     const int zd = ZD ? ZD : zd_rt;
-    if (x >= xd || y >= yd) return;
     const long plane = (long)xd * yd;
-    const float *m_in_ = A.m_in + off;
-    float *m_out_ = A.m_out + off;
-    double *mn_ = A.mn + off;
-    float *acc_ = A.acc + off;
-    double *mn_out_ = A.mn_out ? A.mn_out + off : nullptr;
-    const float *phi_ = A.phi ? A.phi + off : nullptr;
-    const uint8_t *matl_ = A.matl ? A.matl + off : nullptr;
     const long c0 = (long)y * xd + x;
     const float *mx = m_in_, *my = m_in_ + n, *mz = m_in_ + 2 * n;
     const double t = A.t_dev ? *A.t_dev : 0.0;
     float hex = hx, hey = hy, hez = hz;
     if (A.external == 1) { hex = ramp_hx(t, A.inv_mu0); hey = 0.0f; hez = 0.0f; }
     const float dx2 = A.dx * A.dx, dy2 = A.dy * A.dy, dz2 = A.dz * A.dz;
-    const bool need_field = (A.h_given == nullptr);
+    constexpr bool PLAIN = ST >= 0;
+    const int stage = PLAIN ? ST : A.stage;
+    if constexpr (PLAIN) { h_given_ = nullptr; h_out_ = nullptr; k_out_ = nullptr; slope_out_ = nullptr; mn_out_ = nullptr; }
+    const bool need_field = (h_given_ == nullptr);
+    const bool xr = (x != xd - 1), xl = (x != 0), yu = (y != yd - 1), yl = (y != 0);
+    const int ixr = xr ? 1 : 0, ixl = xl ? 1 : 0;
+    const long iyu = yu ? xd : 0, iyl = yl ? xd : 0;
 
     // z-rotation registers
     float pprev = 0.0f, pcur = 0.0f;
@@ -167,47 +176,48 @@ __device__ __forceinline__ void stage_column(const StageArgs &A, int x, int y, i
         if (need_field) {
             Hx = 0.0f; Hy = 0.0f; Hz = 0.0f;
             if (A.demag) {
+                // neighbours outside the film read the cell itself and are
+                // then replaced by 0: unconditional loads, no divergent branches
                 if (ztop) pnext = phi_[c + plane];
-                const float x2 = (x != xd - 1) ? phi_[c + 1] : 0.0f;
-                const float x0 = (x != 0) ? phi_[c - 1] : 0.0f;
-                const float y2 = (y != yd - 1) ? phi_[c + xd] : 0.0f;
-                const float y0 = (y != 0) ? phi_[c - xd] : 0.0f;
+                const float x2 = xr ? phi_[c + ixr] : 0.0f;
+                const float x0 = xl ? phi_[c - ixl] : 0.0f;
+                const float y2 = yu ? phi_[c + iyu] : 0.0f;
+                const float y0 = yl ? phi_[c - iyl] : 0.0f;
                 Hx = (x2 - x0) / (2.0f * A.dx) * A.inv4pi;
                 Hy = (y2 - y0) / (2.0f * A.dy) * A.inv4pi;
                 Hz = (pnext - pprev) / (2.0f * A.dz) * A.inv4pi;
             }
             if (A.exchange) {
-                const bool xr = (x != xd - 1), xl = (x != 0), yu = (y != yd - 1), yl = (y != 0);
                 float Lx, Ly, Lz;
                 {
-                    const float xa = xr ? mx[c + 1] : 0.0f, xb = xl ? mx[c - 1] : 0.0f;
-                    const float ya = yu ? mx[c + xd] : 0.0f, yb = yl ? mx[c - xd] : 0.0f;
+                    const float xa = xr ? mx[c + ixr] : 0.0f, xb = xl ? mx[c - ixl] : 0.0f;
+                    const float ya = yu ? mx[c + iyu] : 0.0f, yb = yl ? mx[c - iyl] : 0.0f;
                     Lx = (xa + xb - 2.0f * mxc) / dx2 + (ya + yb - 2.0f * mxc) / dy2 + (mxn + mxp - 2.0f * mxc) / dz2;
                 }
                 {
-                    const float xa = xr ? my[c + 1] : 0.0f, xb = xl ? my[c - 1] : 0.0f;
-                    const float ya = yu ? my[c + xd] : 0.0f, yb = yl ? my[c - xd] : 0.0f;
+                    const float xa = xr ? my[c + ixr] : 0.0f, xb = xl ? my[c - ixl] : 0.0f;
+                    const float ya = yu ? my[c + iyu] : 0.0f, yb = yl ? my[c - iyl] : 0.0f;
                     Ly = (xa + xb - 2.0f * myc) / dx2 + (ya + yb - 2.0f * myc) / dy2 + (myn + myp - 2.0f * myc) / dz2;
                 }
                 {
-                    const float xa = xr ? mz[c + 1] : 0.0f, xb = xl ? mz[c - 1] : 0.0f;
-                    const float ya = yu ? mz[c + xd] : 0.0f, yb = yl ? mz[c - xd] : 0.0f;
+                    const float xa = xr ? mz[c + ixr] : 0.0f, xb = xl ? mz[c - ixl] : 0.0f;
+                    const float ya = yu ? mz[c + iyu] : 0.0f, yb = yl ? mz[c - iyl] : 0.0f;
                     Lz = (xa + xb - 2.0f * mzc) / dx2 + (ya + yb - 2.0f * mzc) / dy2 + (mzn + mzp - 2.0f * mzc) / dz2;
                 }
                 Hx += A.exch_cm * Lx; Hy += A.exch_cm * Ly; Hz += A.exch_cm * Lz;
             }
             if (A.external) { Hx += hex; Hy += hey; Hz += hez; }
         } else {
-            Hx = A.h_given[3 * c]; Hy = A.h_given[3 * c + 1]; Hz = A.h_given[3 * c + 2];
+            Hx = h_given_[3 * c]; Hy = h_given_[3 * c + 1]; Hz = h_given_[3 * c + 2];
         }
-        if (A.h_out) { A.h_out[3 * c] = Hx; A.h_out[3 * c + 1] = Hy; A.h_out[3 * c + 2] = Hz; }
+        if (h_out_) { h_out_[3 * c] = Hx; h_out_[3 * c + 1] = Hy; h_out_[3 * c + 2] = Hz; }
 
         const bool magnetic = matl_ ? (matl_[c] != 0) : true;
         float *ox = m_out_, *oy = m_out_ + n, *oz = m_out_ + 2 * n;
         if (magnetic) {
             float kx, ky, kz;
             llg_rhs(mxc, myc, mzc, Hx, Hy, Hz, A.alfa, A.gamma, A.Ms, kx, ky, kz);
-            if (A.k_out) { A.k_out[3 * c] = kx; A.k_out[3 * c + 1] = ky; A.k_out[3 * c + 2] = kz; }
+            if (k_out_) { k_out_[3 * c] = kx; k_out_[3 * c + 1] = ky; k_out_[3 * c + 2] = kz; }
             // the FP64 base and the slope accumulator are touched once per
             // stage: non-temporal loads and stores keep them from evicting
             // the neighbour-reused FP32 m and phi from the caches.  The slope
@@ -220,26 +230,25 @@ __device__ __forceinline__ void stage_column(const StageArgs &A, int x, int y, i
             double *bx = mn_, *by = mn_ + n, *bz = mn_ + 2 * n;
             float *ax = acc_, *ay = acc_ + n, *az = acc_ + 2 * n;
             double b0, b1, b2;
-            if (A.stage == 3 || !A.m_base) {
+            if (stage == 3 || !mb_) {
                 b0 = __builtin_nontemporal_load(&bx[c]); b1 = __builtin_nontemporal_load(&by[c]);
                 b2 = __builtin_nontemporal_load(&bz[c]);
-            } else if (A.stage == 0) {
+            } else if (stage == 0) {
                 b0 = mxc; b1 = myc; b2 = mzc;
             } else {
-                const float *mb = A.m_base + off;
-                b0 = __builtin_nontemporal_load(&mb[c]); b1 = __builtin_nontemporal_load(&mb[n + c]);
-                b2 = __builtin_nontemporal_load(&mb[2 * n + c]);
+                b0 = __builtin_nontemporal_load(&mb_[c]); b1 = __builtin_nontemporal_load(&mb_[n + c]);
+                b2 = __builtin_nontemporal_load(&mb_[2 * n + c]);
             }
             float s0, s1, s2;
-            if (A.stage == 0) { s0 = kx; s1 = ky; s2 = kz; }
-            else { const float wk = (A.stage == 3) ? 1.0f : 2.0f;  // w k is exact
+            if (stage == 0) { s0 = kx; s1 = ky; s2 = kz; }
+            else { const float wk = (stage == 3) ? 1.0f : 2.0f;  // w k is exact
                    s0 = __builtin_nontemporal_load(&ax[c]) + wk * kx;
                    s1 = __builtin_nontemporal_load(&ay[c]) + wk * ky;
                    s2 = __builtin_nontemporal_load(&az[c]) + wk * kz; }
-            if (A.slope_out) { A.slope_out[3 * c] = s0; A.slope_out[3 * c + 1] = s1; A.slope_out[3 * c + 2] = s2; }
+            if (slope_out_) { slope_out_[3 * c] = s0; slope_out_[3 * c + 1] = s1; slope_out_[3 * c + 2] = s2; }
             double n0, n1, n2;
-            if (A.stage < 3) {
-                const double h = (A.stage < 2 ? 0.5 : 1.0) * A.dt;
+            if (stage < 3) {
+                const double h = (stage < 2 ? 0.5 : 1.0) * A.dt;
                 n0 = b0 + h * (double)kx; n1 = b1 + h * (double)ky; n2 = b2 + h * (double)kz;
                 __builtin_nontemporal_store(s0, &ax[c]);
                 __builtin_nontemporal_store(s1, &ay[c]);
@@ -266,24 +275,34 @@ __device__ __forceinline__ void stage_column(const StageArgs &A, int x, int y, i
     }
 }
 
-template <int ZD>
+template <int ZD, int ST>
+__device__ __forceinline__ void stage_column(const StageArgs &A, int x, int y, int xd, int yd, int zd_rt, long off,
+                                             long n, float hx, float hy, float hz) {
+    if (x >= xd || y >= yd) return;
+    stage_column_r<ZD, ST>(A, A.m_in + off, A.m_out + off, A.mn + off, A.acc + off, A.mn_out ? A.mn_out + off : nullptr,
+                       A.phi ? A.phi + off : nullptr, A.matl ? A.matl + off : nullptr,
+                       A.m_base ? A.m_base + off : nullptr, A.h_given, A.h_out, A.k_out, A.slope_out, x, y, xd, yd,
+                       zd_rt, n, hx, hy, hz);
+}
+
+template <int ZD, int ST>
 __global__ __launch_bounds__(BX * BY) void stage_kernel(StageArgs A) {
     const int ntx = (A.xd + BX - 1) / BX, nty = (A.yd + BY - 1) / BY;
     int tx, ty;
     if (!xcd_tile(ntx, ntx * nty, tx, ty)) return;
-    stage_column<ZD>(A, tx * BX + threadIdx.x, ty * BY + threadIdx.y, A.xd, A.yd, A.zd, 0, (long)A.xd * A.yd * A.zd,
+    stage_column<ZD, ST>(A, tx * BX + threadIdx.x, ty * BY + threadIdx.y, A.xd, A.yd, A.zd, 0, (long)A.xd * A.yd * A.zd,
                      A.hx, A.hy, A.hz);
 }
 
 // Batched films: block b handles tile tiles[b] = (film, tile x, tile y) of a
 // film whose extent, cell offset and external field come from films[].
-template <int ZD>
+template <int ZD, int ST>
 __global__ __launch_bounds__(BX * BY) void stage_batch_kernel(StageArgs A, const int4 *__restrict__ tiles,
                                                               const BatchFilm *__restrict__ films, long pstride) {
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;  // padding slot of the XCD-ordered tile list
     const BatchFilm f = films[t.x];
-    stage_column<ZD>(A, t.y * BX + threadIdx.x, t.z * BY + threadIdx.y, f.xd, f.yd, f.zd, f.off, pstride, f.hx, f.hy,
+    stage_column<ZD, ST>(A, t.y * BX + threadIdx.x, t.z * BY + threadIdx.y, f.xd, f.yd, f.zd, f.off, pstride, f.hx, f.hy,
                      f.hz);
 }
 
@@ -417,6 +436,33 @@ StageArgs make_args(const Grid &g, const Material &mat) {
     return A;
 }
 
+// The plain resident step runs the stage kernel compiled for its stage; a
+// stage with extra inputs or outputs runs the run-time form.
+template <int ZD>
+void launch_stage_zd(dim3 grid, dim3 block, hipStream_t s, const StageArgs &A, bool plain) {
+    if (!plain) {
+        hipLaunchKernelGGL((stage_kernel<ZD, -1>), grid, block, 0, s, A);
+        return;
+    }
+    switch (A.stage) {
+    case 0: hipLaunchKernelGGL((stage_kernel<ZD, 0>), grid, block, 0, s, A); break;
+    case 1: hipLaunchKernelGGL((stage_kernel<ZD, 1>), grid, block, 0, s, A); break;
+    case 2: hipLaunchKernelGGL((stage_kernel<ZD, 2>), grid, block, 0, s, A); break;
+    default: hipLaunchKernelGGL((stage_kernel<ZD, 3>), grid, block, 0, s, A);
+    }
+}
+
+template <int ZD>
+void launch_stage_batch_zd(dim3 grid, dim3 block, hipStream_t s, const StageArgs &A, const int4 *tiles,
+                           const BatchFilm *films, long pstride) {
+    switch (A.stage) {
+    case 0: hipLaunchKernelGGL((stage_batch_kernel<ZD, 0>), grid, block, 0, s, A, tiles, films, pstride); break;
+    case 1: hipLaunchKernelGGL((stage_batch_kernel<ZD, 1>), grid, block, 0, s, A, tiles, films, pstride); break;
+    case 2: hipLaunchKernelGGL((stage_batch_kernel<ZD, 2>), grid, block, 0, s, A, tiles, films, pstride); break;
+    default: hipLaunchKernelGGL((stage_batch_kernel<ZD, 3>), grid, block, 0, s, A, tiles, films, pstride);
+    }
+}
+
 // workgroups of the XCD-ordered 1D grid: 8 * ceil(tiles / 8)
 unsigned xcd_grid(const Grid &g) {
     const long ntiles = (long)((g.xd + BX - 1) / BX) * ((g.yd + BY - 1) / BY);
@@ -447,11 +493,13 @@ int launch_stage_ex(const Grid &g, const Material &mat, int stage, double dt, co
     A.h_given = ex.h_given; A.h_out = ex.h_out; A.k_out = ex.k_out; A.slope_out = ex.slope_out;
     A.normalize = ex.normalize; A.mn_out = ex.mn_out; A.m_base = ex.m_base;
     MMS_CHECK(stage != 0 || !ex.m_base || ex.m_base == m_in, "stage 0 takes M_n from its own input");
+    MMS_CHECK(stage >= 0 && stage <= 3, "RK4 stage must be 0..3");
     dim3 grid(xcd_grid(g)), block(BX, BY);
+    const bool plain = !ex.h_given && !ex.h_out && !ex.k_out && !ex.slope_out && !ex.mn_out;
     switch (g.zd) {
-    case 3: hipLaunchKernelGGL(stage_kernel<3>, grid, block, 0, s, A); break;
-    case 4: hipLaunchKernelGGL(stage_kernel<4>, grid, block, 0, s, A); break;
-    default: hipLaunchKernelGGL(stage_kernel<0>, grid, block, 0, s, A);
+    case 3: launch_stage_zd<3>(grid, block, s, A, plain); break;
+    case 4: launch_stage_zd<4>(grid, block, s, A, plain); break;
+    default: launch_stage_zd<0>(grid, block, s, A, plain);
     }
     MMS_HIP(hipGetLastError());
     return 0;
@@ -497,10 +545,11 @@ int launch_stage_batch(const Grid &g, const Material &mat, int stage, double dt,
     StageArgs A = make_args(g, mat);
     A.phi = phi; A.m_in = m_in; A.m_out = m_out; A.mn = mn; A.m_base = m_base; A.acc = acc; A.matl = matl;
     A.t_dev = nullptr; A.stage = stage; A.dt = dt; A.external = 2;
+    MMS_CHECK(stage >= 0 && stage <= 3, "RK4 stage must be 0..3");
     if (g.zd == 3)
-        hipLaunchKernelGGL(stage_batch_kernel<3>, dim3(ntiles), dim3(BX, BY), 0, s, A, tiles, films, pstride);
+        launch_stage_batch_zd<3>(dim3(ntiles), dim3(BX, BY), s, A, tiles, films, pstride);
     else
-        hipLaunchKernelGGL(stage_batch_kernel<0>, dim3(ntiles), dim3(BX, BY), 0, s, A, tiles, films, pstride);
+        launch_stage_batch_zd<0>(dim3(ntiles), dim3(BX, BY), s, A, tiles, films, pstride);
     MMS_HIP(hipGetLastError());
     return 0;
 }

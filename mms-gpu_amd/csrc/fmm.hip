@@ -966,29 +966,27 @@ __global__ __launch_bounds__(256) void p2p_l2p_kernel(const float *__restrict__ 
     phi[(long)zt * plane + (long)y * xd + x] = (near + far) * scale;
 }
 
-// -------------------------------------------- P2P + L2P on the matrix cores --
+// ---------------------------------------------- P2P on the matrix cores --
 // Leaf width W (4 or 8): a leaf box has W*W cells = RT 16-row MFMA tiles.  A
 // block owns 4 x 4 leaf boxes (4W x 4W cells, one 16-column tile) and all
 // layers; wave zt (looping when zd > 4) computes for its target layer
-//   phi[cell i of box j] = sum_k G[i][k] q_j[k] + sum_s L2P[i][s] L_j[s]
+//   phi[cell i of box j] = sum_k G[i][k] q_j[k]
 // where k runs over the 3W x 3W x zd source window of box j (its 3 x 3
 // neighbour leaves in every layer) and G[i][k] = 1/r is read from the LDS
 // offset table (self entry 0).  FP32 MFMA 16x16x4 with k = (zs, sy, sx) and
-// sx in groups of 4 matching the 4 lane groups.
-template <int K, int KP, int W, bool FAR>
-__global__ __launch_bounds__(256) void p2p_l2p_mfma_kernel(const float *__restrict__ rho, float *__restrict__ phi,
-                                                           const float *__restrict__ Lleaf,
-                                                           const float *__restrict__ l2pfrag,
-                                                           const float *__restrict__ table, int xd, int yd, int zd,
-                                                           int nb, float scale, FilmSet fs) {
+// sx in groups of 4 matching the 4 lane groups.  The downward pass adds the
+// far field afterwards.
+template <int W>
+__global__ __launch_bounds__(256) void p2p_mfma_kernel(const float *__restrict__ rho, float *__restrict__ phi,
+                                                       const float *__restrict__ table, int xd, int yd, int zd,
+                                                       float scale, FilmSet fs) {
     constexpr int TW = 4 * W, SW = TW + 2 * W, TS = 4 * W - 1;
     if (fs.desc) {
         const FilmDesc d = fs.desc[blockIdx.z];
         xd = d.xd; yd = d.yd; rho += d.off; phi += d.off;
         if ((int)blockIdx.x * TW >= xd || (int)blockIdx.y * TW >= yd) return;  // block-uniform
     }
-    Lleaf += (long)blockIdx.z * fs.stride;
-    constexpr int G = KP / 4, RT = W * W / 16, NSG = 3 * W / 4;
+    constexpr int RT = W * W / 16, NSG = 3 * W / 4;
     extern __shared__ float lds[];
     float *src = lds;                      // [zd][SW][SW]
     float *tab = lds + zd * SW * SW;       // [2zd-1][TS][TS]
@@ -1059,23 +1057,6 @@ __global__ __launch_bounds__(256) void p2p_l2p_mfma_kernel(const float *__restri
                 }
             }
         }
-        // far field: L2P[W*W x KP] * L[KP x 16 boxes] (skipped when the
-        // downward pass adds the far field itself)
-        const int bxg = blockIdx.x * 4 + jx, byg = blockIdx.y * 4 + jy;
-        const bool boxok = (bxg < nb && byg < nb);
-        const float *Lv = Lleaf + (((long)zt * nb + (boxok ? byg : 0)) * nb + (boxok ? bxg : 0)) * KP + g * G;
-        if constexpr (FAR) {
-#pragma unroll
-            for (int u4 = 0; u4 < G / 4; u4++) {
-                f32x4 b = boxok ? *reinterpret_cast<const f32x4 *>(Lv + 4 * u4) : f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int r = 0; r < RT; r++) {
-                    const f32x4 a = *reinterpret_cast<const f32x4 *>(l2pfrag + ((long)r * 64 + lane) * G + 4 * u4);
-#pragma unroll
-                    for (int u = 0; u < 4; u++) acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], acc[r], 0, 0, 0);
-                }
-            }
-        }
         // D row 4g + q of tile r = cell 16 r + 4g + q of box j16: 4 consecutive x cells
 #pragma unroll
         for (int r = 0; r < RT; r++) {
@@ -1109,13 +1090,11 @@ __global__ __launch_bounds__(256) void p2p_l2p_mfma_kernel(const float *__restri
 // window is scaled by s = 2^e (its largest |charge| in [2^14, 2^15)) before
 // the FP16 conversion and the sums by 1 / s afterwards (exact), the 1/r
 // operator (values in (0, 1]) is stored as plain FP16.
-// L2P is fused as in the FP32 kernel.
-template <int K, int KP, int NCB, bool FAR, bool F16>
-__global__ __launch_bounds__(256) void p2p_l2p_bf16_kernel(const float *__restrict__ rho, float *__restrict__ phi,
-                                                           const float *__restrict__ Lleaf,
-                                                           const float *__restrict__ l2pfrag,
-                                                           const unsigned short *__restrict__ afrag, int xd, int yd,
-                                                           int zd, int nb, float scale, FilmSet fs) {
+// The downward pass adds the far field afterwards.
+template <int NCB, bool F16>
+__global__ __launch_bounds__(256) void p2p_16bit_kernel(const float *__restrict__ rho, float *__restrict__ phi,
+                                                        const unsigned short *__restrict__ afrag, int xd, int yd,
+                                                        int zd, float scale, FilmSet fs) {
     // block = (4 NCB) x 4 boxes; NCB column tiles of 16 boxes share each operator load
     constexpr int W = 8, TWX = 32 * NCB, TWY = 32, SWX = TWX + 16, SWY = 48, SP = SWX + 8;
     if (fs.desc) {
@@ -1123,8 +1102,7 @@ __global__ __launch_bounds__(256) void p2p_l2p_bf16_kernel(const float *__restri
         xd = d.xd; yd = d.yd; rho += d.off; phi += d.off;
         if ((int)blockIdx.x * TWX >= xd || (int)blockIdx.y * TWY >= yd) return;  // block-uniform
     }
-    Lleaf += (long)blockIdx.z * fs.stride;
-    constexpr int G = KP / 4, RT = 4;
+    constexpr int RT = 4;
     extern __shared__ __attribute__((aligned(16))) unsigned short sh[];
     unsigned short *qh = sh, *ql = sh + zd * SWY * SP;  // [zd][SWY][SP] hi, lo (F16: hi only)
     typedef typename std::conditional<F16, f16x8, bf16x8>::type BV;
@@ -1300,23 +1278,6 @@ __global__ __launch_bounds__(256) void p2p_l2p_bf16_kernel(const float *__restri
 #pragma unroll
         for (int c = 0; c < NCB; c++) {
             const int jx = 4 * c + (j16 & 3);
-            // far field: L2P[64 x KP] * L[KP x 16 boxes]
-            const int bxg = blockIdx.x * 4 * NCB + jx, byg = blockIdx.y * 4 + jy;
-            const bool boxok = (bxg < nb && byg < nb);
-            const float *Lv = Lleaf + (((long)zt * nb + (boxok ? byg : 0)) * nb + (boxok ? bxg : 0)) * KP + g * G;
-            if constexpr (FAR) {
-#pragma unroll
-                for (int u4 = 0; u4 < G / 4; u4++) {
-                    f32x4 b = boxok ? *reinterpret_cast<const f32x4 *>(Lv + 4 * u4) : f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                    for (int r = 0; r < RT; r++) {
-                        const f32x4 a = *reinterpret_cast<const f32x4 *>(l2pfrag + ((long)r * 64 + lane) * G + 4 * u4);
-#pragma unroll
-                        for (int u = 0; u < 4; u++)
-                            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], acc[r][c], 0, 0, 0);
-                    }
-                }
-            }
             // D row 4g + q of tile r = cell 16 r + 4g + q of box j16: 4 consecutive x cells
 #pragma unroll
             for (int r = 0; r < RT; r++) {
@@ -1402,9 +1363,9 @@ int launch_l2l_sum(const FmmPlan &pl, int L, hipStream_t s) {
 // with the BF16 split M2L, 1313 vs 1265 with the FP16 M2L).
 size_t near_lds_shared(const FmmPlan &pl) { return (pl.near_f16 ? 40 : 82) * 1024; }
 
-template <int K, int KP, int NCB, bool FAR, bool F16>
-int launch_p2p_bf16(const Grid &g, const FmmPlan &pl, const float *rho, float *phi, float scale, bool shared,
-                    hipStream_t s) {
+template <int NCB, bool F16>
+int launch_p2p_16bit(const Grid &g, const FmmPlan &pl, const float *rho, float *phi, float scale, bool shared,
+                     hipStream_t s) {
     constexpr int SP = 32 * NCB + 16 + 8;
     size_t lds = sizeof(unsigned short) * (F16 ? 1 : 2) * (size_t)g.zd * 48 * SP;
     if (shared && lds < near_lds_shared(pl)) lds = near_lds_shared(pl);
@@ -1412,28 +1373,26 @@ int launch_p2p_bf16(const Grid &g, const FmmPlan &pl, const float *rho, float *p
     MMS_CHECK(lds <= (size_t)kMaxDyn, "FMM near field: too many layers for the LDS tile");
     static int attr = 0;
     if (!attr) {
-        MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&p2p_l2p_bf16_kernel<K, KP, NCB, FAR, F16>),
+        MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&p2p_16bit_kernel<NCB, F16>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDyn));
         attr = 1;
     }
     dim3 grid((g.xd + 32 * NCB - 1) / (32 * NCB), (g.yd + 31) / 32, pl.nfilm);
-    hipLaunchKernelGGL((p2p_l2p_bf16_kernel<K, KP, NCB, FAR, F16>), grid, dim3(256), lds, s, rho, phi,
-                       pl.loc + pl.lvl_off[pl.Lleaf], pl.d_l2pf, pl.d_p2pA, g.xd, g.yd, g.zd, 1 << pl.Lleaf, scale,
-                       pl.films());
+    hipLaunchKernelGGL((p2p_16bit_kernel<NCB, F16>), grid, dim3(256), lds, s, rho, phi, pl.d_p2pA, g.xd, g.yd,
+                       g.zd, scale, pl.films());
     MMS_HIP(hipGetLastError());
     return 0;
 }
 
-// Near field (+ L2P when FAR; leaf width 2 always adds it).  FAR is a
-// template parameter: a run-time branch around the L2P products made the
-// compiler rotate the near-field accumulators across the loop back edge.
-template <int K, int KP, int W, bool FAR>
+// Near field: leaf widths 4 and 8 write the near field alone (the downward
+// pass adds the far field); leaf width 2 adds L2P in the same kernel.
+template <int K, int KP, int W>
 int launch_near(const Grid &g, const FmmPlan &pl, const float *rho, float *phi, float scale, bool shared,
                 hipStream_t s) {
     const int zd = g.zd;
     if constexpr (W == 8) {
-        if (pl.near_f16) return launch_p2p_bf16<K, KP, 2, FAR, true>(g, pl, rho, phi, scale, shared, s);
-        if (pl.precision != MMS_M2L_FP32) return launch_p2p_bf16<K, KP, 2, FAR, false>(g, pl, rho, phi, scale, shared, s);
+        if (pl.near_f16) return launch_p2p_16bit<2, true>(g, pl, rho, phi, scale, shared, s);
+        if (pl.precision != MMS_M2L_FP32) return launch_p2p_16bit<2, false>(g, pl, rho, phi, scale, shared, s);
     }
     if constexpr (W == 4 || W == 8) {
         constexpr int TW = 4 * W, SW = TW + 2 * W, TS = 4 * W - 1;
@@ -1441,14 +1400,13 @@ int launch_near(const Grid &g, const FmmPlan &pl, const float *rho, float *phi, 
         if (shared && lds < near_lds_shared(pl)) lds = near_lds_shared(pl);
         static int attr = 0;
         if (!attr) {
-            MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&p2p_l2p_mfma_kernel<K, KP, W, FAR>),
+            MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&p2p_mfma_kernel<W>),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
             attr = 1;
         }
         MMS_CHECK(lds <= 160 * 1024, "FMM near field: too many layers for the LDS tile");
         dim3 grid((g.xd + TW - 1) / TW, (g.yd + TW - 1) / TW, pl.nfilm);
-        hipLaunchKernelGGL((p2p_l2p_mfma_kernel<K, KP, W, FAR>), grid, dim3(256), lds, s, rho, phi,
-                           pl.loc + pl.lvl_off[pl.Lleaf], pl.d_l2pf, pl.d_near, g.xd, g.yd, zd, 1 << pl.Lleaf,
+        hipLaunchKernelGGL((p2p_mfma_kernel<W>), grid, dim3(256), lds, s, rho, phi, pl.d_near, g.xd, g.yd, zd,
                            scale, pl.films());
     } else {
         constexpr int TW = 16, SW = TW + 2 * W, TS = 4 * W - 1;
@@ -1486,7 +1444,7 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
             MMS_HIP(hipEventRecord(pl.ev_fork, s));
         } else {
             KTimer kt(prof, K_P2P_L2P);
-            if (launch_near<K, KP, W, false>(g, pl, rho, phi, scale, false, s)) return 1;
+            if (launch_near<K, KP, W>(g, pl, rho, phi, scale, false, s)) return 1;
         }
     }
     {
@@ -1513,7 +1471,7 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
         // caller's queue and puts the near field on the second one
         if (conc) {
             MMS_HIP(hipStreamWaitEvent(pl.s_near, pl.ev_fork, 0));
-            if (launch_near<K, KP, W, false>(g, pl, rho, phi, scale, true, pl.s_near)) return 1;
+            if (launch_near<K, KP, W>(g, pl, rho, phi, scale, true, pl.s_near)) return 1;
             MMS_HIP(hipEventRecord(pl.ev_join, pl.s_near));
         }
     }
@@ -1584,7 +1542,7 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
     }
     if constexpr (!split) {
         KTimer kt(prof, K_P2P_L2P);
-        if (launch_near<K, KP, W, true>(g, pl, rho, phi, scale, false, s)) return 1;
+        if (launch_near<K, KP, W>(g, pl, rho, phi, scale, false, s)) return 1;
     }
     return 0;
 }

@@ -13,7 +13,9 @@
 // accumulator, so summation error grows with the tile, not with N.  Chunk
 // partials go to an FP64 scratch [nchunk][n] and a second kernel adds them in
 // fixed order (bitwise reproducible run to run).  The self pair is excluded by
-// r^2 > 0.
+// r^2 > 0.  1/r is the hardware reciprocal square root (v_rsq_f32): r^2 of
+// two distinct cells is at least min(1, ay^2, az^2) in cell units, never
+// denormal, so the denormal scaling of rsqrtf is not needed.
 #include "mms_internal.hpp"
 
 namespace mms {
@@ -57,7 +59,7 @@ __global__ __launch_bounds__(TB) void direct_partial_kernel(const float *__restr
             const float4 q = src[k];
             const float ddx = tx - q.x, ddy = ty - q.y, ddz = tz - q.z;
             const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            const float inv = (r2 > 0.f) ? rsqrtf(r2) : 0.f;
+            const float inv = (r2 > 0.f) ? __builtin_amdgcn_rsqf(r2) : 0.f;  // r2 >= min(1, ay^2, az^2): never denormal
             s = fmaf(q.w, inv, s);
         }
         acc += (double)s;
@@ -167,7 +169,7 @@ __global__ __launch_bounds__(TB) void direct_batch_kernel(const float *__restric
             const float4 q = src[k];
             const float ddx = tx - q.x, ddy = ty - q.y, ddz = tz - q.z;
             const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            const float inv = (r2 > 0.f) ? rsqrtf(r2) : 0.f;
+            const float inv = (r2 > 0.f) ? __builtin_amdgcn_rsqf(r2) : 0.f;  // r2 >= min(1, ay^2, az^2): never denormal
             s = fmaf(q.w, inv, s);
         }
         acc += (double)s;

@@ -546,7 +546,7 @@ __global__ __launch_bounds__(256) void down_coarse_kernel(const float *__restric
 // px + 2 py: the 6 x 6 block [-2-px, 3-px] x [-2-py, 3-py] of source boxes
 // minus the 3 x 3 neighbourhood (children of the parent's neighbours minus
 // own neighbours, Box.cu:121-145).
-__constant__ unsigned char c_m2l_offs[4][27] = {
+__constant__ int c_m2l_offs[4][27] = {  // 32-bit entries: wave-uniform scalar loads
     {17, 18, 19, 20, 21, 22, 33, 37, 38, 49, 53, 54, 65, 69, 70, 81, 82, 83, 84, 85, 86, 97, 98, 99, 100, 101, 102},
     {16, 17, 18, 19, 20, 21, 32, 33, 37, 48, 49, 53, 64, 65, 69, 80, 81, 82, 83, 84, 85, 96, 97, 98, 99, 100, 101},
     {1, 2, 3, 4, 5, 6, 17, 18, 19, 20, 21, 22, 33, 37, 38, 49, 53, 54, 65, 69, 70, 81, 82, 83, 84, 85, 86},
@@ -621,7 +621,7 @@ __global__ __launch_bounds__(256 * ZP) void m2l_mfma_kernel(const float *__restr
     // in dz: the blocks of +dz and -dz are equal, so only the dz >= 0 half is
     // read (half the operator data through L2)
     const int dzi = (ztc > zs ? ztc - zs : zs - ztc) + zd - 1;
-    const unsigned char *offs = c_m2l_offs[__builtin_amdgcn_readfirstlane(wave)];
+    const int *offs = c_m2l_offs[__builtin_amdgcn_readfirstlane(wave)];
     // operator fragments of the first three offsets, in flight while the window is staged
     constexpr int NP = (MODE == MMS_M2L_BF16X3) ? 2 : 1;
     constexpr int NK = (MODE == MMS_M2L_FP32) ? G / 4 : G / 8;  // 16-byte A loads per row tile and plane

@@ -143,8 +143,11 @@ __global__ __launch_bounds__(256) void up_subtree_kernel(const float *__restrict
                                                          int D, int xd, int yd, long bufA_floats, FilmSet fs) {
     constexpr int G = KP / 4, RT = KP / 16, GP = W * W / 4, SB = KP + 4;
     extern __shared__ __attribute__((aligned(16))) float lds[];
-    const int S = 1 << D, CS = S * W, nR = 1 << R, Lleaf = R + D;
-    const int rbx = blockIdx.x % nR, rby = blockIdx.x / nR;
+    // S, CS, nR and sp below are powers of two: index splits are shifts and masks
+    static_assert(W == 2 || W == 4 || W == 8, "leaf width must be 2, 4 or 8");
+    constexpr int LW = W == 8 ? 3 : (W == 4 ? 2 : 1);
+    const int S = 1 << D, CS = S * W, LCS = D + LW, nR = 1 << R, Lleaf = R + D;
+    const int rbx = blockIdx.x & (nR - 1), rby = blockIdx.x >> R;
     const int f = blockIdx.y / fs.zd, z = blockIdx.y % fs.zd;
     if (fs.desc) {
         const FilmDesc d = fs.desc[f];
@@ -161,7 +164,7 @@ __global__ __launch_bounds__(256) void up_subtree_kernel(const float *__restrict
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             const int e = e0 + threadIdx.x + 256 * k;
-            const int x = cx0 + e % CS, y = cy0 + e / CS;
+            const int x = cx0 + (e & (CS - 1)), y = cy0 + (e >> LCS);
             vb[k] = (e < CS * CS && x < xd && y < yd) ? rho[(long)z * plane + (long)y * xd + x] : 0.0f;
         }
 #pragma unroll
@@ -186,7 +189,7 @@ __global__ __launch_bounds__(256) void up_subtree_kernel(const float *__restrict
     for (int ct = 0; ct < 4; ct++) {
         const int c = wave + 4 * ct;
         if (c < ntile) {
-            const int n = 16 * c + j16, nx = n % S, ny = n / S;
+            const int n = 16 * c + j16, nx = n & (S - 1), ny = n >> D;
             const bool ok = n < nleaf;
 #pragma unroll
             for (int u = 0; u < GP; u++) {
@@ -205,7 +208,7 @@ __global__ __launch_bounds__(256) void up_subtree_kernel(const float *__restrict
         for (int ct = 0; ct < 4; ct++) {
             const int c = wave + 4 * ct, n = 16 * c + j16;
             if (c < ntile && n < nleaf) {
-                const int gx = rbx * S + n % S, gy = rby * S + n / S;
+                const int gx = rbx * S + (n & (S - 1)), gy = rby * S + (n >> D);
                 float *dst = M + lo.off[Lleaf] + (((long)z * nbl + gy) * nbl + gx) * KP;
 #pragma unroll
                 for (int r = 0; r < RT; r++) {
@@ -219,9 +222,9 @@ __global__ __launch_bounds__(256) void up_subtree_kernel(const float *__restrict
     float *ch = q, *pa = lds + bufA_floats;
     int s = S;
     for (int l = Lleaf - 1; l >= R; l--) {
-        const int sp = s >> 1, np = sp * sp, ntp = (np + 15) / 16, nbp = 1 << l;
+        const int sp = s >> 1, lsp = l - R, np = sp * sp, ntp = (np + 15) / 16, nbp = 1 << l;
         for (int c = wave; c < ntp; c += 4) {
-            const int p = 16 * c + j16, px = p % sp, py = p / sp;
+            const int p = 16 * c + j16, px = p & (sp - 1), py = p >> lsp;
             const bool ok = p < np;
             f32x4 acc2[RT];
 #pragma unroll
@@ -278,7 +281,7 @@ __global__ __launch_bounds__(256) void down_subtree_kernel(const float *__restri
     constexpr bool FUSE = W > 0;
     extern __shared__ __attribute__((aligned(16))) float lds[];
     const int nR = 1 << R, Lleaf = R + D;
-    const int rbx = blockIdx.x % nR, rby = blockIdx.x / nR;
+    const int rbx = blockIdx.x & (nR - 1), rby = blockIdx.x >> R;
     const int f = blockIdx.y / zd, z = blockIdx.y % zd;
     const float *Pf = part + (long)f * film_stride;
     float *L = Lall + (long)f * film_stride;
@@ -324,9 +327,9 @@ __global__ __launch_bounds__(256) void down_subtree_kernel(const float *__restri
     const int cxp = wave & 1, cyp = wave >> 1;
     int sp = 1;
     for (int l = R + 1; l <= Lleaf; l++) {
-        const int s = 2 * sp, np = sp * sp, ntp = (np + 15) / 16, nb = 1 << l;
+        const int s = 2 * sp, lsp = l - 1 - R, np = sp * sp, ntp = (np + 15) / 16, nb = 1 << l;
         for (int c = 0; c < ntp; c++) {
-            const int p = 16 * c + j16, px = p % sp, py = p / sp;
+            const int p = 16 * c + j16, px = p & (sp - 1), py = p >> lsp;
             const bool ok = p < np;
             f32x4 acc[RT];
 #pragma unroll
@@ -374,7 +377,7 @@ __global__ __launch_bounds__(256) void down_subtree_kernel(const float *__restri
         const long plane = (long)xd * yd;
         const int nl = sp * sp;
         for (int c = wave; c < (nl + 15) / 16; c += 4) {
-            const int p = 16 * c + j16, px = p % sp, py = p / sp;
+            const int p = 16 * c + j16, px = p & (sp - 1), py = p >> D;
             const bool ok = p < nl;
             f32x4 acc[RTW];
 #pragma unroll

@@ -265,9 +265,11 @@ __global__ __launch_bounds__(256) void up_subtree_kernel(const float *__restrict
 
 // Downward pass of one subtree: the workgroup owns one box of level R of one
 // layer.  Each level's local expansions are the M2L partial sums over source
-// layers plus L2P from the parent (FP32 MFMA, wave w = child position w);
-// the root takes its parent from HBM (level R - 1), the levels between stay
-// in LDS.  W = 0: the leaf locals are written to HBM.  W = 4, 8: the leaf
+// layers plus L2L from the parent (FP32 MFMA, wave w = child position w);
+// the root takes its parent from HBM (level R - 1), or, with `chain`, forms
+// the locals of its ancestors at levels 2 .. R - 1 itself (one column each,
+// the same sums in the same order as down_coarse_kernel); the levels between
+// stay in LDS.  W = 0: the leaf locals are written to HBM.  W = 4, 8: the leaf
 // locals stay in LDS too and the far field is added to the potential
 // (phi[cells of leaf] += scale * L2P[W*W x KP] * L, FP32 MFMA), so phi must
 // already hold the near field.
@@ -276,7 +278,7 @@ __global__ __launch_bounds__(256) void down_subtree_kernel(const float *__restri
                                                            float *__restrict__ Lall, const float *__restrict__ l2lfrag,
                                                            LevelOffsets lo, int R, int D, int zd, long film_stride,
                                                            float *__restrict__ phi, const float *__restrict__ l2pfrag,
-                                                           int xd, int yd, float scale, FilmSet fs) {
+                                                           int xd, int yd, float scale, int chain, FilmSet fs) {
     constexpr int G = KP / 4, RT = KP / 16, SB = KP + 4;
     constexpr bool FUSE = W > 0;
     extern __shared__ __attribute__((aligned(16))) float lds[];
@@ -291,25 +293,61 @@ __global__ __launch_bounds__(256) void down_subtree_kernel(const float *__restri
     float *pa = lds, *ch = lds + nbuf * SB;
     // root box (level R): wave 0, column 0
     if (wave == 0) {
-        f32x4 acc[RT];
+        // acc = L2L[ci] . parent, the parent vector in column 0 of B
+        auto l2l_col = [&](f32x4 *acc, int ci, const float *par) {
 #pragma unroll
-        for (int r = 0; r < RT; r++) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
-        if (R > 2) {
-            const int ci = (rbx & 1) + 2 * (rby & 1), half = nR >> 1;
-            const float *Lp = L + lo.off[R - 1] + (((long)z * half + (rby >> 1)) * half + (rbx >> 1)) * KP;
+            for (int r = 0; r < RT; r++) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int u4 = 0; u4 < G / 4; u4++) {
                 f32x4 av[RT];
 #pragma unroll
                 for (int r = 0; r < RT; r++)
                     av[r] = *reinterpret_cast<const f32x4 *>(l2lfrag + (((long)ci * RT + r) * 64 + lane) * G + 4 * u4);
-                const f32x4 b = (j16 == 0) ? *reinterpret_cast<const f32x4 *>(Lp + g * G + 4 * u4)
+                const f32x4 b = (j16 == 0) ? *reinterpret_cast<const f32x4 *>(par + g * G + 4 * u4)
                                            : f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int u = 0; u < 4; u++)
 #pragma unroll
                     for (int r = 0; r < RT; r++) acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[r][u], b[u], acc[r], 0, 0, 0);
             }
+        };
+        f32x4 acc[RT];
+#pragma unroll
+        for (int r = 0; r < RT; r++) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const int ci = (rbx & 1) + 2 * (rby & 1);
+        if (R > 2 && chain) {
+            // ancestors at levels 2 .. R - 1; each local goes through the
+            // first KP floats of ch (unused until the level loop) to become
+            // the next level's B column, inside this wave
+            float *cv = ch;
+            for (int l = 2; l < R; l++) {
+                const int ax = rbx >> (R - l), ay = rby >> (R - l), nbl = 1 << l;
+                f32x4 a2[RT];
+                if (l > 2) {
+                    __builtin_amdgcn_wave_barrier();
+                    l2l_col(a2, (ax & 1) + 2 * (ay & 1), cv);
+                } else {
+#pragma unroll
+                    for (int r = 0; r < RT; r++) a2[r] = f32x4{0.f, 0.f, 0.f, 0.f};
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (j16 == 0) {
+                    const long e = lo.off[l] + (((long)z * nbl + ay) * nbl + ax) * KP;
+#pragma unroll
+                    for (int r = 0; r < RT; r++) {
+                        f32x4 v = a2[r];
+#pragma unroll 4
+                        for (int zs = 0; zs < zd; zs++)
+                            v += *reinterpret_cast<const f32x4 *>(Pf + zs * part_stride + e + 16 * r + 4 * g);
+                        *reinterpret_cast<f32x4 *>(cv + 16 * r + 4 * g) = v;
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            l2l_col(acc, ci, cv);
+        } else if (R > 2) {
+            const int half = nR >> 1;
+            l2l_col(acc, ci, L + lo.off[R - 1] + (((long)z * half + (rby >> 1)) * half + (rbx >> 1)) * KP);
         }
         if (j16 == 0) {
             const long e = lo.off[R] + (((long)z * nR + rby) * nR + rbx) * KP;
@@ -1512,7 +1550,9 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
     }
     {
         KTimer kt(prof, K_L2L);
-        if (Lc >= 2) {
+        // subtree roots at levels 3..5 form their ancestors' locals themselves
+        const int chain = (R > 2 && R - 1 == Lc) ? 1 : 0;
+        if (Lc >= 2 && !chain) {
             static int attr = 0;
             if (!attr) {
                 MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&down_coarse_kernel<KP>),
@@ -1540,7 +1580,7 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
         if (conc) MMS_HIP(hipStreamWaitEvent(s, pl.ev_join, 0));
         hipLaunchKernelGGL((down_subtree_kernel<KP, WF>), dim3(nR * nR, zd * pl.nfilm), dim3(256), lds, s, pl.part,
                            pl.film_stride * pl.nfilm, pl.loc, pl.d_l2l, lo, R, D, zd, pl.film_stride, phi, pl.d_l2pf,
-                           g.xd, g.yd, scale, pl.films());
+                           g.xd, g.yd, scale, chain, pl.films());
         MMS_HIP(hipGetLastError());
     }
     if constexpr (!split) {

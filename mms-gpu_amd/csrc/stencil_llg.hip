@@ -69,6 +69,43 @@ __global__ __launch_bounds__(BX * BY) void div_kernel(const float *__restrict__ 
     div_column<ZD>(m, rho, tx * BX + threadIdx.x, ty * BY + threadIdx.y, xd, yd, zd, (long)xd * yd * zd, dx, dy, dz);
 }
 
+// Four consecutive x cells per thread (xd a multiple of 4): 16-byte loads and
+// stores, each cell by div_column's expression, so the charges are bitwise
+// the same.  The tile is 4 BX x BY cells.
+template <int ZD>
+__global__ __launch_bounds__(BX * BY) void div4_kernel(const float *__restrict__ m, float *__restrict__ rho, int xd,
+                                                       int yd, int zd_rt, float dx, float dy, float dz) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const int ntx = (xd + 4 * BX - 1) / (4 * BX), nty = (yd + BY - 1) / BY;
+    int tx, ty;
+    if (!xcd_tile(ntx, ntx * nty, tx, ty)) return;
+    const int x = 4 * (tx * BX + threadIdx.x), y = ty * BY + threadIdx.y;
+    if (x >= xd || y >= yd) return;
+    const int zd = ZD ? ZD : zd_rt;
+    const long plane = (long)xd * yd, n = plane * zd;
+    const float *mx = m, *my = m + n, *mz = m + 2 * n;
+    const long c0 = (long)y * xd + x;
+    const bool xl = (x != 0), xr = (x + 4 != xd), yu = (y != yd - 1), yl = (y != 0);
+    const f4 zero = f4{0.f, 0.f, 0.f, 0.f};
+    f4 zprev = zero, zcur = *reinterpret_cast<const f4 *>(mz + c0);
+#pragma unroll
+    for (int z = 0; z < zd; z++) {
+        const long c = (long)z * plane + c0;
+        const f4 znext = (z != zd - 1) ? *reinterpret_cast<const f4 *>(mz + c + plane) : zero;
+        const f4 xc = *reinterpret_cast<const f4 *>(mx + c);
+        const float xm = xl ? mx[c - 1] : 0.0f, xp = xr ? mx[c + 4] : 0.0f;
+        const f4 y2 = yu ? *reinterpret_cast<const f4 *>(my + c + xd) : zero;
+        const f4 y0 = yl ? *reinterpret_cast<const f4 *>(my + c - xd) : zero;
+        const float x2[4] = {xc[1], xc[2], xc[3], xp}, x0[4] = {xm, xc[0], xc[1], xc[2]};
+        f4 r;
+#pragma unroll
+        for (int i = 0; i < 4; i++) r[i] = ((x2[i] - x0[i]) / dx + (y2[i] - y0[i]) / dy + (znext[i] - zprev[i]) / dz) * 0.5f;
+        *reinterpret_cast<f4 *>(rho + c) = r;
+        zprev = zcur;
+        zcur = znext;
+    }
+}
+
 template <int ZD>
 __global__ __launch_bounds__(BX * BY) void div_batch_kernel(const float *__restrict__ m, float *__restrict__ rho,
                                                             const int4 *__restrict__ tiles,
@@ -476,7 +513,19 @@ unsigned xcd_grid(const Grid &g) {
 
 int launch_div(const Grid &g, const float *m, float *rho, hipStream_t s, Profiler *prof) {
     KTimer kt(prof, K_DIV);
-    dim3 grid(xcd_grid(g)), block(BX, BY);
+    dim3 block(BX, BY);
+    if (g.xd % 4 == 0 && ((reinterpret_cast<uintptr_t>(m) | reinterpret_cast<uintptr_t>(rho)) & 15) == 0) {
+        const long ntiles = (long)((g.xd + 4 * BX - 1) / (4 * BX)) * ((g.yd + BY - 1) / BY);
+        dim3 grid((unsigned)(8 * ((ntiles + 7) / 8)));
+        switch (g.zd) {
+        case 3: hipLaunchKernelGGL(div4_kernel<3>, grid, block, 0, s, m, rho, g.xd, g.yd, g.zd, g.dx, g.dy, g.dz); break;
+        case 4: hipLaunchKernelGGL(div4_kernel<4>, grid, block, 0, s, m, rho, g.xd, g.yd, g.zd, g.dx, g.dy, g.dz); break;
+        default: hipLaunchKernelGGL(div4_kernel<0>, grid, block, 0, s, m, rho, g.xd, g.yd, g.zd, g.dx, g.dy, g.dz);
+        }
+        MMS_HIP(hipGetLastError());
+        return 0;
+    }
+    dim3 grid(xcd_grid(g));
     switch (g.zd) {
     case 3: hipLaunchKernelGGL(div_kernel<3>, grid, block, 0, s, m, rho, g.xd, g.yd, g.zd, g.dx, g.dy, g.dz); break;
     case 4: hipLaunchKernelGGL(div_kernel<4>, grid, block, 0, s, m, rho, g.xd, g.yd, g.zd, g.dx, g.dy, g.dz); break;

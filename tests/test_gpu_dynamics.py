@@ -47,6 +47,26 @@ def test_hfield_matches_oracle(mg, orc):
         assert np.abs(H - Ho).max() <= tol * np.abs(Ho).max(), (demag, exch, ext)
 
 
+@pytest.mark.parametrize("dims", [(36, 20, 3), (32, 32, 4), (64, 8, 2), (4, 4, 1)])
+def test_charge_four_wide_matches_oracle(mg, orc, dims):
+    """Widths divisible by 4 take the four-cells-per-thread charge kernel:
+    bitwise equal to divergence_3d (vector_functions.cpp:8-28), boundary
+    cells (x = 0, x = xd - 1, y and z faces) included."""
+    from mms_gpu import workloads as wl
+    _say("charge x4", dims)
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(sum(dims))
+    M = (rng.standard_normal(3 * n) * wl.MS).astype(F)
+    M[rng.random(3 * n) < 0.2] = 0.0
+    d = (F(5e-9), F(4e-9), F(3e-9))
+    H = np.zeros(3 * n, F)
+    ch = np.zeros(n, F)
+    pot = np.zeros(n, F)
+    st = mg.Hfield(M, H, ch, pot, F(0.0), *dims, *d, 3, wl.MU_0, wl.MS, wl.AEXCH, 1, 0, 0, 0, 1, 0)
+    assert st == 0, mg.last_error()
+    assert np.array_equal(ch, orc.divergence_3d(M, dims, d))
+
+
 @pytest.mark.parametrize("combine64", [1, 0])
 def test_rk4_step_matches_oracle(mg, orc, combine64):
     from mms_gpu import workloads as wl

@@ -57,7 +57,8 @@ def m2l_flops_per_eval(side, zd, P, w, m2l):
     KP = 32 * ((K + 31) // 32)
     Lleaf = int(math.log2(side)) - int(math.log2(w))
     blocks = sum((4 ** L) * zd * 27 * zd for L in range(2, Lleaf + 1))
-    return blocks * 2 * K * K, blocks * 2 * KP * KP * (3 if m2l in ("bf16x3", "bf16x3_f16near") else 1)
+    products = {"bf16x3": 3, "bf16x3_f16near": 3, "fp16x2": 2}.get(m2l, 1)
+    return blocks * 2 * K * K, blocks * 2 * KP * KP * products
 
 
 def p2p_pairs_per_eval(side, zd, w):
@@ -75,7 +76,7 @@ def p2p_flops_per_eval(side, zd, w, m2l):
     product in the FP16 form.  Returns (flops, peak TFLOP/s)."""
     cells = side * side * zd
     if w == 8 and m2l != "fp32":
-        return 2 * cells * zd * (3 * w) * (3 * w) * (1 if m2l in ("fp16", "bf16x3_f16near") else 3), BF16_PEAK_TFLOPS
+        return 2 * cells * zd * (3 * w) * (3 * w) * (1 if m2l in ("fp16", "bf16x3_f16near", "fp16x2") else 3), BF16_PEAK_TFLOPS
     return 2 * cells * zd * 9 * w * w, FP32_PEAK_TFLOPS
 
 
@@ -207,18 +208,19 @@ def field_error(mg, f, w, nprobe=8192):
 
 
 # ---------------------------------------------------------------- main --
-PREC_NAMES = ("fp32", "bf16", "bf16x3", "fp16", "bf16x3_f16near")
+PREC_NAMES = ("fp32", "bf16", "bf16x3", "fp16", "bf16x3_f16near", "fp16x2")
 FORM_DTYPE = {"fp32": "f32",
               "bf16": "f32 (BF16 M2L operands, BF16x3 near field)",
               "bf16x3": "f32 (BF16x3 M2L and near field)",
               "fp16": "f32 (FP16 M2L and near field)",
-              "bf16x3_f16near": "f32 (BF16x3 M2L, FP16 near field)"}
+              "bf16x3_f16near": "f32 (BF16x3 M2L, FP16 near field)",
+              "fp16x2": "f32 (FP16x2 M2L, FP16 near field)"}
 RUN_CHECKPOINTS = (250, 500)  # steps of the C3 run at which the field-error gate is checked again
 
 
 def make_film(mg, wl, w, P, m2l, leaf):
     prec = {"fp32": mg.M2L_FP32, "bf16": mg.M2L_BF16, "bf16x3": mg.M2L_BF16X3, "fp16": mg.M2L_FP16,
-            "bf16x3_f16near": mg.M2L_BF16X3_FP16NEAR}[m2l]
+            "bf16x3_f16near": mg.M2L_BF16X3_FP16NEAR, "fp16x2": mg.M2L_FP16X2}[m2l]
     f = mg.Film(*w.dims, *w.d, solver=mg.SOLVER_FMM, P=P, m2l_precision=prec, leaf_width=leaf,
                 mu_0=wl.MU_0, Ms=wl.MS, Aexch=wl.AEXCH, alfa=wl.ALFA, gamma=wl.GAMMA, material=w.material)
     f.set_m(w.M)
@@ -301,7 +303,7 @@ def bench_c5(args, mg, wl):
     mat = np.concatenate([f.material for f in films])
     M0 = np.concatenate([f.M for f in films])
     prec = {"fp32": mg.M2L_FP32, "bf16": mg.M2L_BF16, "bf16x3": mg.M2L_BF16X3, "fp16": mg.M2L_FP16,
-            "bf16x3_f16near": mg.M2L_BF16X3_FP16NEAR}[args.m2l]
+            "bf16x3_f16near": mg.M2L_BF16X3_FP16NEAR, "fp16x2": mg.M2L_FP16X2}[args.m2l]
     b = mg.FilmBatch(films, 3, *films[0].d, solver=mg.SOLVER_FMM, P=4, m2l_precision=prec, material=mat,
                      mu_0=wl.MU_0, Ms=wl.MS, Aexch=wl.AEXCH, alfa=wl.ALFA, gamma=wl.GAMMA)
     info = b.info()
@@ -385,7 +387,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--m2l", default="bf16x3_f16near", choices=PREC_NAMES)
+    ap.add_argument("--m2l", default="fp16x2", choices=PREC_NAMES)
     ap.add_argument("--leaf", type=int, default=8)
     ap.add_argument("--P", type=int, default=6)
     ap.add_argument("--side", type=int, default=1024)

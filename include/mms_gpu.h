@@ -118,8 +118,11 @@ enum mms_m2l_precision {
     MMS_M2L_FP16 = 3,   /* FP16 operands, FP32 accumulation (MFMA): operator
                            rows/columns and each workgroup's source window are
                            scaled by powers of two into FP16 range          */
-    MMS_M2L_BF16X3_FP16NEAR = 4 /* BF16X3 M2L with the near field (leaf
+    MMS_M2L_BF16X3_FP16NEAR = 4, /* BF16X3 M2L with the near field (leaf
                            width 8) in scaled FP16, one MFMA product     */
+    MMS_M2L_FP16X2 = 5  /* hi/lo FP16 split of the scaled operator against
+                           the scaled FP16 source window, 2 MFMA products;
+                           near field (leaf width 8) in scaled FP16       */
 };
 enum mms_external {
     MMS_EXT_NONE = 0,

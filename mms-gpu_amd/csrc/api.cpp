@@ -156,7 +156,7 @@ static int validate_params(const mms_params *p) {
     MMS_CHECK((long)p->xdim * p->ydim * p->zdim < (1L << 31), "grid too large (>= 2^31 cells)");
     MMS_CHECK(p->dx > 0 && p->dy > 0 && p->dz > 0, "cell sizes must be positive");
     MMS_CHECK(p->solver == MMS_SOLVER_DIRECT || p->solver == MMS_SOLVER_FMM, "unknown solver");
-    MMS_CHECK(p->m2l_precision >= MMS_M2L_FP32 && p->m2l_precision <= MMS_M2L_BF16X3_FP16NEAR, "unknown M2L precision");
+    MMS_CHECK(p->m2l_precision >= MMS_M2L_FP32 && p->m2l_precision <= MMS_M2L_FP16X2, "unknown M2L precision");
     return 0;
 }
 

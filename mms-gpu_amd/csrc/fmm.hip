@@ -664,12 +664,13 @@ __global__ __launch_bounds__(256 * ZP) void m2l_mfma_kernel(const float *__restr
     const int dzi = (ztc > zs ? ztc - zs : zs - ztc) + zd - 1;
     const int *offs = c_m2l_offs[__builtin_amdgcn_readfirstlane(wave)];
     // operator fragments of the first three offsets, in flight while the window is staged
-    constexpr int NP = (MODE == MMS_M2L_BF16X3) ? 2 : 1;
+    constexpr bool F16M = (MODE == MMS_M2L_FP16 || MODE == MMS_M2L_FP16X2);  // scaled FP16 source window
+    constexpr int NP = (MODE == MMS_M2L_BF16X3 || MODE == MMS_M2L_FP16X2) ? 2 : 1;  // operator planes (hi, lo)
     constexpr int NK = (MODE == MMS_M2L_FP32) ? G / 4 : G / 8;  // 16-byte A loads per row tile and plane
-    typedef typename std::conditional<MODE == MMS_M2L_FP16, f16x8, bf16x8>::type BV;  // 8 x 16-bit operand
+    typedef typename std::conditional<F16M, f16x8, bf16x8>::type BV;  // 8 x 16-bit operand
     typedef typename std::conditional<MODE == MMS_M2L_FP32, f32x4, BV>::type AV;
     AV abuf[3][NP][RT][NK];
-    const long lo_base = (long)RT * 64 * G * 49 * (2 * zd - 1);  // bf16 lo plane offset (elements)
+    const long lo_base = (long)RT * 64 * G * 49 * (2 * zd - 1);  // operator lo plane offset (elements)
     auto loadA = [&](AV (&a)[NP][RT][NK], int code) {
         const long fbase = ((long)dzi * 49 + (code >> 4) * 7 + (code & 15)) * RT;
 #pragma unroll
@@ -692,7 +693,7 @@ __global__ __launch_bounds__(256 * ZP) void m2l_mfma_kernel(const float *__restr
     // stage the source window of layer zs (zero outside the grid), in
     // batches of 8 loads per thread issued before any of them is used
     float inv_s = 1.0f;  // FP16: 1 / (window scale)
-    if constexpr (MODE == MMS_M2L_FP16) {
+    if constexpr (F16M) {
         // every thread stages the same 4 slots 4q .. 4q + 3 of its boxes
         const f32x4 *M4 = reinterpret_cast<const f32x4 *>(M);
         constexpr int NT = 256 * ZP, NQ = KP / 4, NE = WX * WY * NQ, NB = (NE + NT - 1) / NT;
@@ -848,6 +849,13 @@ __global__ __launch_bounds__(256 * ZP) void m2l_mfma_kernel(const float *__restr
 #pragma unroll
                         for (int c = 0; c < NC; c++)
                             acc[r][c] = mfma32(abuf[k][0][r][sk], bh[c], acc[r][c]);
+                    if constexpr (MODE == MMS_M2L_FP16X2) {
+#pragma unroll
+                        for (int r = 0; r < RT; r++)
+#pragma unroll
+                            for (int c = 0; c < NC; c++)
+                                acc[r][c] = mfma32(abuf[k][1][r][sk], bh[c], acc[r][c]);
+                    }
                     if constexpr (MODE == MMS_M2L_BF16X3) {
                         bf16x8 bl[NC];
 #pragma unroll
@@ -870,7 +878,7 @@ __global__ __launch_bounds__(256 * ZP) void m2l_mfma_kernel(const float *__restr
         }
     }
     // D tile row 4g + q (q = 0..3), column j16: slots 16r + 4g .. +3 of target j16
-    if constexpr (MODE == MMS_M2L_FP16) {
+    if constexpr (F16M) {
 #pragma unroll
         for (int r = 0; r < RT; r++) {
             const f32x4 rs = *reinterpret_cast<const f32x4 *>(f16scale + 16 * r + 4 * g) * inv_s;
@@ -1544,6 +1552,7 @@ int run_fmm(const Grid &g, FmmPlan &pl, const float *rho, float *phi, hipStream_
         case MMS_M2L_BF16: st = launch_m2l_nc<KP, MMS_M2L_BF16>(pl, s); break;
         case MMS_M2L_BF16X3: st = launch_m2l_nc<KP, MMS_M2L_BF16X3>(pl, s); break;
         case MMS_M2L_FP16: st = launch_m2l_nc<KP, MMS_M2L_FP16>(pl, s); break;
+        case MMS_M2L_FP16X2: st = launch_m2l_nc<KP, MMS_M2L_FP16X2>(pl, s); break;
         default: st = launch_m2l_nc<KP, MMS_M2L_FP32>(pl, s);
         }
         if (st) return 1;
@@ -1653,10 +1662,11 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
     MMS_CHECK(P_ >= 1 && P_ <= 9, "FMM expansion order must be 1..9");
     MMS_CHECK(w_ == 2 || w_ == 4 || w_ == 8, "FMM leaf width must be 2, 4 or 8");
     MMS_CHECK(g.zd >= 1 && g.zd <= 16, "FMM supports 1..16 layers");
-    MMS_CHECK(precision_ >= MMS_M2L_FP32 && precision_ <= MMS_M2L_BF16X3_FP16NEAR, "unknown M2L precision");
+    MMS_CHECK(precision_ >= MMS_M2L_FP32 && precision_ <= MMS_M2L_FP16X2, "unknown M2L precision");
     P = P_; w = w_; precision = precision_; zd = g.zd;
     m2l_mode = (precision == MMS_M2L_BF16X3_FP16NEAR) ? MMS_M2L_BF16X3 : precision;
-    near_f16 = (precision == MMS_M2L_FP16 || precision == MMS_M2L_BF16X3_FP16NEAR);
+    near_f16 = (precision == MMS_M2L_FP16 || precision == MMS_M2L_BF16X3_FP16NEAR || precision == MMS_M2L_FP16X2);
+    const bool f16_ops = (m2l_mode == MMS_M2L_FP16 || m2l_mode == MMS_M2L_FP16X2);  // equilibrated FP16 operator
     m2l_cols = (cols_ == 8) ? 8 : 4;
     K = fmm_ncoef(P);
     KP = ((K + 31) / 32) * 32;
@@ -1732,7 +1742,8 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
     // M2L fragments per level: [dz][49][blk]
     const size_t nblk = (size_t)(2 * zd - 1) * 49;
     m2l_level_bytes = (m2l_mode == MMS_M2L_FP32) ? nblk * blk * sizeof(float)
-                                                 : nblk * blk * sizeof(unsigned short) * (m2l_mode == MMS_M2L_BF16X3 ? 2 : 1);
+                                                 : nblk * blk * sizeof(unsigned short) *
+                                                       ((m2l_mode == MMS_M2L_BF16X3 || m2l_mode == MMS_M2L_FP16X2) ? 2 : 1);
     std::vector<char> all(m2l_level_bytes * (Lleaf - 1));
     std::vector<float> lv(nblk * blk);
     // fragment element i = (blk, rt, lane, u) holds Op[16 rt + lane % 16][G (lane / 16) + u]
@@ -1745,7 +1756,7 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
         char *dst = all.data() + (size_t)(L - 2) * m2l_level_bytes;
         if (m2l_mode == MMS_M2L_FP32) {
             memcpy(dst, lv.data(), m2l_level_bytes);
-        } else if (m2l_mode == MMS_M2L_FP16) {
+        } else if (f16_ops) {
             fp16_levels.insert(fp16_levels.end(), lv.begin(), lv.end());
         } else {
             unsigned short *h = reinterpret_cast<unsigned short *>(dst);
@@ -1755,7 +1766,7 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
             }
         }
     }
-    if (m2l_mode == MMS_M2L_FP16) {
+    if (f16_ops) {
         // Row and column factors a_j, b_k (powers of two) that bring the
         // largest |a_j T_jk b_k| of every row and column over all levels,
         // offsets and layer distances near 1 (alternating square-root
@@ -1780,12 +1791,19 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
             sc[j] = (float)(1.0 / a[j]);
             sc[KP + j] = (float)(1.0 / b[j]);
         }
+        // FP16X2: each level's lo plane (v - hi, FP16) follows its hi plane
         unsigned short *h = reinterpret_cast<unsigned short *>(all.data());
+        const size_t per = lv.size(), np = (m2l_mode == MMS_M2L_FP16X2) ? 2 : 1;
         for (size_t i = 0; i < fp16_levels.size(); i++) {
             const float v = (float)(fp16_levels[i] * a[frow(i)] * b[fcol(i)]);
             MMS_CHECK(std::fabs(v) < 65504.0f, "FP16 M2L operator out of range");
             const _Float16 hv = (_Float16)v;
-            memcpy(&h[i], &hv, 2);
+            const size_t o = (i / per) * np * per + i % per;
+            memcpy(&h[o], &hv, 2);
+            if (np == 2) {
+                const _Float16 lv16 = (_Float16)(v - (float)hv);
+                memcpy(&h[o + per], &lv16, 2);
+            }
         }
         if (upload((void **)&d_m2l_scale, sc.data(), sizeof(float) * sc.size())) return 1;
     }

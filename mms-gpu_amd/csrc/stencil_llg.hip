@@ -14,6 +14,7 @@
 // y-neighbour rows of a block are resident in L1/L2).
 // Out-of-grid neighbours read as 0 (the reference's zero padding: smeared
 // surface charge, Dirichlet-zero exchange).
+#include <cstdlib>
 #include "mms_internal.hpp"
 
 namespace mms {
@@ -334,6 +335,217 @@ __global__ __launch_bounds__(BX * BY) void stage_kernel(StageArgs A) {
                      A.hx, A.hy, A.hz);
 }
 
+// The plain resident step at stage ST with four consecutive x cells per
+// thread (xd a multiple of 4, 16-byte aligned arrays): 16-byte loads and
+// stores, and each cell by stage_column_r's expressions in the same order,
+// so the results are bitwise the same.  A group of four with a non-magnetic
+// cell stores its magnetic cells one by one and leaves the others' slope
+// and FP64 base untouched, as the one-cell form does.
+template <int ZD, int ST>
+__device__ __forceinline__ void stage_column4(const StageArgs &A, const float *__restrict__ m_in_,
+                                              float *__restrict__ m_out_, double *__restrict__ mn_,
+                                              float *__restrict__ acc_, const float *__restrict__ phi_,
+                                              const uint8_t *__restrict__ matl_, const float *__restrict__ mb_,
+                                              int x, int y, int xd, int yd, int zd_rt, long n) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    constexpr int stage = ST;
+    const int zd = ZD ? ZD : zd_rt;
+    const long plane = (long)xd * yd;
+    const long c0 = (long)y * xd + x;
+    const float *mx = m_in_, *my = m_in_ + n, *mz = m_in_ + 2 * n;
+    const double t = A.t_dev ? *A.t_dev : 0.0;
+    float hex = A.hx, hey = A.hy, hez = A.hz;
+    if (A.external == 1) { hex = ramp_hx(t, A.inv_mu0); hey = 0.0f; hez = 0.0f; }
+    const float rdx2 = 1.0f / (A.dx * A.dx), rdy2 = 1.0f / (A.dy * A.dy), rdz2 = 1.0f / (A.dz * A.dz);
+    const float gx = A.inv4pi / (2.0f * A.dx), gy = A.inv4pi / (2.0f * A.dy), gz = A.inv4pi / (2.0f * A.dz);
+    const bool xr = (x + 4 != xd), xl = (x != 0), yu = (y != yd - 1), yl = (y != 0);
+    const f4 zero = f4{0.f, 0.f, 0.f, 0.f};
+    auto ld4 = [](const float *p) { return *reinterpret_cast<const f4 *>(p); };
+    // the x neighbours of the four cells: cells x - 1 .. x + 4 of one row
+    auto row6 = [&](const float *p, long c, float (&v)[6]) {
+        const f4 cv = ld4(p + c);
+        v[0] = xl ? p[c - 1] : 0.0f;
+        v[1] = cv[0]; v[2] = cv[1]; v[3] = cv[2]; v[4] = cv[3];
+        v[5] = xr ? p[c + 4] : 0.0f;
+    };
+    f4 pprev = zero, pcur = zero;
+    f4 mxp = zero, myp = zero, mzp = zero;
+    f4 mxc = ld4(mx + c0), myc = ld4(my + c0), mzc = ld4(mz + c0);
+    if (A.demag) pcur = ld4(phi_ + c0);
+#pragma unroll
+    for (int z = 0; z < zd; z++) {
+        const long c = (long)z * plane + c0;
+        const bool ztop = (z != zd - 1);
+        f4 mxn = zero, myn = zero, mzn = zero, pnext = zero;
+        if (ztop) { mxn = ld4(mx + c + plane); myn = ld4(my + c + plane); mzn = ld4(mz + c + plane); }
+        f4 Hx = zero, Hy = zero, Hz = zero;
+        if (A.demag) {
+            if (ztop) pnext = ld4(phi_ + c + plane);
+            float pr[6];
+            row6(phi_, c, pr);
+            const f4 y2 = yu ? ld4(phi_ + c + xd) : zero, y0 = yl ? ld4(phi_ + c - xd) : zero;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                // the film's edge cells take 0 for the missing neighbour
+                const float x2 = (i < 3 || xr) ? pr[i + 2] : 0.0f, x0 = (i > 0 || xl) ? pr[i] : 0.0f;
+                Hx[i] = (x2 - x0) * gx;
+                Hy[i] = (y2[i] - y0[i]) * gy;
+                Hz[i] = (pnext[i] - pprev[i]) * gz;
+            }
+        }
+        if (A.exchange) {
+            float rx[6], ry[6], rz[6];
+            row6(mx, c, rx); row6(my, c, ry); row6(mz, c, rz);
+            const f4 xya = yu ? ld4(mx + c + xd) : zero, xyb = yl ? ld4(mx + c - xd) : zero;
+            const f4 yya = yu ? ld4(my + c + xd) : zero, yyb = yl ? ld4(my + c - xd) : zero;
+            const f4 zya = yu ? ld4(mz + c + xd) : zero, zyb = yl ? ld4(mz + c - xd) : zero;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                float Lx, Ly, Lz;
+                {
+                    const float xa = (i < 3 || xr) ? rx[i + 2] : 0.0f, xb = (i > 0 || xl) ? rx[i] : 0.0f;
+                    Lx = (xa + xb - 2.0f * mxc[i]) * rdx2 + (xya[i] + xyb[i] - 2.0f * mxc[i]) * rdy2 +
+                         (mxn[i] + mxp[i] - 2.0f * mxc[i]) * rdz2;
+                }
+                {
+                    const float xa = (i < 3 || xr) ? ry[i + 2] : 0.0f, xb = (i > 0 || xl) ? ry[i] : 0.0f;
+                    Ly = (xa + xb - 2.0f * myc[i]) * rdx2 + (yya[i] + yyb[i] - 2.0f * myc[i]) * rdy2 +
+                         (myn[i] + myp[i] - 2.0f * myc[i]) * rdz2;
+                }
+                {
+                    const float xa = (i < 3 || xr) ? rz[i + 2] : 0.0f, xb = (i > 0 || xl) ? rz[i] : 0.0f;
+                    Lz = (xa + xb - 2.0f * mzc[i]) * rdx2 + (zya[i] + zyb[i] - 2.0f * mzc[i]) * rdy2 +
+                         (mzn[i] + mzp[i] - 2.0f * mzc[i]) * rdz2;
+                }
+                Hx[i] += A.exch_cm * Lx; Hy[i] += A.exch_cm * Ly; Hz[i] += A.exch_cm * Lz;
+            }
+        }
+        if (A.external) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) { Hx[i] += hex; Hy[i] += hey; Hz[i] += hez; }
+        }
+        bool mag[4] = {true, true, true, true};
+        if (matl_) {
+            const uchar4 mt = *reinterpret_cast<const uchar4 *>(matl_ + c);
+            mag[0] = mt.x != 0; mag[1] = mt.y != 0; mag[2] = mt.z != 0; mag[3] = mt.w != 0;
+        }
+        const bool all = mag[0] && mag[1] && mag[2] && mag[3];
+        float *ox = m_out_, *oy = m_out_ + n, *oz = m_out_ + 2 * n;
+        double *bx = mn_, *by = mn_ + n, *bz = mn_ + 2 * n;
+        float *ax = acc_, *ay = acc_ + n, *az = acc_ + 2 * n;
+        f4 kx, ky, kz;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            float k0, k1, k2;
+            llg_rhs(mxc[i], myc[i], mzc[i], Hx[i], Hy[i], Hz[i], A.alfa, A.gamma, A.Ms, k0, k1, k2);
+            kx[i] = k0; ky[i] = k1; kz[i] = k2;
+        }
+        // M_n: FP64 base at stage 3, FP32 copy (or this stage's input at stage 0) before
+        double b0[4], b1[4], b2[4];
+        if (stage == 3 || !mb_) {
+            const d2 u0 = __builtin_nontemporal_load(reinterpret_cast<const d2 *>(bx + c));
+            const d2 u1 = __builtin_nontemporal_load(reinterpret_cast<const d2 *>(bx + c + 2));
+            const d2 v0 = __builtin_nontemporal_load(reinterpret_cast<const d2 *>(by + c));
+            const d2 v1 = __builtin_nontemporal_load(reinterpret_cast<const d2 *>(by + c + 2));
+            const d2 w0 = __builtin_nontemporal_load(reinterpret_cast<const d2 *>(bz + c));
+            const d2 w1 = __builtin_nontemporal_load(reinterpret_cast<const d2 *>(bz + c + 2));
+            b0[0] = u0[0]; b0[1] = u0[1]; b0[2] = u1[0]; b0[3] = u1[1];
+            b1[0] = v0[0]; b1[1] = v0[1]; b1[2] = v1[0]; b1[3] = v1[1];
+            b2[0] = w0[0]; b2[1] = w0[1]; b2[2] = w1[0]; b2[3] = w1[1];
+        } else if (stage == 0) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) { b0[i] = mxc[i]; b1[i] = myc[i]; b2[i] = mzc[i]; }
+        } else {
+            const f4 u = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(mb_ + c));
+            const f4 v = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(mb_ + n + c));
+            const f4 w = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(mb_ + 2 * n + c));
+#pragma unroll
+            for (int i = 0; i < 4; i++) { b0[i] = u[i]; b1[i] = v[i]; b2[i] = w[i]; }
+        }
+        f4 s0, s1, s2;
+        if (stage == 0) { s0 = kx; s1 = ky; s2 = kz; }
+        else {
+            const float wk = (stage == 3) ? 1.0f : 2.0f;  // w k is exact
+            const f4 a0 = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(ax + c));
+            const f4 a1 = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(ay + c));
+            const f4 a2 = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(az + c));
+#pragma unroll
+            for (int i = 0; i < 4; i++) { s0[i] = a0[i] + wk * kx[i]; s1[i] = a1[i] + wk * ky[i]; s2[i] = a2[i] + wk * kz[i]; }
+        }
+        double n0[4], n1[4], n2[4];
+        if (stage < 3) {
+            const double h = (stage < 2 ? 0.5 : 1.0) * A.dt;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                n0[i] = b0[i] + h * (double)kx[i]; n1[i] = b1[i] + h * (double)ky[i]; n2[i] = b2[i] + h * (double)kz[i];
+            }
+        } else {
+            const double h = A.dt / 6.0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                n0[i] = b0[i] + h * (double)s0[i]; n1[i] = b1[i] + h * (double)s1[i]; n2[i] = b2[i] + h * (double)s2[i];
+                if (A.normalize) {
+                    const double f = (double)A.Ms / sqrt(n0[i] * n0[i] + n1[i] * n1[i] + n2[i] * n2[i]);
+                    n0[i] *= f; n1[i] *= f; n2[i] *= f;
+                }
+            }
+        }
+        f4 o0, o1, o2;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            o0[i] = mag[i] ? (float)n0[i] : mxc[i];
+            o1[i] = mag[i] ? (float)n1[i] : myc[i];
+            o2[i] = mag[i] ? (float)n2[i] : mzc[i];
+        }
+        *reinterpret_cast<f4 *>(ox + c) = o0;
+        *reinterpret_cast<f4 *>(oy + c) = o1;
+        *reinterpret_cast<f4 *>(oz + c) = o2;
+        if (all) {
+            if (stage < 3) {
+                __builtin_nontemporal_store(s0, reinterpret_cast<f4 *>(ax + c));
+                __builtin_nontemporal_store(s1, reinterpret_cast<f4 *>(ay + c));
+                __builtin_nontemporal_store(s2, reinterpret_cast<f4 *>(az + c));
+            } else {
+                __builtin_nontemporal_store(d2{n0[0], n0[1]}, reinterpret_cast<d2 *>(bx + c));
+                __builtin_nontemporal_store(d2{n0[2], n0[3]}, reinterpret_cast<d2 *>(bx + c + 2));
+                __builtin_nontemporal_store(d2{n1[0], n1[1]}, reinterpret_cast<d2 *>(by + c));
+                __builtin_nontemporal_store(d2{n1[2], n1[3]}, reinterpret_cast<d2 *>(by + c + 2));
+                __builtin_nontemporal_store(d2{n2[0], n2[1]}, reinterpret_cast<d2 *>(bz + c));
+                __builtin_nontemporal_store(d2{n2[2], n2[3]}, reinterpret_cast<d2 *>(bz + c + 2));
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                if (!mag[i]) continue;
+                if (stage < 3) {
+                    __builtin_nontemporal_store(s0[i], &ax[c + i]);
+                    __builtin_nontemporal_store(s1[i], &ay[c + i]);
+                    __builtin_nontemporal_store(s2[i], &az[c + i]);
+                } else {
+                    __builtin_nontemporal_store(n0[i], &bx[c + i]);
+                    __builtin_nontemporal_store(n1[i], &by[c + i]);
+                    __builtin_nontemporal_store(n2[i], &bz[c + i]);
+                }
+            }
+        }
+        pprev = pcur; pcur = pnext;
+        mxp = mxc; myp = myc; mzp = mzc;
+        mxc = mxn; myc = myn; mzc = mzn;
+    }
+}
+
+template <int ZD, int ST>
+__global__ __launch_bounds__(BX * BY) void stage4_kernel(StageArgs A) {
+    const int ntx = (A.xd + 4 * BX - 1) / (4 * BX), nty = (A.yd + BY - 1) / BY;
+    int tx, ty;
+    if (!xcd_tile(ntx, ntx * nty, tx, ty)) return;
+    const int x = 4 * (tx * BX + threadIdx.x), y = ty * BY + threadIdx.y;
+    if (x >= A.xd || y >= A.yd) return;
+    stage_column4<ZD, ST>(A, A.m_in, A.m_out, A.mn, A.acc, A.phi, A.matl, A.m_base, x, y, A.xd, A.yd, A.zd,
+                          (long)A.xd * A.yd * A.zd);
+}
+
 // Batched films: block b handles tile tiles[b] = (film, tile x, tile y) of a
 // film whose extent, cell offset and external field come from films[].
 template <int ZD, int ST>
@@ -493,6 +705,16 @@ void launch_stage_zd(dim3 grid, dim3 block, hipStream_t s, const StageArgs &A, b
 }
 
 template <int ZD>
+void launch_stage4_zd(dim3 grid, dim3 block, hipStream_t s, const StageArgs &A) {
+    switch (A.stage) {
+    case 0: hipLaunchKernelGGL((stage4_kernel<ZD, 0>), grid, block, 0, s, A); break;
+    case 1: hipLaunchKernelGGL((stage4_kernel<ZD, 1>), grid, block, 0, s, A); break;
+    case 2: hipLaunchKernelGGL((stage4_kernel<ZD, 2>), grid, block, 0, s, A); break;
+    default: hipLaunchKernelGGL((stage4_kernel<ZD, 3>), grid, block, 0, s, A);
+    }
+}
+
+template <int ZD>
 void launch_stage_batch_zd(dim3 grid, dim3 block, hipStream_t s, const StageArgs &A, const int4 *tiles,
                            const BatchFilm *films, long pstride) {
     switch (A.stage) {
@@ -548,6 +770,24 @@ int launch_stage_ex(const Grid &g, const Material &mat, int stage, double dt, co
     MMS_CHECK(stage >= 0 && stage <= 3, "RK4 stage must be 0..3");
     dim3 grid(xcd_grid(g)), block(BX, BY);
     const bool plain = !ex.h_given && !ex.h_out && !ex.k_out && !ex.slope_out && !ex.mn_out;
+    const uintptr_t al = reinterpret_cast<uintptr_t>(m_in) | reinterpret_cast<uintptr_t>(m_out) |
+                         reinterpret_cast<uintptr_t>(mn) | reinterpret_cast<uintptr_t>(acc) |
+                         reinterpret_cast<uintptr_t>(phi) | reinterpret_cast<uintptr_t>(ex.m_base);
+    // MMS_GPU_STAGE_ONE_CELL=1 keeps the one-cell form (the four-cell form's
+    // results are bitwise the same; the switch lets tests show it)
+    const char *one = std::getenv("MMS_GPU_STAGE_ONE_CELL");
+    if (plain && !(one && one[0] == '1') && g.xd % 4 == 0 && (al & 15) == 0 &&
+        (reinterpret_cast<uintptr_t>(matl) & 3) == 0) {
+        const long ntiles = (long)((g.xd + 4 * BX - 1) / (4 * BX)) * ((g.yd + BY - 1) / BY);
+        dim3 grid4((unsigned)(8 * ((ntiles + 7) / 8)));
+        switch (g.zd) {
+        case 3: launch_stage4_zd<3>(grid4, block, s, A); break;
+        case 4: launch_stage4_zd<4>(grid4, block, s, A); break;
+        default: launch_stage4_zd<0>(grid4, block, s, A);
+        }
+        MMS_HIP(hipGetLastError());
+        return 0;
+    }
     switch (g.zd) {
     case 3: launch_stage_zd<3>(grid, block, s, A, plain); break;
     case 4: launch_stage_zd<4>(grid, block, s, A, plain); break;

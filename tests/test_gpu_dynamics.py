@@ -67,6 +67,41 @@ def test_charge_four_wide_matches_oracle(mg, orc, dims):
     assert np.array_equal(ch, orc.divergence_3d(M, dims, d))
 
 
+@pytest.mark.parametrize("case", ["c2_dots", "ragged_direct"])
+def test_stage_four_wide_matches_one_cell(mg, case, monkeypatch):
+    """Widths divisible by 4 run the resident step's stage kernel four cells
+    per thread; MMS_GPU_STAGE_ONE_CELL=1 forces the one-cell form.  Both give
+    bitwise identical states, energies and torques: on the C2 dots film
+    (non-magnetic cells between the dots, constant external field, FMM) and
+    on a 36 x 20 x 3 film with a ragged mask and the ramp field (direct sum)."""
+    from mms_gpu import workloads as wl
+    if case == "c2_dots":
+        w = wl.c2_dots()
+        kw = dict(solver=mg.SOLVER_FMM, P=4, m2l_precision=mg.M2L_FP32, leaf_width=4,
+                  external=mg.EXT_CONST, hext=(7957.7, 0.0, 0.0))
+        dims, d, mat, M, dt = w.dims, w.d, w.material, w.M, 1e-13
+    else:
+        dims = (36, 20, 3)
+        rng = np.random.default_rng(9)
+        mat = (rng.random(int(np.prod(dims))) < 0.8).astype(np.uint8)
+        M = wl.random_unit(2010, 3, int(np.prod(dims))).astype(F).ravel() * F(wl.MS)
+        M = (M.reshape(-1, 3) * mat[:, None]).astype(F).ravel()
+        d = (F(5e-9), F(5e-9), F(5e-9))
+        kw = dict(solver=mg.SOLVER_DIRECT, external=mg.EXT_RAMP)
+        dt = 1e-13
+    out = []
+    for one in ("0", "1"):
+        _say("stage four-wide vs one-cell", case, one)
+        monkeypatch.setenv("MMS_GPU_STAGE_ONE_CELL", one)
+        with mg.Film(*dims, *d, Ms=wl.MS, Aexch=wl.AEXCH, mu_0=wl.MU_0, material=mat, **kw) as f:
+            f.set_m(M)
+            f.steps(3, dt, use_graph=False)
+            f.sync()
+            out.append((f.get_m(), f.energy_torque()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
+
+
 @pytest.mark.parametrize("combine64", [1, 0])
 def test_rk4_step_matches_oracle(mg, orc, combine64):
     from mms_gpu import workloads as wl

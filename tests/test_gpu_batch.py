@@ -24,15 +24,18 @@ def _films():
     return out
 
 
+@pytest.mark.parametrize("form", ["fp32_leaf4", "fp16x2_leaf8"])
 @pytest.mark.parametrize("use_graph", [True, False])
-def test_batch_matches_single_films(mg, use_graph):
+def test_batch_matches_single_films(mg, use_graph, form):
     from mms_gpu import workloads as wl
+    prec, leaf = {"fp32_leaf4": (mg.M2L_FP32, 4), "fp16x2_leaf8": (mg.M2L_FP16X2, 8)}[form]
     films = _films()
     mat = np.concatenate([f.material for f in films])
     M0 = np.concatenate([f.M for f in films])
     kw = dict(mu_0=wl.MU_0, Ms=wl.MS, Aexch=wl.AEXCH, alfa=wl.ALFA, gamma=wl.GAMMA)
-    _say("batch of", len(films), "films, graph", use_graph)
-    with mg.FilmBatch(films, 3, *films[0].d, solver=mg.SOLVER_FMM, P=4, material=mat, **kw) as b:
+    _say("batch of", len(films), "films, graph", use_graph, form)
+    with mg.FilmBatch(films, 3, *films[0].d, solver=mg.SOLVER_FMM, P=4, m2l_precision=prec, leaf_width=leaf,
+                      material=mat, **kw) as b:
         info = b.info()
         _say(info)
         assert info["direct_films"] == 2 and info["fmm_films"] == 4 and info["fmm_size_classes"] == 2
@@ -43,8 +46,8 @@ def test_batch_matches_single_films(mg, use_graph):
     worst = 0.0
     for i, f in enumerate(films):
         solver = mg.SOLVER_DIRECT if f.solver == "direct" else mg.SOLVER_FMM
-        with mg.Film(*f.dims, *f.d, solver=solver, P=4, external=mg.EXT_CONST, hext=f.hext,
-                     material=f.material, **kw) as s:
+        with mg.Film(*f.dims, *f.d, solver=solver, P=4, m2l_precision=prec, leaf_width=leaf,
+                     external=mg.EXT_CONST, hext=f.hext, material=f.material, **kw) as s:
             s.set_m(f.M)
             s.steps(10, F(1e-13), use_graph=False)
             Ms = s.get_m()

@@ -1409,7 +1409,8 @@ int launch_l2l_sum(const FmmPlan &pl, int L, hipStream_t s) {
 // two near-field workgroups filling it.  FP16 near field: at least 40 KB
 // (its window needs 34 KB), beside one 77 KB or three 39 KB M2L workgroups
 // (measured on the 1024 x 1024 x 4 film against 82 KB: 1210 vs 1162 steps/s
-// with the BF16 split M2L, 1313 vs 1265 with the FP16 M2L).
+// with the BF16 split M2L, 1313 vs 1265 with the FP16 M2L; with the FP16x2
+// M2L 34, 40, 56 and 80 KB gave 1451, 1452, 1380 and 1281 steps/s).
 size_t near_lds_shared(const FmmPlan &pl) { return (pl.near_f16 ? 40 : 82) * 1024; }
 
 template <int NCB, bool F16>

@@ -30,6 +30,20 @@ def test_library_exports_every_declared_symbol(mg):
     assert "gfx950" in mg.version()
 
 
+def test_enum_values_match_python_constants(mg):
+    """Every solver, M2L precision and external-field value declared in the
+    header has the same value in the Python package."""
+    src = open(os.path.join(ROOT, "include", "mms_gpu.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    header = {k: int(v) for k, v in re.findall(r"\bMMS_(\w+)\s*=\s*(\d+)", src)}
+    checked = 0
+    for k, v in header.items():
+        if k.startswith(("SOLVER_", "M2L_", "EXT_")):
+            assert getattr(mg, k) == v, k
+            checked += 1
+    assert checked >= 10 and header["M2L_FP16X2"] == 5
+
+
 def test_reference_entry_points_present():
     declared = _declared_functions()
     for name in ("calc_potential_exact", "calc_potential_exact_gpu", "fmm_calc", "Hfield", "rk4_step",

@@ -120,9 +120,12 @@ enum mms_m2l_precision {
                            scaled by powers of two into FP16 range          */
     MMS_M2L_BF16X3_FP16NEAR = 4, /* BF16X3 M2L with the near field (leaf
                            width 8) in scaled FP16, one MFMA product     */
-    MMS_M2L_FP16X2 = 5  /* hi/lo FP16 split of the scaled operator against
+    MMS_M2L_FP16X2 = 5, /* hi/lo FP16 split of the scaled operator against
                            the scaled FP16 source window, 2 MFMA products;
                            near field (leaf width 8) in scaled FP16       */
+    MMS_M2L_FP16X2_BF16X3NEAR = 6 /* FP16X2 M2L with the near field (leaf
+                           width 8) in the BF16 hi/lo split of operator and
+                           charges, 3 MFMA products                      */
 };
 enum mms_external {
     MMS_EXT_NONE = 0,
@@ -228,6 +231,10 @@ int mms_batch_get_m(mms_batch *b, float *M);
  * stage over all films; use_graph != 0 replays a captured step. */
 int mms_batch_steps(mms_batch *b, int nsteps, float dt, int use_graph);
 int mms_batch_sync(mms_batch *b);
+/* Demag potential of every film from packed host charge densities (A/m^2,
+ * film order, as mms_batch_set_m packs M) with the batch's own solvers (the
+ * direct sum or the film's FMM size class); synchronous. */
+int mms_batch_potential(mms_batch *b, const float *charge, float *potential);
 int mms_batch_info(mms_batch *b, int *n_direct, int *n_fmm, int *n_classes, long *cells, int *launches_per_step);
 /* Autotuner: per FMM size class, leaf width x M2L column tiles on the current state. */
 int mms_batch_autotune(mms_batch *b, int reps, char *report, int report_len);

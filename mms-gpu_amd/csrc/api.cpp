@@ -80,7 +80,6 @@ static int ctx_alloc(mms_ctx *c, const unsigned char *material) {
     MMS_HIP(hipMalloc(&c->acc32, sizeof(float) * 3 * n));
     MMS_HIP(hipMalloc(&c->rho, sizeof(float) * n));
     MMS_HIP(hipMalloc(&c->phi, sizeof(float) * n));
-    MMS_HIP(hipMalloc(&c->phi_n, sizeof(float) * n));
     MMS_HIP(hipMalloc(&c->matl, n));
     MMS_HIP(hipMalloc(&c->t_dev, sizeof(double)));
     MMS_HIP(hipMalloc(&c->h_aos, sizeof(float) * 3 * n));
@@ -114,7 +113,7 @@ static void ctx_free(mms_ctx *c) {
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
     void *bufs[] = {c->m32[0], c->m32[1], c->m32[2], c->mn64, c->mnew64, c->acc32, c->rho, c->phi,
-                    c->phi_n, c->matl, c->t_dev, c->h_aos, c->aux_aos, c->red};
+                    c->matl, c->t_dev, c->h_aos, c->aux_aos, c->red};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     c->direct.release();
@@ -156,7 +155,7 @@ static int validate_params(const mms_params *p) {
     MMS_CHECK((long)p->xdim * p->ydim * p->zdim < (1L << 31), "grid too large (>= 2^31 cells)");
     MMS_CHECK(p->dx > 0 && p->dy > 0 && p->dz > 0, "cell sizes must be positive");
     MMS_CHECK(p->solver == MMS_SOLVER_DIRECT || p->solver == MMS_SOLVER_FMM, "unknown solver");
-    MMS_CHECK(p->m2l_precision >= MMS_M2L_FP32 && p->m2l_precision <= MMS_M2L_FP16X2, "unknown M2L precision");
+    MMS_CHECK(p->m2l_precision >= MMS_M2L_FP32 && p->m2l_precision <= MMS_M2L_FP16X2_BF16X3NEAR, "unknown M2L precision");
     return 0;
 }
 
@@ -282,10 +281,10 @@ extern "C" int mms_hfield(mms_ctx *c, float *H) {
     return 0;
 }
 
-// energy (eV) and torque from h_aos at state m (reference units)
-static int energy_torque_of(mms_ctx *c, const float *m, double *E, double *T) {
+// energy (eV) and torque from the AoS field h at state m (reference units)
+static int energy_torque_of(mms_ctx *c, const float *m, const float *h, double *E, double *T) {
     const long n = c->g.n();
-    if (launch_energy_h(m, c->h_aos, c->matl, n, c->red, c->red_blocks, c->stream, &c->prof)) return 1;
+    if (launch_energy_h(m, h, c->matl, n, c->red, c->red_blocks, c->stream, &c->prof)) return 1;
     std::vector<double> part(2 * c->red_blocks);
     MMS_HIP(hipMemcpyAsync(part.data(), c->red, sizeof(double) * 2 * c->red_blocks, hipMemcpyDeviceToHost,
                            c->stream));
@@ -304,7 +303,7 @@ static int energy_torque_of(mms_ctx *c, const float *m, double *E, double *T) {
 extern "C" int mms_energy_torque(mms_ctx *c, double *E, double *T) {
     MMS_CHECK(c && E && T, "null argument");
     if (field_now(c, c->m32[c->cur])) return 1;
-    return energy_torque_of(c, c->m32[c->cur], E, T);
+    return energy_torque_of(c, c->m32[c->cur], c->h_aos, E, T);
 }
 
 extern "C" int mms_potential_dev(mms_ctx *c, const float *rho_dev, float *phi_dev) {
@@ -612,6 +611,7 @@ extern "C" int rk4_step(float t, float dt, float *t2, const unsigned char *mater
                         int use_gpu, int verbosity) {
     (void)use_gpu; (void)verbosity;
     if (use_fmm && (P < 1 || P > 9)) { set_error("rk4_step: P must be in 1..9"); return EXIT_FAILURE; }
+    if (!material || !M || !M2 || !H) { set_error("rk4_step: null material, M, M2 or H"); return EXIT_FAILURE; }
     std::lock_guard<std::mutex> lk(g_cache_mu);
     mms_params p = field_params(xdim, ydim, zdim, dx, dy, dz, P, mu_0, Ms, Aexch, demag, exchange, external, use_fmm);
     p.alfa = alfa; p.gamma = gamma;
@@ -619,16 +619,23 @@ extern "C" int rk4_step(float t, float dt, float *t2, const unsigned char *mater
     if (!c) return EXIT_FAILURE;
     const long n = c->g.n();
     hipStream_t s = c->stream;
-    // device staging: aux_aos = [H in | k out | slope out]
+    // device staging: aux_aos = [H in | k out | slope out]; the caller's M2
+    // (FP32 SoA) goes to the unused FP64 proposal buffer: non-material cells
+    // keep it, and this_slope's, as the reference leaves them untouched there
     float *h_in = c->aux_aos, *k_out = c->aux_aos + 3 * n, *s_out = c->aux_aos + 6 * n;
+    float *m2_keep = reinterpret_cast<float *>(c->mnew64);
     if (mms_set_time_nolock(c, (double)t)) return EXIT_FAILURE;
     if (hipMemcpyAsync(c->h_aos, M, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s) != hipSuccess ||
         launch_soa_from_aos(c->h_aos, c->m32[0], c->mn64, n, s) ||
-        hipMemcpyAsync(h_in, H, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s) != hipSuccess) {
+        hipMemcpyAsync(c->h_aos, M2, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        launch_soa_from_aos(c->h_aos, m2_keep, nullptr, n, s) ||
+        hipMemcpyAsync(h_in, H, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        (this_slope &&
+         hipMemcpyAsync(k_out, this_slope, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s) != hipSuccess)) {
         set_error("rk4_step: upload failed");
         return EXIT_FAILURE;
     }
-    StageExtra ex{nullptr, nullptr, nullptr, nullptr, normalize, nullptr, c->m32[0]};
+    StageExtra ex{nullptr, nullptr, nullptr, nullptr, normalize, nullptr, c->m32[0], m2_keep};
     const float *in[4] = {c->m32[0], c->m32[1], c->m32[2], c->m32[1]};
     float *outp[4] = {c->m32[1], c->m32[2], c->m32[1], c->m32[2]};
     for (int st = 0; st < 4; st++) {
@@ -657,8 +664,6 @@ extern "C" int rk4_step(float t, float dt, float *t2, const unsigned char *mater
         ok = hipMemcpyAsync(potential, c->phi, sizeof(float) * n, hipMemcpyDeviceToHost, s) == hipSuccess;
     if (ok) ok = hipStreamSynchronize(s) == hipSuccess;
     if (!ok) { set_error("rk4_step: download failed"); return EXIT_FAILURE; }
-    // the reference leaves M2 untouched on non-material cells
-    (void)material;
     if (t2) *t2 = t + dt;
     return EXIT_SUCCESS;
 }
@@ -716,50 +721,57 @@ extern "C" int time_marching(unsigned char *material, float *M, float finaltime,
     }
     int status = 0;
     if (upload_m(c, M)) status = 1;
-    // buffers: m32[cur] = M_n (f32), mn64 = M_n (f64); phi_n = potential at M_n
+    // buffers: m32[cur] = M_n (f32), mn64 = M_n (f64); h_n = the field H of
+    // the reference loop (at M_n, evaluated at the time of the step that
+    // produced M_n), h_2 = the post-step field H2 at the proposal.  Both are
+    // AoS; aux_aos[0, 3n) stays free as the download staging of mms_get_m.
+    float *h_n = c->aux_aos + 3 * n, *h_2 = c->h_aos;
     double dt = timestep, t = 0.0;
     int tindex = 0, consecutive_error_count = 0;
+    bool accepted = false;
     std::vector<float> hostM(3 * n);
-    auto field_into = [&](const float *m, float *phi_out) -> int {
+    auto field_into = [&](const float *m, float *h_out) -> int {
         if (demag) {
-            if (launch_div(c->g, m, c->rho, s, &c->prof) || c->potential(c->rho, phi_out)) return 1;
+            if (launch_div(c->g, m, c->rho, s, &c->prof) || c->potential(c->rho, c->phi)) return 1;
         }
-        return launch_hfield(c->g, c->mat, phi_out, m, c->t_dev, c->h_aos, s, &c->prof);
+        return launch_hfield(c->g, c->mat, c->phi, m, c->t_dev, h_out, s, &c->prof);
     };
     float E = 0;
     double Ed = 0, torque0 = 0;
     if (!status && mms_set_time_nolock(c, 0.0)) status = 1;
-    if (!status && field_into(c->m32[c->cur], c->phi_n)) status = 1;
-    bool h_valid = true;  // h_aos holds the field at M_n
+    if (!status && field_into(c->m32[c->cur], h_n)) status = 1;
+    // the reference's proposal M2 starts at zero and is never written on
+    // non-material cells (ode_functions.cpp:165,102), so every stage and
+    // accepted state holds 0 there; the initial field above used M as given
+    if (!status && !c->all_magnetic && launch_zero_nonmagnetic(c->m32[c->cur], c->matl, n, s)) status = 1;
     while (!status && t <= finaltime) {
         if (mms_set_time_nolock(c, t)) { status = 1; break; }
-        if (!h_valid && launch_hfield(c->g, c->mat, c->phi_n, c->m32[c->cur], c->t_dev, c->h_aos, s, &c->prof)) {
-            status = 1;
-            break;
-        }
-        if (energy_torque_of(c, c->m32[c->cur], &Ed, &torque0)) { status = 1; break; }
+        if (energy_torque_of(c, c->m32[c->cur], h_n, &Ed, &torque0)) { status = 1; break; }
         E = (float)Ed;
-        // RK4 step: stages read/write m32 slots other than cur; stage 3 proposes mnew64
+        // RK4 step: stages read/write m32 slots other than cur; stage 3
+        // proposes mnew64; k1 takes the field H as given (H3 = H,
+        // ode_functions.cpp:246-247), not recomputed at the current time
         const int a = c->cur, b = (c->cur + 1) % 3, d = (c->cur + 2) % 3;
         const float *in[4] = {c->m32[a], c->m32[b], c->m32[d], c->m32[b]};
         float *outp[4] = {c->m32[b], c->m32[d], c->m32[b], c->m32[d]};
-        StageExtra ex{nullptr, nullptr, nullptr, nullptr, 1, c->mnew64, c->m32[a]};
+        StageExtra ex{nullptr, nullptr, nullptr, nullptr, 1, c->mnew64, c->m32[a], nullptr};
         for (int st = 0; st < 4 && !status; st++) {
-            const float *phi_use = (st == 0) ? c->phi_n : c->phi;
+            ex.h_given = (st == 0) ? h_n : nullptr;
             if (st > 0 && demag) {
                 if (launch_div(c->g, in[st], c->rho, s, &c->prof) || c->potential(c->rho, c->phi)) { status = 1; break; }
             }
-            if (launch_stage_ex(c->g, c->mat, st, dt, phi_use, in[st], outp[st], c->mn64, c->acc32, c->matl, c->t_dev,
+            if (launch_stage_ex(c->g, c->mat, st, dt, c->phi, in[st], outp[st], c->mn64, c->acc32, c->matl, c->t_dev,
                                 ex, s, &c->prof))
                 status = 1;
         }
         if (status) break;
-        // post-step field at the proposal (m32[d]); potential kept in phi
+        // post-step field H2 at the proposal (m32[d]), at the unadvanced t
+        // as in the reference (ode_functions.cpp:267)
         double E2d = 0, torqued = 0;
-        if (field_into(c->m32[d], c->phi) || energy_torque_of(c, c->m32[d], &E2d, &torqued)) { status = 1; break; }
+        if (field_into(c->m32[d], h_2) || energy_torque_of(c, c->m32[d], h_2, &E2d, &torqued)) { status = 1; break; }
         const float E2 = (float)E2d, torque = (float)torqued;
         if (E2 - E > .1) {
-            h_valid = false;
+            // rejected: M and H stay as they were
             dt = (float)(dt / 2.5);
             printf("PANIC!! energy is increasing, so halving the step.\n");
             fprintf(fhp, "PANIC!! energy is increasing, so halving the step. %d, %g, %g, %g, %g \n", tindex, t, dt,
@@ -772,14 +784,14 @@ extern "C" int time_marching(unsigned char *material, float *M, float finaltime,
             }
         } else {
             consecutive_error_count = 0;
-            h_valid = true;
+            accepted = true;
             const double t2 = (double)((float)t + (float)dt);
             dt = (float)(1e-14 / torque / 2);
             t = t2;
             tindex++;
             c->cur = d;
             std::swap(c->mn64, c->mnew64);
-            std::swap(c->phi_n, c->phi);
+            std::swap(h_n, h_2);
             // log line
             double avg[5];
             if (mavg_of(c, c->m32[c->cur], avg)) { status = 1; break; }
@@ -798,7 +810,8 @@ extern "C" int time_marching(unsigned char *material, float *M, float finaltime,
         }
         fflush(NULL);
     }
-    if (!status && mms_get_m(c, M)) status = 1;
+    // M is overwritten with the last accepted state (unchanged if none was)
+    if (!status && accepted && mms_get_m(c, M)) status = 1;
     fclose(fh);
     fclose(fhM);
     fclose(fhp);

@@ -255,6 +255,22 @@ extern "C" int mms_batch_steps(mms_batch *b, int nsteps, float dt, int use_graph
     return 0;
 }
 
+extern "C" int mms_batch_potential(mms_batch *b, const float *charge, float *potential) {
+    MMS_CHECK(b && charge && potential, "null argument");
+    MMS_HIP(hipMemcpyAsync(b->rho, charge, sizeof(float) * b->ntot, hipMemcpyHostToDevice, b->stream));
+    if (b->ndirect_items) {
+        const double scale = (double)(b->gcommon.dx * b->gcommon.dy * b->gcommon.dz) / (double)b->gcommon.dx;
+        if (direct_batch_potential(b->rho, b->phi, b->d_direct, b->ndirect_items, b->d_films, b->gcommon.ay,
+                                   b->gcommon.az, scale, b->stream, &b->prof))
+            return 1;
+    }
+    for (size_t k = 0; k < b->fmm.size(); k++)
+        if (fmm_potential(b->fmm_grid[k], *b->fmm[k], b->rho, b->phi, b->stream, &b->prof)) return 1;
+    MMS_HIP(hipMemcpyAsync(potential, b->phi, sizeof(float) * b->ntot, hipMemcpyDeviceToHost, b->stream));
+    MMS_HIP(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
 extern "C" int mms_batch_sync(mms_batch *b) {
     MMS_CHECK(b, "null batch");
     MMS_HIP(hipStreamSynchronize(b->stream));

@@ -1663,9 +1663,11 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
     MMS_CHECK(P_ >= 1 && P_ <= 9, "FMM expansion order must be 1..9");
     MMS_CHECK(w_ == 2 || w_ == 4 || w_ == 8, "FMM leaf width must be 2, 4 or 8");
     MMS_CHECK(g.zd >= 1 && g.zd <= 16, "FMM supports 1..16 layers");
-    MMS_CHECK(precision_ >= MMS_M2L_FP32 && precision_ <= MMS_M2L_FP16X2, "unknown M2L precision");
+    MMS_CHECK(precision_ >= MMS_M2L_FP32 && precision_ <= MMS_M2L_FP16X2_BF16X3NEAR, "unknown M2L precision");
     P = P_; w = w_; precision = precision_; zd = g.zd;
-    m2l_mode = (precision == MMS_M2L_BF16X3_FP16NEAR) ? MMS_M2L_BF16X3 : precision;
+    m2l_mode = (precision == MMS_M2L_BF16X3_FP16NEAR)     ? MMS_M2L_BF16X3
+               : (precision == MMS_M2L_FP16X2_BF16X3NEAR) ? MMS_M2L_FP16X2
+                                                          : precision;
     near_f16 = (precision == MMS_M2L_FP16 || precision == MMS_M2L_BF16X3_FP16NEAR || precision == MMS_M2L_FP16X2);
     const bool f16_ops = (m2l_mode == MMS_M2L_FP16 || m2l_mode == MMS_M2L_FP16X2);  // equilibrated FP16 operator
     m2l_cols = (cols_ == 8) ? 8 : 4;

@@ -166,9 +166,11 @@ int launch_energy_h(const float *m, const float *h_aos, const uint8_t *matl, lon
 // h_given replaces the computed field; h_out/k_out/slope_out receive the field,
 // dM/dt and the float slope sum.  Null pointers disable each.
 // m_base: FP32 copy of M_n that stays intact through stages 0-2 (the stage-0
-// input); null makes every stage read the FP64 base
+// input); null makes every stage read the FP64 base.  m_keep: FP32 SoA values
+// written to the non-material cells of the stage output (null: the input's);
+// slope_out is 0 on non-material cells, k_out is not written there.
 struct StageExtra { const float *h_given; float *h_out; float *k_out; float *slope_out; int normalize; double *mn_out;
-                    const float *m_base; };
+                    const float *m_base; const float *m_keep; };
 int launch_stage_ex(const Grid &g, const Material &mat, int stage, double dt, const float *phi,
                     const float *m_in, float *m_out, double *mn, float *acc, const uint8_t *matl,
                     const double *t_dev, const StageExtra &ex, hipStream_t s, Profiler *prof);
@@ -188,6 +190,8 @@ int direct_batch_potential(const float *rho, float *phi, const int2 *items, int 
                            float ay, float az, double scale, hipStream_t s, Profiler *prof);
 int launch_aos_from_soa(const float *soa32, float *aos, long n, hipStream_t s);
 int launch_add_time(double *t_dev, double dt, hipStream_t s);
+// zero the FP32 SoA planes [3][n] of m on non-material cells
+int launch_zero_nonmagnetic(float *m, const uint8_t *matl, long n, hipStream_t s);
 int launch_fill_f32(float *p, float v, long n, hipStream_t s);
 
 }  // namespace mms
@@ -205,7 +209,6 @@ struct mms_ctx {
     double *mnew64 = nullptr;  // FP64 proposal M_{n+1} (adaptive time loop)
     float *acc32 = nullptr;    // FP32 slope accumulator (k1 + 2 k2 + 2 k3 + k4)
     float *rho = nullptr, *phi = nullptr;
-    float *phi_n = nullptr;    // potential at M_n (adaptive time loop)
     uint8_t *matl = nullptr;
     double *t_dev = nullptr;
     float *h_aos = nullptr;    // field output / staging, AoS [n][3]

@@ -131,6 +131,7 @@ struct StageArgs {
     double *mn_out;        // FP64 SoA destination of M_{n+1} at stage 3 (null = mn)
     const uint8_t *matl;   // material flags (null = all magnetic)
     const double *t_dev;   // simulation time (device)
+    const float *m_keep;   // FP32 SoA values of the non-material cells' output (null: copy the input)
     const float *h_given;  // AoS field to use instead of computing it (stage 0 of rk4_step)
     float *h_out;          // optional AoS output of the field used
     float *k_out;          // optional AoS output of this stage's dM/dt
@@ -178,7 +179,8 @@ __device__ __forceinline__ void stage_column_r(const StageArgs &A, const float *
                                                float *__restrict__ m_out_, double *__restrict__ mn_,
                                                float *__restrict__ acc_, double *__restrict__ mn_out_,
                                                const float *__restrict__ phi_, const uint8_t *__restrict__ matl_,
-                                               const float *__restrict__ mb_, const float *__restrict__ h_given_,
+                                               const float *__restrict__ mb_, const float *__restrict__ mk_,
+                                               const float *__restrict__ h_given_,
                                                float *__restrict__ h_out_, float *__restrict__ k_out_,
                                                float *__restrict__ slope_out_, int x, int y, int xd, int yd,
                                                int zd_rt, long n, float hx, float hy, float hz) {
@@ -195,7 +197,9 @@ __device__ __forceinline__ void stage_column_r(const StageArgs &A, const float *
     const float gx = A.inv4pi / (2.0f * A.dx), gy = A.inv4pi / (2.0f * A.dy), gz = A.inv4pi / (2.0f * A.dz);
     constexpr bool PLAIN = ST >= 0;
     const int stage = PLAIN ? ST : A.stage;
-    if constexpr (PLAIN) { h_given_ = nullptr; h_out_ = nullptr; k_out_ = nullptr; slope_out_ = nullptr; mn_out_ = nullptr; }
+    if constexpr (PLAIN) {
+        h_given_ = nullptr; h_out_ = nullptr; k_out_ = nullptr; slope_out_ = nullptr; mn_out_ = nullptr; mk_ = nullptr;
+    }
     const bool need_field = (h_given_ == nullptr);
     const bool xr = (x != xd - 1), xl = (x != 0), yu = (y != yd - 1), yl = (y != 0);
     const int ixr = xr ? 1 : 0, ixl = xl ? 1 : 0;
@@ -308,7 +312,12 @@ __device__ __forceinline__ void stage_column_r(const StageArgs &A, const float *
             }
             ox[c] = (float)n0; oy[c] = (float)n1; oz[c] = (float)n2;
         } else {
-            ox[c] = mxc; oy[c] = myc; oz[c] = mzc;
+            // non-material cells: the reference leaves M2 and this_slope as
+            // the caller passed them and the slope sum at 0
+            // (ode_functions.cpp:91-95,102)
+            if (mk_) { ox[c] = mk_[c]; oy[c] = mk_[n + c]; oz[c] = mk_[2 * n + c]; }
+            else { ox[c] = mxc; oy[c] = myc; oz[c] = mzc; }
+            if (slope_out_) { slope_out_[3 * c] = 0.0f; slope_out_[3 * c + 1] = 0.0f; slope_out_[3 * c + 2] = 0.0f; }
         }
         pprev = pcur; pcur = pnext;
         mxp = mxc; myp = myc; mzp = mzc;
@@ -322,7 +331,8 @@ __device__ __forceinline__ void stage_column(const StageArgs &A, int x, int y, i
     if (x >= xd || y >= yd) return;
     stage_column_r<ZD, ST>(A, A.m_in + off, A.m_out + off, A.mn + off, A.acc + off, A.mn_out ? A.mn_out + off : nullptr,
                        A.phi ? A.phi + off : nullptr, A.matl ? A.matl + off : nullptr,
-                       A.m_base ? A.m_base + off : nullptr, A.h_given, A.h_out, A.k_out, A.slope_out, x, y, xd, yd,
+                       A.m_base ? A.m_base + off : nullptr, A.m_keep ? A.m_keep + off : nullptr, A.h_given, A.h_out,
+                       A.k_out, A.slope_out, x, y, xd, yd,
                        zd_rt, n, hx, hy, hz);
 }
 
@@ -664,6 +674,10 @@ __global__ void aos_from_soa_kernel(const float *__restrict__ s32, float *__rest
         aos[3 * i] = s32[i]; aos[3 * i + 1] = s32[n + i]; aos[3 * i + 2] = s32[2 * n + i];
     }
 }
+__global__ void zero_nonmagnetic_kernel(float *__restrict__ m, const uint8_t *__restrict__ matl, long n) {
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L)
+        if (!matl[i]) { m[i] = 0.0f; m[n + i] = 0.0f; m[2 * n + i] = 0.0f; }
+}
 __global__ void add_time_kernel(double *t, double dt) { *t += dt; }
 __global__ void fill_f32_kernel(float *p, float v, long n) {
     for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L) p[i] = v;
@@ -765,11 +779,11 @@ int launch_stage_ex(const Grid &g, const Material &mat, int stage, double dt, co
     A.phi = phi; A.m_in = m_in; A.m_out = m_out; A.mn = mn; A.acc = acc; A.matl = matl;
     A.t_dev = t_dev; A.stage = stage; A.dt = dt;
     A.h_given = ex.h_given; A.h_out = ex.h_out; A.k_out = ex.k_out; A.slope_out = ex.slope_out;
-    A.normalize = ex.normalize; A.mn_out = ex.mn_out; A.m_base = ex.m_base;
+    A.normalize = ex.normalize; A.mn_out = ex.mn_out; A.m_base = ex.m_base; A.m_keep = ex.m_keep;
     MMS_CHECK(stage != 0 || !ex.m_base || ex.m_base == m_in, "stage 0 takes M_n from its own input");
     MMS_CHECK(stage >= 0 && stage <= 3, "RK4 stage must be 0..3");
     dim3 grid(xcd_grid(g)), block(BX, BY);
-    const bool plain = !ex.h_given && !ex.h_out && !ex.k_out && !ex.slope_out && !ex.mn_out;
+    const bool plain = !ex.h_given && !ex.h_out && !ex.k_out && !ex.slope_out && !ex.mn_out && !ex.m_keep;
     const uintptr_t al = reinterpret_cast<uintptr_t>(m_in) | reinterpret_cast<uintptr_t>(m_out) |
                          reinterpret_cast<uintptr_t>(mn) | reinterpret_cast<uintptr_t>(acc) |
                          reinterpret_cast<uintptr_t>(phi) | reinterpret_cast<uintptr_t>(ex.m_base);
@@ -867,6 +881,11 @@ int launch_soa_from_aos(const float *aos, float *soa32, double *soa64, long n, h
 }
 int launch_aos_from_soa(const float *soa32, float *aos, long n, hipStream_t s) {
     hipLaunchKernelGGL(aos_from_soa_kernel, dim3(grid1d(n)), dim3(256), 0, s, soa32, aos, n);
+    MMS_HIP(hipGetLastError());
+    return 0;
+}
+int launch_zero_nonmagnetic(float *m, const uint8_t *matl, long n, hipStream_t s) {
+    hipLaunchKernelGGL(zero_nonmagnetic_kernel, dim3(grid1d(n)), dim3(256), 0, s, m, matl, n);
     MMS_HIP(hipGetLastError());
     return 0;
 }

@@ -72,6 +72,7 @@ SIGNATURES = {
     "mms_batch_get_m": (I, [VP, f32p]),
     "mms_batch_steps": (I, [VP, I, F, I]),
     "mms_batch_sync": (I, [VP]),
+    "mms_batch_potential": (I, [VP, f32p, f32p]),
     "mms_batch_info": (I, [VP, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(C.c_long), C.POINTER(I)]),
     "mms_batch_autotune": (I, [VP, I, C.c_char_p, I]),
     "mms_batch_profile_begin": (I, [VP]),

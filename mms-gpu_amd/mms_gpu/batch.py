@@ -75,6 +75,14 @@ class FilmBatch:
     def sync(self):
         check(lib().mms_batch_sync(self._b), "mms_batch_sync")
 
+    def potential(self, charge):
+        """Potential of every film (packed host arrays, film order) through
+        the batch's own solvers."""
+        out = np.zeros(self.ntot, np.float32)
+        check(lib().mms_batch_potential(self._b, np.ascontiguousarray(charge, np.float32).ravel(), out),
+              "mms_batch_potential")
+        return out
+
     def info(self):
         a, b, c, e = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         d = C.c_long()

@@ -170,6 +170,53 @@ def probes(seed, n_cells, count, stream=5):
     return np.sort(idx[rng_order][:count])
 
 
+def charge_of(M, dims, d):
+    """rho = div M with the reference stencil (vector_functions.cpp:8-28:
+    central differences, out-of-grid neighbours 0, times 1/2), float32."""
+    xd, yd, zd = dims
+    m = np.asarray(M, np.float32).reshape(zd, yd, xd, 3)
+
+    def sh(a, ax, step):
+        out = np.zeros_like(a)
+        src = [slice(None)] * 3
+        dst = [slice(None)] * 3
+        if step > 0:
+            dst[ax], src[ax] = slice(0, -1), slice(1, None)
+        else:
+            dst[ax], src[ax] = slice(1, None), slice(0, -1)
+        out[tuple(dst)] = a[tuple(src)]
+        return out
+    dx, dy, dz = (np.float32(v) for v in d)
+    return (((sh(m[..., 0], 2, 1) - sh(m[..., 0], 2, -1)) / dx + (sh(m[..., 1], 1, 1) - sh(m[..., 1], 1, -1)) / dy
+             + (sh(m[..., 2], 0, 1) - sh(m[..., 2], 0, -1)) / dz) * np.float32(0.5)).astype(np.float32).ravel()
+
+
+def demag_field_error(film, w, nprobe=8192, seed=7):
+    """Relative error of the demag field H = grad(phi)/4pi (reference central
+    differences) of the film's current state at `nprobe` seeded interior
+    probe cells (all six neighbours inside the grid): the film's solver
+    against an FP64 direct sum evaluated on the device at each probe and its
+    six neighbours.  Returns (relative L2 over the probes, max |dH| / max |H|,
+    number of probes)."""
+    ch = charge_of(film.get_m(), w.dims, w.d)
+    phi = film.potential(ch)
+    xd, yd, zd = w.dims
+    ix, iy, iz = xd - 2, yd - 2, zd - 2
+    pr = probes(seed, ix * iy * iz, nprobe)
+    pz, py, px = 1 + pr // (ix * iy), 1 + (pr // ix) % iy, 1 + pr % ix
+    offs = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
+    idx = np.stack([((pz + oz) * yd + (py + oy)) * xd + (px + ox) for ox, oy, oz in offs], 1)
+    ex = film.potential_probes_f64(ch, idx.ravel()).reshape(-1, 6)
+    got = phi[idx].astype(float)
+
+    def grad(v):
+        return np.stack([(v[:, 0] - v[:, 1]) / (2 * float(w.dx)), (v[:, 2] - v[:, 3]) / (2 * float(w.dy)),
+                         (v[:, 4] - v[:, 5]) / (2 * float(w.dz))], 1) / (4 * np.pi)
+    He, Hg = grad(ex), grad(got)
+    return (float(np.linalg.norm(Hg - He) / np.linalg.norm(He)),
+            float(np.abs(Hg - He).max() / np.abs(He).max()), int(len(pr)))
+
+
 def single_domain_square(xdim, ydim, zdim):
     """The reference's default geometry (main.cpp:120-129,190-220): the 1-cell
     border of the mask is non-magnetic and only layers 1..zdim-2 are magnetic,

@@ -162,14 +162,27 @@ def hfield(M, dims, d, P, mu_0, Ms, A, demag=1, exchange=1, external=0, use_fmm=
 
 def rk4_step(M, H, material, dims, d, dt, P, mu_0, Ms, A, alfa, gamma, demag=1, exchange=1,
              external=0, use_fmm=0, normalize=1, t=0.0, hext=None, combine64=0):
-    """One reference rk4_step.  H is the field at M (k1); returns (M2, H_last)."""
+    """One reference rk4_step.  H is the field at M (k1); returns (M2, H_last).
+    M2 starts as a copy of M (non-material cells keep it)."""
+    M2, Hb, _, _ = rk4_step_full(M, H, material, dims, d, dt, P, mu_0, Ms, A, alfa, gamma, demag,
+                                 exchange, external, use_fmm, normalize, t, hext, combine64)
+    return M2, Hb
+
+
+def rk4_step_full(M, H, material, dims, d, dt, P, mu_0, Ms, A, alfa, gamma, demag=1, exchange=1,
+                  external=0, use_fmm=0, normalize=1, t=0.0, hext=None, combine64=0, M2_in=None,
+                  this_slope_in=None):
+    """One reference rk4_step with every output: returns (M2, H_last, slope,
+    this_slope).  M2_in / this_slope_in are the caller's arrays on entry (the
+    reference leaves them untouched on non-material cells); default: a copy
+    of M and zeros."""
     xd, yd, zd = dims
     n = xd * yd * zd
     M = _c(M).ravel()
-    M2 = M.copy()  # non-material cells keep M (zero there), as in the reference
+    M2 = M.copy() if M2_in is None else _c(M2_in).ravel().copy()
     Hb = _c(H).ravel().copy()
     slope = np.zeros(3 * n, np.float32)
-    ts = np.zeros(3 * n, np.float32)
+    ts = np.zeros(3 * n, np.float32) if this_slope_in is None else _c(this_slope_in).ravel().copy()
     ch = np.zeros(n, np.float32)
     pot = np.zeros(n, np.float32)
     t2 = C.c_float(0)
@@ -180,7 +193,7 @@ def rk4_step(M, H, material, dims, d, dt, P, mu_0, Ms, A, alfa, gamma, demag=1, 
     lib().or_rk4_step(t, dt, C.byref(t2), _c(material, np.uint8).ravel(), M, M2, Hb, slope, ts,
                       ch, pot, xd, yd, zd, *d, P, mu_0, Ms, A, alfa, gamma, demag, exchange,
                       external, use_fmm, normalize, hp, combine64)
-    return M2, Hb
+    return M2, Hb, slope, ts
 
 
 def energy_torque(material, M, H, mu_0, Ms, d):

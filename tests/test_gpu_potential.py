@@ -332,3 +332,26 @@ def test_autotune_keeps_results(mg, solver):
     else:
         assert shape["direct_split"] in (1, 2, 4)
         assert np.abs(after - before).max() <= 1e-6 * np.abs(before).max()
+
+
+@pytest.mark.parametrize("side,leaf", [(1024, 2), (2048, 2), (2048, 4), (4096, 8)])
+def test_fmm_deep_trees_against_fp64_probes(mg, side, leaf):
+    """Grids deep enough for 16 x 16-leaf subtrees (leaf level >= 9): at
+    1024^2 / leaf 2 the subtree roots at level 5 form the coarse locals
+    themselves; at 2048^2 / leaf 2 (roots at level 6) the coarse M2M, the
+    coarse downward pass and the per-level L2L launches run; 2048^2 / leaf 4
+    and 4096^2 / leaf 8 end the downward pass with the fused L2P on 256
+    leaves.  One layer of domain-pattern charges, P = 6, FP32: potential on
+    4,096 seeded probes within 2e-4 (relative L2) of the device FP64 direct
+    sum (the same expansion on shallower trees sits at 4e-5)."""
+    from mms_gpu import workloads as wl
+    w = wl.c3_domains(side=side, zd=1, domain=32)
+    ch = wl.charge_of(w.M, w.dims, w.d)
+    _say("fmm deep tree", side, "leaf", leaf)
+    probes = wl.probes(2010, w.n, 4096)
+    with mg.Film(*w.dims, *w.d, solver=mg.SOLVER_FMM, P=6, leaf_width=leaf, m2l_precision=mg.M2L_FP32) as f:
+        pot = f.potential(ch)
+        ex = f.potential_probes_f64(ch, probes)
+    err = _rel(pot[probes], ex)
+    _say(f"  rel L2 {err:.2e}, rel max {np.abs(pot[probes] - ex).max() / np.abs(ex).max():.2e}")
+    assert err < 2e-4

@@ -49,7 +49,11 @@ int calc_potential_exact(const float *charge, int xdim, int ydim, int zdim,
                          float dx, float dy, float dz, float *potential,
                          int use_gpu, int verbosity);
 
-/* Same computation as calc_potential_exact.
+/* Same physics as calc_potential_exact, in the float arithmetic and
+ * summation order of the reference's CUDA kernel calc_potential_exact_kernel
+ * (potential_calc.cu:18-116: per target, 1024 partial sums over contiguous
+ * source chunks, then a halving tree), so results match that kernel bit for
+ * bit wherever its compiler rounded every operation on its own.
  * Replaces: int calc_potential_exact_gpu(...)  helper_functions.hpp:37-40,
  *           defined potential_calc.cu:120-188. */
 int calc_potential_exact_gpu(const float *charge, int xdim, int ydim, int zdim,
@@ -109,7 +113,12 @@ typedef struct mms_ctx mms_ctx;
 
 enum mms_solver {
     MMS_SOLVER_DIRECT = 0, /* O(N^2) direct sum                                */
-    MMS_SOLVER_FMM = 1     /* P2M, M2M, M2L, L2L, L2P + near-field P2P          */
+    MMS_SOLVER_FMM = 1,    /* P2M, M2M, M2L, L2L, L2P + near-field P2P          */
+    MMS_SOLVER_DIRECT_REFGPU = 2 /* O(N^2) direct sum in the float arithmetic
+                              and summation order of the reference's CUDA
+                              kernel (potential_calc.cu:18-116): 1024
+                              contiguous-chunk partial sums per target and
+                              the halving tree, bit for bit             */
 };
 enum mms_m2l_precision {
     MMS_M2L_FP32 = 0,   /* FP32 operands, FP32 accumulation                   */

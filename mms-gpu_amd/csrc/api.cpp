@@ -124,6 +124,7 @@ static void ctx_free(mms_ctx *c) {
 
 int mms_ctx::potential(const float *rho_d, float *phi_d) {
     if (p.solver == MMS_SOLVER_FMM) return fmm_potential(g, fmm, rho_d, phi_d, stream, &prof);
+    if (p.solver == MMS_SOLVER_DIRECT_REFGPU) return direct_refgpu_potential(g, rho_d, phi_d, stream, &prof);
     return direct_potential(g, direct, rho_d, phi_d, stream, &prof);
 }
 
@@ -154,7 +155,8 @@ static int validate_params(const mms_params *p) {
     MMS_CHECK(p->xdim > 0 && p->ydim > 0 && p->zdim > 0, "grid dimensions must be positive");
     MMS_CHECK((long)p->xdim * p->ydim * p->zdim < (1L << 31), "grid too large (>= 2^31 cells)");
     MMS_CHECK(p->dx > 0 && p->dy > 0 && p->dz > 0, "cell sizes must be positive");
-    MMS_CHECK(p->solver == MMS_SOLVER_DIRECT || p->solver == MMS_SOLVER_FMM, "unknown solver");
+    MMS_CHECK(p->solver == MMS_SOLVER_DIRECT || p->solver == MMS_SOLVER_FMM || p->solver == MMS_SOLVER_DIRECT_REFGPU,
+              "unknown solver");
     MMS_CHECK(p->m2l_precision >= MMS_M2L_FP32 && p->m2l_precision <= MMS_M2L_FP16X2_BF16X3NEAR, "unknown M2L precision");
     return 0;
 }
@@ -182,7 +184,7 @@ extern "C" int mms_create(mms_ctx **out, const mms_params *p, const unsigned cha
     if (c->p.direct_split == 0) c->p.direct_split = 1;
     if (c->p.solver == MMS_SOLVER_FMM)
         st = c->fmm.init(c->g, c->p.P, c->p.leaf_width, c->p.m2l_precision, c->p.m2l_cols);
-    else
+    else if (c->p.solver == MMS_SOLVER_DIRECT)
         st = c->direct.init(c->g, c->p.direct_split);
     if (st) { ctx_free(c.get()); return 1; }
     *out = c.release();
@@ -389,6 +391,8 @@ extern "C" int mms_autotune(mms_ctx *c, int reps, char *report, int report_len) 
         if (c->fmm.init(c->g, c->p.P, best_w, c->p.m2l_precision, best_c)) return 1;
         snprintf(line, sizeof line, "chosen: leaf=%d m2l_cols=%d\n", best_w, best_c);
         rep += line;
+    } else if (c->p.solver == MMS_SOLVER_DIRECT_REFGPU) {
+        rep += "reference-order direct sum: fixed launch shape (the reference's 1024-way split)\n";
     } else {
         int best_s = c->p.direct_split;
         double best = 1e30;
@@ -491,12 +495,21 @@ mms_ctx *cached_ctx(const mms_params &p, const unsigned char *material) {
     return c;
 }
 
+// MMS_GPU_REFERENCE_ORDER=1: every direct-sum solve of the reference entry
+// points (calc_potential_exact, Hfield, rk4_step, time_marching without the
+// FMM) runs in the reference CUDA kernel's summation order, as the
+// reference's use_gpu = 1 path does, instead of the tuned kernel
+int direct_solver() {
+    const char *e = std::getenv("MMS_GPU_REFERENCE_ORDER");
+    return (e && e[0] == '1') ? MMS_SOLVER_DIRECT_REFGPU : MMS_SOLVER_DIRECT;
+}
+
 mms_params base_params(int xd, int yd, int zd, float dx, float dy, float dz) {
     mms_params p;
     memset(&p, 0, sizeof(p));
     p.xdim = xd; p.ydim = yd; p.zdim = zd;
     p.dx = dx; p.dy = dy; p.dz = dz;
-    p.solver = MMS_SOLVER_DIRECT;
+    p.solver = direct_solver();
     p.P = 4;
     p.leaf_width = 4;
     p.demag = 1;
@@ -525,7 +538,7 @@ void set_solver(mms_params &p, int use_fmm, int P) {
             return;
         }
     }
-    p.solver = MMS_SOLVER_DIRECT;
+    p.solver = direct_solver();
     p.P = 0;
 }
 }  // namespace
@@ -547,7 +560,12 @@ extern "C" int calc_potential_exact(const float *charge, int xdim, int ydim, int
 
 extern "C" int calc_potential_exact_gpu(const float *charge, int xdim, int ydim, int zdim, float dx, float dy,
                                         float dz, float *potential) {
-    return calc_potential_exact(charge, xdim, ydim, zdim, dx, dy, dz, potential, 1, 0);
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    mms_params p = base_params(xdim, ydim, zdim, dx, dy, dz);
+    p.solver = MMS_SOLVER_DIRECT_REFGPU;
+    mms_ctx *c = cached_ctx(p, nullptr);
+    if (!c || mms_potential(c, charge, potential)) return EXIT_FAILURE;
+    return EXIT_SUCCESS;
 }
 
 extern "C" int fmm_calc(const float *charge, float *potential, int xdim, int ydim, int zdim, float dx, float dy,

@@ -177,7 +177,86 @@ __global__ __launch_bounds__(TB) void direct_batch_kernel(const float *__restric
     if (t < n) phi[t] = (float)(acc * scale);
 }
 
+// The summation order and float arithmetic of the reference's CUDA kernel
+// calc_potential_exact_kernel (potential_calc.cu:18-116), for callers that
+// want its results bit for bit:
+//   - 1024 partial sums per target: partial t adds the sources
+//     [t * tstride, (t + 1) * tstride), tstride = ceilf(N / 1024.f), in
+//     index order, each term (charge_i * dV) / sqrtf(Rx^2 + Ry^2 + Rz^2)
+//     with R = ((x - x_) dx, (y - y_) dy, (z - z_) dz), dV = dx dy dz, the
+//     self term skipped (potential_calc.cu:57-75);
+//   - the partials added by the halving tree s[t] += s[t + h], h = 512 .. 1
+//     (potential_calc.cu:82-111).
+// Every operation is rounded on its own (no fused multiply-add), with
+// IEEE division and square root.  A workgroup is the reference's 1024
+// virtual threads applied to RT targets at once: each source is loaded and
+// its coordinates decoded once for the RT targets, and the RT reduction
+// trees run together in LDS.
+constexpr int RT = 8;
+
+__global__ __launch_bounds__(1024) void direct_refgpu_kernel(const float *__restrict__ rho, float *__restrict__ phi,
+                                                             int xd, int yd, int n, float dx, float dy, float dz,
+                                                             int tstride) {
+#pragma clang fp contract(off)
+    __shared__ float sdata[RT][1024];
+    const int t = threadIdx.x;
+    const int b0 = blockIdx.x * RT;
+    const int plane = xd * yd;
+    int tx[RT], ty[RT], tz[RT];
+#pragma unroll
+    for (int k = 0; k < RT; k++) {
+        const int bi = (b0 + k < n) ? b0 + k : 0;
+        tx[k] = bi % xd;
+        ty[k] = ((bi - tx[k]) / xd) % yd;
+        tz[k] = (bi - tx[k] - ty[k] * xd) / plane;
+    }
+    float pot[RT];
+#pragma unroll
+    for (int k = 0; k < RT; k++) pot[k] = 0.0f;
+    const int ti1 = t * tstride;
+    if (ti1 < n) {
+        const float dV = dx * dy * dz;
+        const int i1 = (ti1 + tstride < n) ? ti1 + tstride : n;
+        for (int i = ti1; i < i1; i++) {
+            const float q = rho[i] * dV;
+            const int x_ = i % xd;
+            const int y_ = ((i - x_) / xd) % yd;
+            const int z_ = (i - x_ - y_ * xd) / plane;
+#pragma unroll
+            for (int k = 0; k < RT; k++) {
+                if (b0 + k == i) continue;
+                const float Rx = (float)(tx[k] - x_) * dx, Ry = (float)(ty[k] - y_) * dy, Rz = (float)(tz[k] - z_) * dz;
+                const float r = sqrtf(Rx * Rx + Ry * Ry + Rz * Rz);
+                pot[k] += q / r;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RT; k++) sdata[k][t] = pot[k];
+    __syncthreads();
+    for (int h = 512; h > 0; h >>= 1) {
+        if (t < h) {
+#pragma unroll
+            for (int k = 0; k < RT; k++) sdata[k][t] += sdata[k][t + h];
+        }
+        __syncthreads();
+    }
+    if (t < RT && b0 + t < n) phi[b0 + t] = sdata[t][0];
+}
+
 }  // namespace
+
+int direct_refgpu_potential(const Grid &g, const float *rho, float *phi, hipStream_t s, Profiler *prof) {
+    KTimer kt(prof, K_DIRECT);
+    const long n = g.n();
+    MMS_CHECK(n < (1L << 30), "reference-order direct sum: grid too large");
+    // tstride as the reference computes it: ceilf(N / (float)blockDim.x)
+    const int tstride = (int)ceilf((float)n / 1024.0f);
+    hipLaunchKernelGGL(direct_refgpu_kernel, dim3((unsigned)((n + RT - 1) / RT)), dim3(1024), 0, s, rho, phi, g.xd,
+                       g.yd, (int)n, g.dx, g.dy, g.dz, tstride);
+    MMS_HIP(hipGetLastError());
+    return 0;
+}
 
 int direct_batch_potential(const float *rho, float *phi, const int2 *items, int nitems, const BatchFilm *films,
                            float ay, float az, double scale, hipStream_t s, Profiler *prof) {

@@ -80,6 +80,8 @@ struct DirectPlan {
 };
 int direct_potential(const Grid &g, DirectPlan &plan, const float *rho, float *phi,
                      hipStream_t s, Profiler *prof);
+// the float arithmetic and summation order of the reference's CUDA direct-sum kernel
+int direct_refgpu_potential(const Grid &g, const float *rho, float *phi, hipStream_t s, Profiler *prof);
 int direct_probes_f64(const Grid &g, const float *rho, const int64_t *probes, int np,
                       double *out, hipStream_t s, Profiler *prof);
 

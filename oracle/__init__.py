@@ -46,6 +46,7 @@ def lib():
         _sig(L, "or_gradient_3d", None, [f32p, I, I, I, F, F, F, F, f32p])
         _sig(L, "or_add_exchange_field", None, [f32p, F, F, F, I, I, I, F, F, F, f32p])
         _sig(L, "or_calc_potential_exact", I, [f32p, I, I, I, F, F, F, f32p])
+        _sig(L, "or_calc_potential_exact_refgpu", I, [f32p, I, I, I, F, F, F, f32p])
         _sig(L, "or_potential_exact_f64", None, [f32p, I, I, I, D, D, D, f64p])
         _sig(L, "or_potential_probes_f64", None, [f32p, I, I, I, D, D, D, i64p, I, f64p])
         _sig(L, "or_tree_lists", None, [I, i32p, i32p])
@@ -114,6 +115,14 @@ def calc_potential_exact(charge, dims, d):
     xd, yd, zd = dims
     pot = np.zeros(xd * yd * zd, np.float32)
     lib().or_calc_potential_exact(_c(charge).ravel(), xd, yd, zd, *d, pot)
+    return pot
+
+
+def calc_potential_exact_refgpu(charge, dims, d):
+    """The reference CUDA kernel's order (potential_calc.cu:18-116), FP32."""
+    xd, yd, zd = dims
+    pot = np.zeros(xd * yd * zd, np.float32)
+    lib().or_calc_potential_exact_refgpu(_c(charge).ravel(), xd, yd, zd, *d, pot)
     return pot
 
 

@@ -25,6 +25,11 @@
  *   or_calc_potential_exact (FP32) / or_potential_*_f64
  *       two-charge closed form, self-exclusion, linearity, FP32 vs FP64 gap,
  *       brute force vs an independent FFT convolution (tests/).
+ *   or_calc_potential_exact_refgpu
+ *       known answers derived by hand from the kernel's documented order
+ *       (charges with exact terms whose FP32 sum depends on the order: chunk
+ *       sequence and tree pairing give one value, sequential and exact sums
+ *       others), FP32 vs FP64 gap, closed forms (tests/).
  *   or_fmm_calc (reference tree-code, P <= 4)
  *       vs FP64 direct sum within the order-P truncation bound; error falls
  *       with P; reduces to the exact sum when every box is a leaf neighbour.
@@ -224,6 +229,52 @@ int or_calc_potential_exact(const float *charge, int xdim, int ydim, int zdim,
                         }
                 }
             }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* potential_calc.cu:18-116 calc_potential_exact_kernel (the reference's     */
+/* CUDA direct sum), restated on the CPU: for target bi, thread t of the      */
+/* 1024-thread block sums sources [t tstride, (t+1) tstride) in index order,  */
+/* tstride = ceilf(N / 1024.f) (potential_calc.cu:26,57-75); the 1024         */
+/* partials are added by the halving tree s[t] += s[t + h], h = 512 .. 1      */
+/* (potential_calc.cu:82-111).  Every operation is rounded on its own (the    */
+/* oracle is built with -ffp-contract=off).  The block/target assignment of   */
+/* the kernel (bstride) does not change any value and is not restated.        */
+/* ------------------------------------------------------------------------ */
+int or_calc_potential_exact_refgpu(const float *charge, int xdim, int ydim, int zdim,
+                                   float dx, float dy, float dz, float *potential) {
+    const int N = xdim * ydim * zdim;
+    const int tstride = (int)ceilf(N / (fptype)1024);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int bi = 0; bi < N; bi++) {
+        float sdata[1024];
+        const int xs = bi % xdim;
+        const int ys = ((bi - xs) / xdim) % ydim;
+        const int zs = (bi - xs - ys * xdim) / (xdim * ydim);
+        for (int t = 0; t < 1024; t++) {
+            sdata[t] = 0;
+            const int ti1 = t * tstride;
+            if (ti1 >= N) continue;
+            float pot = 0;
+            for (int i = ti1; i < ti1 + tstride; i++) {
+                if (i >= N) continue;
+                float dV = dx * dy * dz;
+                float q = charge[i] * dV;
+                int x_ = i % xdim;
+                int y_ = ((i - x_) / xdim) % ydim;
+                int z_ = (i - x_ - y_ * xdim) / (xdim * ydim);
+                if (bi != i) {
+                    v3 R = v3_make((xs - x_) * dx, (ys - y_) * dy, (zs - z_) * dz);
+                    pot += q / v3_mag(R);
+                }
+            }
+            sdata[t] = pot;
+        }
+        for (int h = 512; h > 0; h >>= 1)
+            for (int t = 0; t < h; t++) sdata[t] += sdata[t + h];
+        potential[bi] = sdata[0];
+    }
     return 0;
 }
 

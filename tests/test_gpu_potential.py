@@ -35,9 +35,11 @@ def test_direct_c1_against_fp64_and_reference_path(mg, orc):
     assert np.abs(pot - ex).max() < 1e-5 * np.abs(ex).max()
     refpath = orc.calc_potential_exact(ch, w.dims, w.d)   # the reference's FP32 CPU loop
     assert _rel(pot, refpath) < 3e-6
+    # calc_potential_exact_gpu: the reference CUDA kernel's order, bit for bit
     pot2 = np.zeros_like(ch)
     assert mg.calc_potential_exact_gpu(ch, *w.dims, *w.d, pot2) == 0
-    assert np.array_equal(pot, pot2)
+    assert np.array_equal(pot2, orc.calc_potential_exact_refgpu(ch, w.dims, w.d))
+    assert _rel(pot2, ex) < 1e-5
 
 
 @pytest.mark.parametrize("dims", [(1, 1, 1), (2, 1, 1), (7, 5, 3), (33, 1, 2), (1, 40, 1), (17, 19, 4)])
@@ -355,3 +357,59 @@ def test_fmm_deep_trees_against_fp64_probes(mg, side, leaf):
     err = _rel(pot[probes], ex)
     _say(f"  rel L2 {err:.2e}, rel max {np.abs(pot[probes] - ex).max() / np.abs(ex).max():.2e}")
     assert err < 2e-4
+
+
+@pytest.mark.parametrize("dims", [(32, 32, 3), (7, 5, 3), (17, 19, 4), (1, 1, 1), (2, 1, 1), (64, 16, 1),
+                                  (2049, 1, 1), (45, 45, 3)])
+def test_calc_potential_exact_gpu_matches_reference_kernel_order(mg, orc, dims):
+    """calc_potential_exact_gpu reproduces the float arithmetic and summation
+    order of the reference's CUDA kernel (potential_calc.cu:18-116): bitwise
+    equal to its CPU restatement for N below, at and above 1024 and not a
+    multiple of it, anisotropic cells included."""
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(n)
+    ch = (rng.normal(size=n) * 1e14).astype(F)
+    ch[rng.random(n) < 0.1] = 0.0
+    d = (F(5e-9), F(4e-9), F(3e-9)) if n % 2 else D5
+    _say("reference-order direct sum", dims)
+    pot = np.full(n, np.nan, F)
+    assert mg.calc_potential_exact_gpu(ch, *dims, *d, pot) == 0, mg.last_error()
+    assert np.array_equal(pot, orc.calc_potential_exact_refgpu(ch, dims, d))
+
+
+def test_reference_order_known_answers_on_gpu(mg):
+    """The hand-derived known answers of the kernel's order (exact terms whose
+    FP32 sum is 1 in the kernel's order, 0 sequentially, 2 exactly)."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(__file__))
+    from test_oracle_math import refgpu_order_kats
+    one = (F(1.0), F(1.0), F(1.0))
+    for n, q, expect, _, _ in refgpu_order_kats():
+        pot = np.zeros(n, F)
+        assert mg.calc_potential_exact_gpu(q, n, 1, 1, *one, pot) == 0, mg.last_error()
+        assert pot[0] == expect, (n, pot[0])
+        with mg.Film(n, 1, 1, *one, solver=mg.SOLVER_DIRECT_REFGPU) as f:
+            assert f.potential(q)[0] == expect
+
+
+def test_reference_order_switch_for_entry_points(mg, orc, monkeypatch):
+    """MMS_GPU_REFERENCE_ORDER=1 runs every direct-sum solve of the reference
+    entry points in the reference kernel's order (the reference's use_gpu = 1
+    path): Hfield's potential then equals the restated kernel bit for bit."""
+    from mms_gpu import workloads as wl
+    w = wl.c1_square()
+    n = w.n
+    monkeypatch.setenv("MMS_GPU_REFERENCE_ORDER", "1")
+    H = np.zeros(3 * n, F)
+    ch = np.zeros(n, F)
+    pot = np.zeros(n, F)
+    st = mg.Hfield(w.M, H, ch, pot, F(0), *w.dims, *w.d, 0, wl.MU_0, wl.MS, wl.AEXCH, 1, 1, 0, 0, 1, 0)
+    assert st == 0, mg.last_error()
+    assert np.array_equal(pot, orc.calc_potential_exact_refgpu(ch, w.dims, w.d))
+    pe = np.zeros(n, F)
+    assert mg.calc_potential_exact(ch, *w.dims, *w.d, pe, 1, 0) == 0
+    assert np.array_equal(pe, pot)
+    monkeypatch.setenv("MMS_GPU_REFERENCE_ORDER", "0")
+    assert mg.calc_potential_exact(ch, *w.dims, *w.d, pe, 1, 0) == 0
+    assert not np.array_equal(pe, pot) and _rel(pe, pot) < 3e-6

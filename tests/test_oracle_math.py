@@ -188,3 +188,65 @@ def test_llg_orthogonal_and_renormalised(orc):
         h = (rng.normal(size=3) * 1e5).astype(F)
         orc.lib().or_llg_mprime(m, h, 0.5, 2.211e5, 8.6e5, o)
         assert abs(np.dot(o.astype(float), m.astype(float))) <= 1e-5 * np.linalg.norm(o) * np.linalg.norm(m)
+
+
+def refgpu_order_kats():
+    """Known answers for the reference CUDA kernel's summation order
+    (potential_calc.cu:18-116), derived by hand.  On a line of cells with
+    dx = dy = dz = 1 the term of source i at target 0 is charge_i / i, so
+    charge_i = c_i * i makes every term exactly c_i; the FP32 sum of
+    c = (2^24, 1, 1, -2^24) then depends on the order: exact 2, sequential
+    (((2^24 + 1) + 1) - 2^24 = 0 under round-to-even), and the kernel's
+    order gives 1.
+      * N = 8 (tstride 1): partial t holds term t; the halving tree adds
+        ((p0 + p4) + (p2 + p6)) + ((p1 + p5) + (p3 + p7)); with terms
+        p1 = 2^24, p5 = 1, p3 = 1, p7 = -2^24: (2^24 + 1) -> 2^24,
+        (1 - 2^24) exact, total 1.
+      * N = 2048 (tstride 2): partial 0 = term 1, partial t = term 2t + term
+        2t+1; with terms 1 = 1, 2 = 2^24, 3 = 1, 1026 = -2^24: partial 1 =
+        2^24 + 1 -> 2^24, partial 513 = -2^24, the tree's first level pairs
+        1 with 513 (0 exactly) and the total is partial 0 = 1.
+    Returns [(N, charge, expected phi at target 0, sequential, exact)]."""
+    big = float(2 ** 24)
+    out = []
+    c = {1: big, 5: 1.0, 3: 1.0, 7: -big}
+    q = np.zeros(8, F)
+    for i, v in c.items():
+        q[i] = F(v * i)
+    out.append((8, q, 1.0, 0.0, 2.0))
+    c = {1: 1.0, 2: big, 3: 1.0, 1026: -big}
+    q = np.zeros(2048, F)
+    for i, v in c.items():
+        q[i] = F(v * i)
+    out.append((2048, q, 1.0, 0.0, 2.0))
+    return out
+
+
+def test_refgpu_order_known_answers(orc):
+    one = (F(1.0), F(1.0), F(1.0))
+    for n, q, expect, seq, exact in refgpu_order_kats():
+        dims = (n, 1, 1)
+        got = orc.calc_potential_exact_refgpu(q, dims, one)
+        assert got[0] == expect, (n, got[0])
+        # the CPU branch adds in source order: the sequential value
+        assert orc.calc_potential_exact(q, dims, one)[0] == seq
+        assert orc.potential_exact_f64(q, dims, one)[0] == exact
+
+
+def test_refgpu_order_accuracy_and_closed_forms(orc):
+    """FP32 vs FP64 gap on random charges (N above and below 1024 and not a
+    multiple of it), the two-charge closed form, self exclusion."""
+    rng = np.random.default_rng(8)
+    for dims, d in (((7, 5, 3), DX), ((33, 17, 3), (F(5e-9), F(4e-9), F(3e-9))), ((32, 32, 1), DX)):
+        n = int(np.prod(dims))
+        q = rng.normal(size=n).astype(F) * F(1e14)
+        got = orc.calc_potential_exact_refgpu(q, dims, d)
+        ex = orc.potential_exact_f64(q, dims, d)
+        assert np.abs(got - ex).max() <= 2e-6 * np.abs(ex).max(), dims
+    q = np.zeros(5, F)
+    q[0], q[3] = 2e14, -1e14
+    phi = orc.calc_potential_exact_refgpu(q, (5, 1, 1), DX)
+    assert np.allclose(phi, orc.potential_exact_f64(q, (5, 1, 1), DX), rtol=2e-6)
+    one = np.zeros(24, F)
+    one[7] = 1e14
+    assert orc.calc_potential_exact_refgpu(one, (4, 3, 2), DX)[7] == 0.0

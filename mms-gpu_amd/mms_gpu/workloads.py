@@ -226,6 +226,57 @@ def single_domain_square(xdim, ydim, zdim):
     return layered_single_domain(mask, zdim)
 
 
+def reference_initial_condition(mask2, zdim, ic, seed, Ms=MS):
+    """The reference's initial magnetisation (main.cpp:167-227) on the 2D
+    material mask `mask2` (1 = magnetic, [ydim][xdim]): layers 1..zdim-2
+    carry material; in-plane angle phi = 0 (ic 0, single domain), the four
+    quadrant angles -pi/4, pi/4, 3pi/4, -3pi/4 around (xdim/2, ydim/2) (ic 1,
+    vortex), or frand_atob(0, 360) degrees from the C library's
+    srand(seed)/rand() (ic 2, random; numerics.cu:125-130), drawn cell by
+    cell in z, y, x order over the magnetic cells.  The unit vector is
+    formed in double from the float angles and rounded to float, then
+    scaled by Ms in float.  Returns (material [n] uint8, M [3n] float32)."""
+    import ctypes
+    libc = ctypes.CDLL("libc.so.6")
+    libc.rand.restype = ctypes.c_int
+    libc.srand.argtypes = [ctypes.c_uint]
+    RAND_MAX = 2147483647
+    if ic == 2:
+        libc.srand(seed)
+    ydim, xdim = mask2.shape
+    mat = np.zeros((zdim, ydim, xdim), np.uint8)
+    M = np.zeros((zdim, ydim, xdim, 3), np.float32)
+    theta = np.float32(np.pi / 2)
+    for z in range(1, zdim - 1):
+        for y in range(ydim):
+            for x in range(xdim):
+                if not mask2[y, x]:
+                    continue
+                if ic == 0:
+                    phi = np.float32(0.0)
+                elif ic == 2:
+                    r = libc.rand() / float(RAND_MAX)
+                    r = 0.0 + float(np.float32(360.0) - np.float32(0.0)) * r
+                    phi = np.float32(float(np.float32(r)) * np.pi / 180)
+                elif ic == 1:
+                    a, b = x - xdim // 2, y - ydim // 2
+                    if a > 0 and b > 0:
+                        phi = np.float32(-np.pi / 4)
+                    elif a <= 0 and b > 0:
+                        phi = np.float32(np.pi / 4)
+                    elif a <= 0 and b <= 0:
+                        phi = np.float32(3 * np.pi / 4)
+                    else:
+                        phi = np.float32(-3 * np.pi / 4)
+                else:
+                    raise ValueError("unknown initial condition")
+                u = np.array([np.sin(float(theta)) * np.cos(float(phi)), np.sin(float(theta)) * np.sin(float(phi)),
+                              np.cos(float(theta))], np.float32)
+                M[z, y, x] = np.float32(Ms) * u
+                mat[z, y, x] = 1
+    return mat.ravel(), M.ravel()
+
+
 def layered_single_domain(mask2, zdim):
     ydim, xdim = mask2.shape
     mat = np.zeros((zdim, ydim, xdim), np.uint8)

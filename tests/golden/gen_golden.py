@@ -117,6 +117,16 @@ def gen_logs():
         "square32_variable": ("sim_square_variablestep_32x32x3_5nm.log", 1,
                               dict(xdim=32, ydim=32, zdim=3, width=1.6e-7, height=1.6e-7, depth=1.5e-8,
                                    mask="border", timestep=1e-13)),
+        # built-in border mask, vortex (IC 1) and random (IC 2, srand(2010)) initial states
+        "square32_vortex": ("sim_square_variablestep_32x32x3_5nm_ICvortex.log", 3,
+                            dict(xdim=32, ydim=32, zdim=3, width=1.6e-7, height=1.6e-7, depth=1.5e-8,
+                                 mask="border", timestep=1e-14, ic=1, seed=2010)),
+        "square32_random": ("sim_square_variablestep_32x32x3_5nm_ICrandom.log", 3,
+                            dict(xdim=32, ydim=32, zdim=3, width=1.6e-7, height=1.6e-7, depth=1.5e-8,
+                                 mask="border", timestep=1e-14, ic=2, seed=2010)),
+        "square16_random": ("sim_square_variablestep_16x16x3_5nm_ICrandom.log", 3,
+                            dict(xdim=16, ydim=16, zdim=3, width=1.6e-7, height=1.6e-7, depth=1.5e-8,
+                                 mask="border", timestep=1e-14, ic=2, seed=2010)),
     }
     for key, (fname, nkeep, meta) in cases.items():
         recs = _log_records(os.path.join(REF, "log-files", fname))

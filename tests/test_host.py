@@ -126,3 +126,77 @@ def test_png_mask_reader(tmp_path, filt):
 def test_nanowire4_fixture_mask():
     mask = np.loadtxt(os.path.join(ROOT, "tests", "golden", "nanowire4_mask.txt"), dtype=np.uint8)
     assert mask.shape == (26, 42) and mask.sum() == 360
+
+
+# ---------------------------------------------------------- command line --
+CLI = os.path.join(ROOT, "mms-gpu_amd", "mmsgpu")
+
+
+def _cli_files(tmp_path, args):
+    """Runs the command-line driver with a negative final time: M.dat and
+    material.dat are written before the time loop (which needs a GPU; without
+    one the driver exits with an error after writing them)."""
+    import subprocess
+    assert os.path.exists(CLI), "mms-gpu_amd/mmsgpu is not built"
+    subprocess.run([CLI] + args, cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    sim = tmp_path / args[15]
+    return open(sim / "M.dat").read().split("\n")[:-1], open(sim / "material.dat").read().split("\n")[:-1]
+
+
+def _vec_lines(M, dims):
+    xd, yd, zd = dims
+    v = M.reshape(zd, yd, xd, 3)
+    return ["%s %s %s " % tuple("%g" % float(c) for c in v[z, y, x])
+            for z in range(zd) for x in range(xd) for y in range(yd)]
+
+
+def _mat_lines(mat, dims):
+    xd, yd, zd = dims
+    m = mat.reshape(zd, yd, xd)[1]
+    return ["".join("%g " % float(m[y, x]) for x in range(xd)) for y in range(yd)]
+
+
+
+@pytest.mark.parametrize("ic", [0, 1, 2])
+def test_cli_front_end_built_in_mask(tmp_path, ic):
+    """main.cpp:113-245 without a mask image: border mask, layers 1..zdim-2,
+    the single-domain / vortex / srand(seed) random states (the last two pin
+    against the reference's logged runs through the oracle tests), written
+    as M.dat (save_vector3d order) and material.dat (layer 1)."""
+    from mms_gpu import workloads as wl
+    dims = (20, 12, 4)
+    args = ["dummy.png", "3", "-1", "1e-14", "1e-7", "6e-8", "2e-8", *map(str, dims), "1", "1", "0", "0", "1",
+            "sim", "2010", str(ic)]
+    Mlines, matlines = _cli_files(tmp_path, args)
+    mask = np.zeros(dims[1::-1], np.uint8)
+    mask[1:-1, 1:-1] = 1
+    mat, M = wl.reference_initial_condition(mask, dims[2], ic, 2010)
+    assert Mlines == _vec_lines(M, dims)
+    assert matlines == _mat_lines(mat, dims)
+
+
+def test_cli_front_end_png_mask(tmp_path):
+    """A PNG mask replaces xdim, ydim; grey == 0 is magnetic, the image's
+    bottom row is y = 0."""
+    import sys
+    sys.path.insert(0, os.path.dirname(__file__))
+    from mms_gpu import workloads as wl
+    from test_gpu_cli import _png_rgba
+    mask = np.loadtxt(os.path.join(ROOT, "tests", "golden", "nanowire4_mask.txt"), dtype=np.uint8)
+    (tmp_path / "nw.png").write_bytes(_png_rgba(mask))
+    Mlines, matlines = _cli_files(tmp_path, ["nw.png", "3", "-1", "1e-13", "2.1e-07", "1.3e-07", "1.5e-08", "8",
+                                             "8", "3", "1", "1", "0", "0", "1", "nw", "1", "0"])
+    mat, M = wl.layered_single_domain(mask, 3)
+    assert Mlines == _vec_lines(M, (42, 26, 3))
+    assert matlines == _mat_lines(mat, (42, 26, 3))
+
+
+def test_cli_rejects_bad_arguments(tmp_path):
+    import subprocess
+    assert os.path.exists(CLI)
+    r = subprocess.run([CLI, "none", "3", "1e-14", "1e-14", "1e-7", "1e-7", "1e-8", "8", "8", "3", "1", "1", "0",
+                        "0", "1", "x", "1", "7"], cwd=str(tmp_path), capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "Unknown Initial Condition" in r.stderr
+    (tmp_path / "bad.png").write_bytes(b"not a png")
+    r = subprocess.run([CLI, "bad.png"], cwd=str(tmp_path), capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "not a PNG" in r.stderr

@@ -101,6 +101,10 @@ def _kat_case(key, orc):
     if k["mask"] == "nanowire4":
         mask = np.loadtxt(os.path.join(GOLD, "nanowire4_mask.txt"), dtype=np.uint8)
         mat, M = wl.layered_single_domain(mask, zd)
+    elif "ic" in k:
+        mask = np.zeros((yd, xd), np.uint8)
+        mask[1:yd - 1, 1:xd - 1] = 1
+        mat, M = wl.reference_initial_condition(mask, zd, k["ic"], k["seed"])
     else:
         mat, M = wl.single_domain_square(xd, yd, zd)
     d = (np.float32(np.float32(k["width"]) / xd), np.float32(np.float32(k["height"]) / yd),
@@ -108,7 +112,9 @@ def _kat_case(key, orc):
     return k, mat, M, (xd, yd, zd), d
 
 
-def _step_records(orc, mat, M, dims, d, dt, nsteps, gamma):
+def _step_records(orc, mat, M, dims, d, dt, nsteps, gamma, adaptive=False):
+    """Accepted-step records of the reference loop; adaptive: the current
+    code's dt = 1e-14 / torque / 2 after each step (ode_functions.cpp:299)."""
     from mms_gpu import workloads as wl
     out = []
     H, _, _ = orc.hfield(M, dims, d, 3, wl.MU_0, wl.MS, wl.AEXCH)
@@ -119,19 +125,36 @@ def _step_records(orc, mat, M, dims, d, dt, nsteps, gamma):
         m = M2.reshape(-1, 3)[mat == 1].astype(np.float64) / float(wl.MS)
         out.append((E2, *m.mean(0), T))
         M = M2
+        if adaptive:
+            dt = np.float32(1e-14 / float(T) / 2)
     return out
 
 
-@pytest.mark.parametrize("key,nsteps", [("nanowire4_fixed_1e-13", 3), ("square32_variable", 1)])
+@pytest.mark.parametrize("key,nsteps", [("nanowire4_fixed_1e-13", 3), ("square32_variable", 1),
+                                        ("square32_vortex", 3), ("square32_random", 3), ("square16_random", 3)])
 def test_oracle_reproduces_logged_steps(orc, key, nsteps):
+    """The reference's own logged step records, reproduced from the same
+    geometry and initial state: the single-domain, the four-quadrant vortex
+    and the srand(2010) random initial conditions (main.cpp:144-227) through
+    Hfield / rk4_step / energy and torque."""
     k, mat, M, dims, d = _kat_case(key, orc)
-    recs = _step_records(orc, mat, M, dims, d, np.float32(k["timestep"]), nsteps, k["gamma"])
+    recs = _step_records(orc, mat, M, dims, d, np.float32(k["timestep"]), nsteps, k["gamma"],
+                         adaptive="ic" in k)
     for got, rec in zip(recs, k["records"]):
         E2, mx, my, mz, T = got
         _, _, _, _, E2r, mxr, myr, mzr, _, Tr = rec
-        assert abs(E2 - E2r) <= 1e-3, (E2, E2r)            # printed to 6 digits
-        assert abs(T - Tr) <= 2e-6 * abs(Tr) + 1e-7, (T, Tr)
+        assert _printed(E2, E2r), (E2, E2r)
+        assert _printed(T, Tr), (T, Tr)
         assert abs(mx - mxr) <= 2e-5
         if abs(myr) > 1e-6:                                  # deterministic asymmetry signals
             assert abs(my - myr) <= 1e-5 * abs(myr), (my, myr)
+        if abs(mzr) > 1e-6:                                  # below that <Mz> is FP32 rounding noise
             assert abs(mz - mzr) <= 1e-5 * abs(mzr), (mz, mzr)
+
+
+def _printed(x, ref):
+    """x agrees with the %g (6 significant digit) print `ref`: within half a
+    unit of the 6th digit, plus 4e-6 relative for the logged run's GPU
+    summation order against the CPU branch restated here."""
+    e = np.floor(np.log10(abs(ref)))
+    return abs(x - ref) <= 0.51 * 10.0 ** (e - 5) + 4e-6 * abs(ref)

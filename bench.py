@@ -57,7 +57,8 @@ def m2l_flops_per_eval(side, zd, P, w, m2l):
     KP = 32 * ((K + 31) // 32)
     Lleaf = int(math.log2(side)) - int(math.log2(w))
     blocks = sum((4 ** L) * zd * 27 * zd for L in range(2, Lleaf + 1))
-    products = {"bf16x3": 3, "bf16x3_f16near": 3, "fp16x2": 2, "fp16x2_bf16x3near": 2}.get(m2l, 1)
+    products = {"bf16x3": 3, "bf16x3_f16near": 3, "fp16x2": 2, "fp16x2_bf16x3near": 2,
+                "fp16x2_f16x2near": 2}.get(m2l, 1)
     return blocks * 2 * K * K, blocks * 2 * KP * KP * products
 
 
@@ -76,7 +77,8 @@ def p2p_flops_per_eval(side, zd, w, m2l):
     product in the FP16 form.  Returns (flops, peak TFLOP/s)."""
     cells = side * side * zd
     if w == 8 and m2l != "fp32":
-        return 2 * cells * zd * (3 * w) * (3 * w) * (1 if m2l in ("fp16", "bf16x3_f16near", "fp16x2") else 3), BF16_PEAK_TFLOPS
+        prods = {"fp16": 1, "bf16x3_f16near": 1, "fp16x2": 1, "fp16x2_f16x2near": 2}.get(m2l, 3)
+        return 2 * cells * zd * (3 * w) * (3 * w) * prods, BF16_PEAK_TFLOPS
     return 2 * cells * zd * 9 * w * w, FP32_PEAK_TFLOPS
 
 
@@ -175,26 +177,28 @@ def field_error(mg, f, w, nprobe=8192):
 
 
 # ---------------------------------------------------------------- main --
-PREC_NAMES = ("fp32", "bf16", "bf16x3", "fp16", "bf16x3_f16near", "fp16x2", "fp16x2_bf16x3near")
+PREC_NAMES = ("fp32", "bf16", "bf16x3", "fp16", "bf16x3_f16near", "fp16x2", "fp16x2_bf16x3near", "fp16x2_f16x2near")
 FORM_DTYPE = {"fp32": "f32",
               "bf16": "f32 (BF16 M2L operands, BF16x3 near field)",
               "bf16x3": "f32 (BF16x3 M2L and near field)",
               "fp16": "f32 (FP16 M2L and near field)",
               "bf16x3_f16near": "f32 (BF16x3 M2L, FP16 near field)",
               "fp16x2": "f32 (FP16x2 M2L, FP16 near field)",
-              "fp16x2_bf16x3near": "f32 (FP16x2 M2L, BF16x3 near field)"}
+              "fp16x2_bf16x3near": "f32 (FP16x2 M2L, BF16x3 near field)",
+              "fp16x2_f16x2near": "f32 (FP16x2 M2L, FP16x2 near field)"}
 # Forms inside the 1024^2 x 4 workload's precision split: FP32 everywhere
 # except 16-bit M2L inputs with FP32 accumulation, and a near field in FP32
-# or a hi/lo split of both operands (BF16x3).  The single-product FP16 near
-# field and the plain BF16 M2L are outside it (reported as extras only).
-CONFORMING = ("fp32", "bf16x3", "fp16x2_bf16x3near")
+# or a compensated hi/lo split (BF16x3: both operands; FP16x2: the 1/r
+# operator against scaled FP16 charges).  The single-product FP16 near field
+# and the plain BF16 M2L are outside it (reported as extras only).
+CONFORMING = ("fp32", "bf16x3", "fp16x2_bf16x3near", "fp16x2_f16x2near")
 RUN_CHECKPOINTS = (250, 500)  # steps of the C3 run at which the field-error gate is checked again
 
 
 def prec_of(mg, m2l):
     return {"fp32": mg.M2L_FP32, "bf16": mg.M2L_BF16, "bf16x3": mg.M2L_BF16X3, "fp16": mg.M2L_FP16,
             "bf16x3_f16near": mg.M2L_BF16X3_FP16NEAR, "fp16x2": mg.M2L_FP16X2,
-            "fp16x2_bf16x3near": mg.M2L_FP16X2_BF16X3NEAR}[m2l]
+            "fp16x2_bf16x3near": mg.M2L_FP16X2_BF16X3NEAR, "fp16x2_f16x2near": mg.M2L_FP16X2_FP16X2NEAR}[m2l]
 
 
 def make_film(mg, wl, w, P, m2l, leaf):

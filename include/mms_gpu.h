@@ -132,9 +132,12 @@ enum mms_m2l_precision {
     MMS_M2L_FP16X2 = 5, /* hi/lo FP16 split of the scaled operator against
                            the scaled FP16 source window, 2 MFMA products;
                            near field (leaf width 8) in scaled FP16       */
-    MMS_M2L_FP16X2_BF16X3NEAR = 6 /* FP16X2 M2L with the near field (leaf
+    MMS_M2L_FP16X2_BF16X3NEAR = 6, /* FP16X2 M2L with the near field (leaf
                            width 8) in the BF16 hi/lo split of operator and
                            charges, 3 MFMA products                      */
+    MMS_M2L_FP16X2_FP16X2NEAR = 7 /* FP16X2 M2L with the near field (leaf
+                           width 8) as FP16 hi/lo operator planes against
+                           the scaled FP16 charges, 2 MFMA products      */
 };
 enum mms_external {
     MMS_EXT_NONE = 0,

@@ -157,7 +157,7 @@ static int validate_params(const mms_params *p) {
     MMS_CHECK(p->dx > 0 && p->dy > 0 && p->dz > 0, "cell sizes must be positive");
     MMS_CHECK(p->solver == MMS_SOLVER_DIRECT || p->solver == MMS_SOLVER_FMM || p->solver == MMS_SOLVER_DIRECT_REFGPU,
               "unknown solver");
-    MMS_CHECK(p->m2l_precision >= MMS_M2L_FP32 && p->m2l_precision <= MMS_M2L_FP16X2_BF16X3NEAR, "unknown M2L precision");
+    MMS_CHECK(p->m2l_precision >= MMS_M2L_FP32 && p->m2l_precision <= MMS_M2L_FP16X2_FP16X2NEAR, "unknown M2L precision");
     return 0;
 }
 

@@ -1140,10 +1140,15 @@ __global__ __launch_bounds__(256) void p2p_mfma_kernel(const float *__restrict__
 // the FP16 conversion and the sums by 1 / s afterwards (exact), the 1/r
 // operator (values in (0, 1]) is stored as plain FP16.
 // The downward pass adds the far field afterwards.
-template <int NCB, bool F16>
+// NF: 0 = BF16 hi/lo split of operator and charges (3 products), 1 = scaled
+// FP16 operator and charges (1 product), 2 = FP16 hi/lo split of the operator
+// against the scaled FP16 charges (2 products).
+template <int NCB, int NF>
 __global__ __launch_bounds__(256) void p2p_16bit_kernel(const float *__restrict__ rho, float *__restrict__ phi,
                                                         const unsigned short *__restrict__ afrag, int xd, int yd,
                                                         int zd, float scale, FilmSet fs) {
+    constexpr bool F16 = NF != 0;          // scaled FP16 charge window (one plane)
+    constexpr bool OP_LO = NF != 1;        // operator lo plane read
     // block = (4 NCB) x 4 boxes; NCB column tiles of 16 boxes share each operator load
     constexpr int W = 8, TWX = 32 * NCB, TWY = 32, SWX = TWX + 16, SWY = 48, SP = SWX + 8;
     if (fs.desc) {
@@ -1266,7 +1271,7 @@ __global__ __launch_bounds__(256) void p2p_16bit_kernel(const float *__restrict_
             const unsigned short *Ab = afrag + (long)dzi * 42 * 2 * 64 * 8 + lane * 8;
             auto loadA = [&](BV &h, BV &l, int op) {
                 h = *reinterpret_cast<const BV *>(Ab + (long)(op * 2) * 64 * 8);
-                if constexpr (!F16) l = *reinterpret_cast<const BV *>(Ab + (long)(op * 2 + 1) * 64 * 8);
+                if constexpr (OP_LO) l = *reinterpret_cast<const BV *>(Ab + (long)(op * 2 + 1) * 64 * 8);
             };
             const int bbase = (zs * SWY + W * jy) * SP;  // row 0 of this lane's box window in layer zs
 #pragma unroll
@@ -1298,6 +1303,8 @@ __global__ __launch_bounds__(256) void p2p_16bit_kernel(const float *__restrict_
                             accz[p][c] = mfma32(Ah[p], bh[c], accz[p][c]);
                             if constexpr (!F16) {
                                 accz[p][c] = mfma32(Ah[p], bl[c], accz[p][c]);
+                                accz[p][c] = mfma32(Al[p], bh[c], accz[p][c]);
+                            } else if constexpr (NF == 2) {
                                 accz[p][c] = mfma32(Al[p], bh[c], accz[p][c]);
                             }
                         }
@@ -1413,9 +1420,10 @@ int launch_l2l_sum(const FmmPlan &pl, int L, hipStream_t s) {
 // M2L 34, 40, 56 and 80 KB gave 1451, 1452, 1380 and 1281 steps/s).
 size_t near_lds_shared(const FmmPlan &pl) { return (pl.near_f16 ? 40 : 82) * 1024; }
 
-template <int NCB, bool F16>
+template <int NCB, int NF>
 int launch_p2p_16bit(const Grid &g, const FmmPlan &pl, const float *rho, float *phi, float scale, bool shared,
                      hipStream_t s) {
+    constexpr bool F16 = NF != 0;
     constexpr int SP = 32 * NCB + 16 + 8;
     size_t lds = sizeof(unsigned short) * (F16 ? 1 : 2) * (size_t)g.zd * 48 * SP;
     if (shared && lds < near_lds_shared(pl)) lds = near_lds_shared(pl);
@@ -1423,12 +1431,12 @@ int launch_p2p_16bit(const Grid &g, const FmmPlan &pl, const float *rho, float *
     MMS_CHECK(lds <= (size_t)kMaxDyn, "FMM near field: too many layers for the LDS tile");
     static int attr = 0;
     if (!attr) {
-        MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&p2p_16bit_kernel<NCB, F16>),
+        MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&p2p_16bit_kernel<NCB, NF>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDyn));
         attr = 1;
     }
     dim3 grid((g.xd + 32 * NCB - 1) / (32 * NCB), (g.yd + 31) / 32, pl.nfilm);
-    hipLaunchKernelGGL((p2p_16bit_kernel<NCB, F16>), grid, dim3(256), lds, s, rho, phi, pl.d_p2pA, g.xd, g.yd,
+    hipLaunchKernelGGL((p2p_16bit_kernel<NCB, NF>), grid, dim3(256), lds, s, rho, phi, pl.d_p2pA, g.xd, g.yd,
                        g.zd, scale, pl.films());
     MMS_HIP(hipGetLastError());
     return 0;
@@ -1441,8 +1449,11 @@ int launch_near(const Grid &g, const FmmPlan &pl, const float *rho, float *phi, 
                 hipStream_t s) {
     const int zd = g.zd;
     if constexpr (W == 8) {
-        if (pl.near_f16) return launch_p2p_16bit<2, true>(g, pl, rho, phi, scale, shared, s);
-        if (pl.precision != MMS_M2L_FP32) return launch_p2p_16bit<2, false>(g, pl, rho, phi, scale, shared, s);
+        if (pl.near_f16) {
+            if (pl.near_f16x2) return launch_p2p_16bit<2, 2>(g, pl, rho, phi, scale, shared, s);
+            return launch_p2p_16bit<2, 1>(g, pl, rho, phi, scale, shared, s);
+        }
+        if (pl.precision != MMS_M2L_FP32) return launch_p2p_16bit<2, 0>(g, pl, rho, phi, scale, shared, s);
     }
     if constexpr (W == 4 || W == 8) {
         constexpr int TW = 4 * W, SW = TW + 2 * W, TS = 4 * W - 1;
@@ -1663,12 +1674,14 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
     MMS_CHECK(P_ >= 1 && P_ <= 9, "FMM expansion order must be 1..9");
     MMS_CHECK(w_ == 2 || w_ == 4 || w_ == 8, "FMM leaf width must be 2, 4 or 8");
     MMS_CHECK(g.zd >= 1 && g.zd <= 16, "FMM supports 1..16 layers");
-    MMS_CHECK(precision_ >= MMS_M2L_FP32 && precision_ <= MMS_M2L_FP16X2_BF16X3NEAR, "unknown M2L precision");
+    MMS_CHECK(precision_ >= MMS_M2L_FP32 && precision_ <= MMS_M2L_FP16X2_FP16X2NEAR, "unknown M2L precision");
     P = P_; w = w_; precision = precision_; zd = g.zd;
-    m2l_mode = (precision == MMS_M2L_BF16X3_FP16NEAR)     ? MMS_M2L_BF16X3
-               : (precision == MMS_M2L_FP16X2_BF16X3NEAR) ? MMS_M2L_FP16X2
-                                                          : precision;
-    near_f16 = (precision == MMS_M2L_FP16 || precision == MMS_M2L_BF16X3_FP16NEAR || precision == MMS_M2L_FP16X2);
+    m2l_mode = (precision == MMS_M2L_BF16X3_FP16NEAR) ? MMS_M2L_BF16X3
+               : (precision == MMS_M2L_FP16X2_BF16X3NEAR || precision == MMS_M2L_FP16X2_FP16X2NEAR) ? MMS_M2L_FP16X2
+                                                                                                     : precision;
+    near_f16x2 = (precision == MMS_M2L_FP16X2_FP16X2NEAR);
+    near_f16 = (precision == MMS_M2L_FP16 || precision == MMS_M2L_BF16X3_FP16NEAR || precision == MMS_M2L_FP16X2 ||
+                near_f16x2);
     const bool f16_ops = (m2l_mode == MMS_M2L_FP16 || m2l_mode == MMS_M2L_FP16X2);  // equilibrated FP16 operator
     m2l_cols = (cols_ == 8) ? 8 : 4;
     K = fmm_ncoef(P);
@@ -1843,9 +1856,13 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
                             const int fl = 32 * ph + 8 * gg + t, row = fl / 24 + d - r + 7, col = fl % 24 - ix + 7;
                             const float v = tab[((size_t)dzi * ts + row) * ts + col];
                             const size_t o = (((size_t)((dzi * 3 + ph) * 14 + di) * 2) * 64 + lane) * 8 + t;
-                            if (near_f16) {  // plain FP16, lo plane unused
+                            if (near_f16) {  // FP16 (lo plane: v - hi in FP16 for the hi/lo split, else unused)
                                 const _Float16 hv = (_Float16)v;
                                 memcpy(&af[o], &hv, 2);
+                                if (near_f16x2) {
+                                    const _Float16 lv = (_Float16)(v - (float)hv);
+                                    memcpy(&af[o + 64 * 8], &lv, 2);
+                                }
                             } else {
                                 const unsigned short h = host_bf16(v);
                                 af[o] = h;

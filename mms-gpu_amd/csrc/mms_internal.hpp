@@ -99,6 +99,7 @@ struct FmmPlan {
     int precision = MMS_M2L_FP32;  // requested form (mms_m2l_precision)
     int m2l_mode = MMS_M2L_FP32;   // M2L operands: FP32, BF16, BF16X3 or FP16
     bool near_f16 = false;         // leaf-8 near field in scaled FP16 (else BF16 split / FP32)
+    bool near_f16x2 = false;       // ... with the FP16 hi/lo split of its operator (two products)
     int m2l_cols = 4;        // MFMA column tiles (16 targets each) per wave in M2L: 4 or 8
     // device buffers; box vectors have KP slots (K rounded up to 32, zero padded)
     float *mult = nullptr;   // all levels, [level][z][by][bx][KP]

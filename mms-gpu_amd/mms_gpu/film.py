@@ -6,7 +6,8 @@ import numpy as np
 from ._lib import MmsParams, check, lib
 
 SOLVER_DIRECT, SOLVER_FMM, SOLVER_DIRECT_REFGPU = 0, 1, 2
-M2L_FP32, M2L_BF16, M2L_BF16X3, M2L_FP16, M2L_BF16X3_FP16NEAR, M2L_FP16X2, M2L_FP16X2_BF16X3NEAR = 0, 1, 2, 3, 4, 5, 6
+(M2L_FP32, M2L_BF16, M2L_BF16X3, M2L_FP16, M2L_BF16X3_FP16NEAR, M2L_FP16X2, M2L_FP16X2_BF16X3NEAR,
+ M2L_FP16X2_FP16X2NEAR) = 0, 1, 2, 3, 4, 5, 6, 7
 EXT_NONE, EXT_RAMP, EXT_CONST = 0, 1, 2
 
 MU_0 = np.float32(4 * np.pi * 1e-7)

@@ -19,10 +19,11 @@ def _say(*a):
     print("[gpu-test]", *a, flush=True)
 
 
-@pytest.mark.parametrize("form", ["fp16x2_bf16x3near", "bf16x3", "fp32"])
+@pytest.mark.parametrize("form", ["fp16x2_f16x2near", "fp16x2_bf16x3near", "bf16x3", "fp32"])
 def test_c3_full_size_field_error_along_run(mg, form):
     from mms_gpu import workloads as wl
-    prec = {"fp32": mg.M2L_FP32, "bf16x3": mg.M2L_BF16X3, "fp16x2_bf16x3near": mg.M2L_FP16X2_BF16X3NEAR}[form]
+    prec = {"fp32": mg.M2L_FP32, "bf16x3": mg.M2L_BF16X3, "fp16x2_bf16x3near": mg.M2L_FP16X2_BF16X3NEAR,
+            "fp16x2_f16x2near": mg.M2L_FP16X2_FP16X2NEAR}[form]
     w = wl.c3_domains()
     assert w.dims == (1024, 1024, 4)
     _say("C3 1024x1024x4", form)

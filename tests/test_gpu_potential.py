@@ -180,7 +180,8 @@ def test_fmm_linearity_and_sign(mg):
 
 @pytest.mark.parametrize("leaf", [4, 8])
 @pytest.mark.parametrize("mode,tol", [("fp32", 2e-6), ("bf16x3", 2e-5), ("bf16", 3e-3), ("fp16", 8e-4),
-                                      ("bf16x3_f16near", 8e-4), ("fp16x2", 8e-4)])
+                                      ("bf16x3_f16near", 8e-4), ("fp16x2", 8e-4), ("fp16x2_bf16x3near", 8e-4),
+                                      ("fp16x2_f16x2near", 8e-4)])
 def test_fmm_m2l_precision_modes(mg, mode, tol, leaf):
     """M2L on the matrix cores in each operand precision vs the FP64 FMM
     oracle (same P, leaf width, tree): FP32 to rounding; hi/lo BF16 split
@@ -194,7 +195,8 @@ def test_fmm_m2l_precision_modes(mg, mode, tol, leaf):
     xd, zd, P = 128, 4, 6
     q = (rng.normal(size=(zd, xd, xd)) * 1e14).astype(F)
     prec = {"fp32": mg.M2L_FP32, "bf16": mg.M2L_BF16, "bf16x3": mg.M2L_BF16X3, "fp16": mg.M2L_FP16,
-            "bf16x3_f16near": mg.M2L_BF16X3_FP16NEAR, "fp16x2": mg.M2L_FP16X2}[mode]
+            "bf16x3_f16near": mg.M2L_BF16X3_FP16NEAR, "fp16x2": mg.M2L_FP16X2,
+            "fp16x2_bf16x3near": mg.M2L_FP16X2_BF16X3NEAR, "fp16x2_f16x2near": mg.M2L_FP16X2_FP16X2NEAR}[mode]
     with mg.Film(xd, xd, zd, *D5, solver=mg.SOLVER_FMM, P=P, m2l_precision=prec, leaf_width=leaf) as f:
         pot = f.potential(q.ravel())
     ref = fo.FmmOracle(xd, xd, zd, P, w=leaf).potential_units(q.astype(float)).ravel() * float(D5[0]) ** 2
@@ -204,7 +206,7 @@ def test_fmm_m2l_precision_modes(mg, mode, tol, leaf):
 
 
 @pytest.mark.parametrize("leaf,prec", [(8, "bf16x3"), (8, "fp16"), (8, "bf16x3_f16near"), (8, "fp16x2"),
-                                       (8, "fp32"), (4, "fp32")])
+                                       (8, "fp16x2_bf16x3near"), (8, "fp16x2_f16x2near"), (8, "fp32"), (4, "fp32")])
 def test_concurrent_near_field_matches_sequential(mg, leaf, prec):
     """The near field runs on a second stream beside the multipole passes
     (fork/join) except inside a profiling window, where every kernel runs in
@@ -213,7 +215,8 @@ def test_concurrent_near_field_matches_sequential(mg, leaf, prec):
     from mms_gpu import workloads as wl
     w = wl.c3_domains(side=128, zd=4, domain=16)
     p = {"fp32": mg.M2L_FP32, "bf16x3": mg.M2L_BF16X3, "fp16": mg.M2L_FP16,
-         "bf16x3_f16near": mg.M2L_BF16X3_FP16NEAR, "fp16x2": mg.M2L_FP16X2}[prec]
+         "bf16x3_f16near": mg.M2L_BF16X3_FP16NEAR, "fp16x2": mg.M2L_FP16X2,
+         "fp16x2_bf16x3near": mg.M2L_FP16X2_BF16X3NEAR, "fp16x2_f16x2near": mg.M2L_FP16X2_FP16X2NEAR}[prec]
     ch = (np.random.default_rng(3).normal(size=w.n) * 1e14).astype(F)
     with mg.Film(*w.dims, *w.d, solver=mg.SOLVER_FMM, P=6, m2l_precision=p, leaf_width=leaf,
                  material=w.material) as f:
@@ -274,7 +277,7 @@ def test_fmm_high_order_bf16_split(mg, P):
 
 
 @pytest.mark.parametrize("leaf", [4, 8])
-@pytest.mark.parametrize("form", ["fp16", "fp16x2"])
+@pytest.mark.parametrize("form", ["fp16", "fp16x2", "fp16x2_f16x2near"])
 def test_fp16_m2l_scaling_is_exact(mg, leaf, form):
     """The FP16 M2L forms (one operator plane, or its hi/lo split) scale
     operators and staged windows by powers of two only, and every other
@@ -283,7 +286,7 @@ def test_fp16_m2l_scaling_is_exact(mg, leaf, form):
     rng = np.random.default_rng(44)
     xd, zd = 96, 3
     q = (rng.normal(size=(zd, xd, xd)) * 1e14).astype(F)
-    prec = {"fp16": mg.M2L_FP16, "fp16x2": mg.M2L_FP16X2}[form]
+    prec = {"fp16": mg.M2L_FP16, "fp16x2": mg.M2L_FP16X2, "fp16x2_f16x2near": mg.M2L_FP16X2_FP16X2NEAR}[form]
     with mg.Film(xd, xd, zd, *D5, solver=mg.SOLVER_FMM, P=6, m2l_precision=prec, leaf_width=leaf) as f:
         p0 = f.potential(q.ravel())
         for e in (40, -40):

@@ -97,74 +97,82 @@ def pmc_traffic(kernel, workload):
 
 
 # ----------------------------------------------------------- CPU leg --
-def cpu_baseline(w, ch, budget_s=15.0):
-    """Reference tree-code (fmm_calc CPU branch, P = 4: the reference's
-    maximum order) restated in oracle/mms_oracle.c, single-threaded as in the
-    reference (its OpenMP pragmas are commented out there).  Timed on a
-    bounded sample: every `stride`-th box of one mid level for one charge
-    layer; the per-term cost is extrapolated with the exact term count of the
-    whole evaluation (all levels, boxes, layers) to steps/s (4 evaluations
-    per step)."""
+def host_cpu():
+    """CPU model, physical cores and the threads this process may use."""
+    model, phys = "unknown", set()
+    try:
+        cur = {}
+        for line in open("/proc/cpuinfo"):
+            if ":" not in line:
+                if "core id" in cur:
+                    phys.add((cur.get("physical id", "0"), cur["core id"]))
+                cur = {}
+                continue
+            k, v = (t.strip() for t in line.split(":", 1))
+            cur[k] = v
+            if k == "model name":
+                model = v
+        if "core id" in cur:
+            phys.add((cur.get("physical id", "0"), cur["core id"]))
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(avail, omp) if omp > 0 else avail
+    return model, (len(phys) or None), threads
+
+
+def cpu_baseline(w, ch, budget_s=20.0):
+    """The reference tree-code (fmm_calc CPU branch, P = 4: the reference's
+    maximum order) restated in oracle/mms_oracle.c -- the same body as the
+    oracle's full evaluation -- timed on one whole level of one charge layer
+    on this host's cores (the boxes of the level split over the threads), then
+    extrapolated with the exact M2P term count of a whole evaluation (every
+    level and layer) to steps/s (4 evaluations per step).  The level is the
+    finest above the leaves (most boxes to share among the threads) whose
+    estimated time fits the budget."""
     import oracle
     L = oracle.lib()
     side, zd, P = w.xdim, w.zdim, 4
     K = (P + 1) ** 2
     logN = int(math.log2(side))
     q = ch.reshape(zd, side, side)
-    # exact term counts per (level, box): M2P terms = K * zd * (valid interaction cells)
-    total_terms = 0.0
-    sample_level = max(2, logN - 4)
-    counts = {}
+    model, phys, threads = host_cpu()
+    # exact M2P term counts per level: sum over non-empty source boxes of
+    # (valid interaction-list cells) x zd target layers x K coefficients
+    level_terms = {}
     for lev in range(2, logN + 1):
         wl_ = 2 ** (logN - lev)
         nb = 2 ** lev
         nz = (np.abs(q).reshape(zd, nb, wl_, nb, wl_).sum(axis=(2, 4)) != 0)  # non-empty boxes per layer
         bx = np.arange(nb)
-        # number of valid interaction-list boxes per box (edge-clipped), x and y separable
-        def n_il_1d(b):
-            lo = np.maximum(2 * (b // 2) - 2, 0)
-            hi = np.minimum(2 * (b // 2) + 3, nb - 1)
-            return lo, hi
-        lo, hi = n_il_1d(bx)
+        lo = np.maximum(2 * (bx // 2) - 2, 0)
+        hi = np.minimum(2 * (bx // 2) + 3, nb - 1)
         span = hi - lo + 1
         nbr = np.minimum(bx + 1, nb - 1) - np.maximum(bx - 1, 0) + 1
-        # interaction boxes = (span_x * span_y) - (nbr_x * nbr_y)
         il = span[None, :] * span[:, None] - nbr[None, :] * nbr[:, None]
-        cells = il * wl_ * wl_
-        per_box = cells * zd * K
-        t = float((per_box[None] * nz).sum())
-        counts[lev] = (per_box, nz)
-        total_terms += t
-    # sample: every stride-th box of sample_level, layer 0
-    per_box, nz = counts[sample_level]
-    nb = 2 ** sample_level
-    per_box_flat = np.zeros(nb * nb)
-    # box index is Morton order; map (by, bx) -> index
-    def morton(x, y, lev):
-        idx = 0
-        for b in range(lev):
-            idx |= ((x >> b) & 1) << (2 * b) | ((y >> b) & 1) << (2 * b + 1)
-        return idx
-    for by in range(nb):
-        for bx_ in range(nb):
-            per_box_flat[morton(bx_, by, sample_level)] = per_box[by, bx_] * nz[0, by, bx_]
-    est_rate = 1.4e7  # rough terms/s used only to size the sample
-    stride = max(1, int(per_box_flat.sum() / (est_rate * budget_s)))
-    sample_terms = float(per_box_flat[::stride].sum())
+        per_box = il * wl_ * wl_ * zd * K
+        level_terms[lev] = [float((per_box * nz[z]).sum()) for z in range(zd)]
+    total_terms = sum(sum(v) for v in level_terms.values())
+    est_rate = 1.4e7 * threads  # rough terms/s, only to pick the level
+    cand = [lev for lev in range(2, logN) if level_terms[lev][0] / est_rate <= budget_s]
+    lev = cand[-1] if cand else 2
+    sample_terms = level_terms[lev][0]
     pot = np.zeros(ch.size, np.float32)
     t0 = time.perf_counter()
-    st = L.or_fmm_bfs_sample(ch, pot, side, side, zd, float(w.dx), float(w.dy), float(w.dz), P, 0,
-                             sample_level, stride)
+    st = L.or_fmm_bfs_sample(ch, pot, side, side, zd, float(w.dx), float(w.dy), float(w.dz), P, 0, lev, lev, 1,
+                             threads)
     dt = time.perf_counter() - t0
     if st != 0 or sample_terms == 0:
         return None
-    per_term = dt / sample_terms
-    t_eval = per_term * total_terms
-    return {"value": 1.0 / (4.0 * t_eval), "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": (f"reference tree-code (fmm_calc CPU branch, P=4) restated in oracle/mms_oracle.c, "
-                       f"single thread; timed every {stride}th box of level {sample_level} for charge "
-                       f"layer 0 ({sample_terms:.3g} M2P terms in {dt:.1f} s), extrapolated by exact term "
-                       f"count ({total_terms:.3g} terms per evaluation, 4 evaluations per step)"),
+    t_eval = dt / sample_terms * total_terms
+    return {"value": 1.0 / (4.0 * t_eval), "unit": "steps/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "physical_cores": phys,
+            "sample": (f"reference tree-code (fmm_calc CPU branch, P=4) restated in oracle/mms_oracle.c; one "
+                       f"whole level ({lev}, {4 ** lev} boxes) of charge layer 0 on {threads} threads "
+                       f"({sample_terms:.3g} M2P terms in {dt:.1f} s), extrapolated by exact term count "
+                       f"({total_terms:.3g} terms per evaluation over {logN - 1} levels and {zd} layers, "
+                       f"4 evaluations per step)"),
             "seconds_per_eval_extrapolated": t_eval}
 
 

@@ -51,7 +51,7 @@ def lib():
         _sig(L, "or_potential_probes_f64", None, [f32p, I, I, I, D, D, D, i64p, I, f64p])
         _sig(L, "or_tree_lists", None, [I, i32p, i32p])
         _sig(L, "or_fmm_calc", I, [f32p, f32p, I, I, I, F, F, F, I])
-        _sig(L, "or_fmm_bfs_sample", I, [f32p, f32p, I, I, I, F, F, F, I, I, I, I])
+        _sig(L, "or_fmm_bfs_sample", I, [f32p, f32p, I, I, I, F, F, F, I, I, I, I, I, I])
         _sig(L, "or_llg_mprime", None, [f32p, f32p, F, F, F, f32p])
         _sig(L, "or_hfield", I, [f32p, f32p, f32p, f32p, F, I, I, I, F, F, F, I, F, F, F,
                                  I, I, I, I, C.c_void_p])

@@ -405,91 +405,130 @@ static inline float box_centre(int limit, int level, int x) {
 /* non-empty box's multipole at every cell of its interaction list in every  */
 /* target layer (M2P); at the 1-cell leaves add the near-field P2P.          */
 /* ------------------------------------------------------------------------ */
-static void or_fmm_bfs(const float *charge, float *potential, int limit, int P,
-                       int xdim, int ydim, int zdim, float dx, float dy, float dz, int zc,
-                       cx *mpc) {
+/* one source box b of `level` (charge layer zc) added into potential */
+static void or_fmm_box(const float *charge, float *potential, int limit, int P, int xdim, int ydim, int zdim,
+                       float dx, float dy, float dz, int zc, int level, unsigned b, cx *mpc) {
     const int K = (P + 1) * (P + 1);
-    for (int level = 2; level <= limit; level++) {
-        unsigned nbox = 1u << (2 * level);
-        float width = (float)pow(2, limit - level);
-        for (unsigned b = 0; b < nbox; b++) {
-            int bx, by;
-            box_xy(level, b, &bx, &by);
-            float ncx = box_centre(limit, level, bx), ncy = box_centre(limit, level, by);
-            float q = 0;
-            memset(mpc, 0, sizeof(cx) * K);
-            float charge_found = 0;
-            int yy1 = (int)ceilf(ncy - width / 2), yy2 = (int)floorf(ncy + width / 2);
-            for (int yy = yy1; yy <= yy2; yy++) {
-                for (int xx = (int)ceilf(ncx - width / 2); xx <= (int)floorf(ncx + width / 2); xx++) {
-                    float dV = dx * dy * dz;
-                    q = charge[yy * xdim + xx] * dV;
-                    if (q != 0) {
-                        charge_found = 1;
-                        cx r_ = cx_make((xx - ncx) * dx, (yy - ncy) * dy);
-                        float rmag = sqrtf(r_.re * r_.re + r_.im * r_.im);
-                        float rang = atan2f(r_.im, r_.re);
-                        for (int l = 0; l <= P; l++)
-                            for (int m = -l; m <= l; m++) {
-                                cx sph = or_sph(l, m, (float)(M_PI / 2), rang);
-                                sph.im = -sph.im; /* conjugate */
-                                cx_scaleeq(&sph, (float)((double)q * pow((double)rmag, (double)l)));
-                                cx_addeq(&mpc[l * l + l + m], sph);
-                            }
+    float width = (float)pow(2, limit - level);
+    int bx, by;
+    box_xy(level, b, &bx, &by);
+    float ncx = box_centre(limit, level, bx), ncy = box_centre(limit, level, by);
+    float q = 0;
+    memset(mpc, 0, sizeof(cx) * K);
+    float charge_found = 0;
+    int yy1 = (int)ceilf(ncy - width / 2), yy2 = (int)floorf(ncy + width / 2);
+    for (int yy = yy1; yy <= yy2; yy++) {
+        for (int xx = (int)ceilf(ncx - width / 2); xx <= (int)floorf(ncx + width / 2); xx++) {
+            float dV = dx * dy * dz;
+            q = charge[yy * xdim + xx] * dV;
+            if (q != 0) {
+                charge_found = 1;
+                cx r_ = cx_make((xx - ncx) * dx, (yy - ncy) * dy);
+                float rmag = sqrtf(r_.re * r_.re + r_.im * r_.im);
+                float rang = atan2f(r_.im, r_.re);
+                for (int l = 0; l <= P; l++)
+                    for (int m = -l; m <= l; m++) {
+                        cx sph = or_sph(l, m, (float)(M_PI / 2), rang);
+                        sph.im = -sph.im; /* conjugate */
+                        cx_scaleeq(&sph, (float)((double)q * pow((double)rmag, (double)l)));
+                        cx_addeq(&mpc[l * l + l + m], sph);
                     }
-                }
-            }
-            if (!charge_found) continue;
-
-            int il[27];
-            box_interactions(level, bx, by, il);
-            for (int i = 0; i < 27; i++) {
-                if (il[i] < 0) continue;
-                int ix, iy;
-                box_xy(level, (unsigned)il[i], &ix, &iy);
-                float icx = box_centre(limit, level, ix), icy = box_centre(limit, level, iy);
-                for (int yy = (int)ceilf(icy - width / 2); yy <= (int)floorf(icy + width / 2); yy++)
-                    for (int xx = (int)ceilf(icx - width / 2); xx <= (int)floorf(icx + width / 2); xx++)
-                        for (int zp = 0; zp < zdim; zp++) {
-                            v3 r = v3_make((xx - ncx) * dx, (yy - ncy) * dy, (zp - zc) * dz);
-                            float colat = v3_colat(r), azim = v3_azim(r), rmag = v3_mag(r);
-                            cx sum_lm = cx_make(0, 0);
-                            for (int l = 0; l <= P; l++) {
-                                cx sum_m = cx_make(0, 0);
-                                for (int m = -l; m <= l; m++) {
-                                    cx sph = or_sph(l, m, colat, azim);
-                                    int am = abs(m);
-                                    cx_scaleeq(&sph, (float)((1.0 * or_factorial(l - am)) / or_factorial(l + am)));
-                                    cx_muleq(&sph, mpc[l * l + l + m]);
-                                    cx_addeq(&sum_m, sph);
-                                }
-                                cx_scaleeq(&sum_m, (float)(1 / pow((double)rmag, (double)(l + 1))));
-                                cx_addeq(&sum_lm, sum_m);
-                            }
-                            potential[zp * ydim * xdim + yy * xdim + xx] += sum_lm.re;
-                        }
-            }
-
-            if (level == limit) {
-                int nb[8];
-                box_neighbors(level, bx, by, nb);
-                for (int zp = 0; zp < zdim; zp++) {
-                    if (zp != zc) {
-                        v3 r = v3_make(0, 0, (zp - zc) * dz);
-                        potential[zp * ydim * xdim + (int)(ncy * xdim + ncx)] += q / v3_mag(r);
-                    }
-                    for (int i = 0; i < 8; i++) {
-                        if (nb[i] < 0) continue;
-                        int ix, iy;
-                        box_xy(level, (unsigned)nb[i], &ix, &iy);
-                        float icx = box_centre(limit, level, ix), icy = box_centre(limit, level, iy);
-                        v3 r = v3_make((icx - ncx) * dx, (icy - ncy) * dy, (zp - zc) * dz);
-                        potential[zp * ydim * xdim + (int)(icy * xdim + icx)] += q / v3_mag(r);
-                    }
-                }
             }
         }
     }
+    if (!charge_found) return;
+
+    int il[27];
+    box_interactions(level, bx, by, il);
+    for (int i = 0; i < 27; i++) {
+        if (il[i] < 0) continue;
+        int ix, iy;
+        box_xy(level, (unsigned)il[i], &ix, &iy);
+        float icx = box_centre(limit, level, ix), icy = box_centre(limit, level, iy);
+        for (int yy = (int)ceilf(icy - width / 2); yy <= (int)floorf(icy + width / 2); yy++)
+            for (int xx = (int)ceilf(icx - width / 2); xx <= (int)floorf(icx + width / 2); xx++)
+                for (int zp = 0; zp < zdim; zp++) {
+                    v3 r = v3_make((xx - ncx) * dx, (yy - ncy) * dy, (zp - zc) * dz);
+                    float colat = v3_colat(r), azim = v3_azim(r), rmag = v3_mag(r);
+                    cx sum_lm = cx_make(0, 0);
+                    for (int l = 0; l <= P; l++) {
+                        cx sum_m = cx_make(0, 0);
+                        for (int m = -l; m <= l; m++) {
+                            cx sph = or_sph(l, m, colat, azim);
+                            int am = abs(m);
+                            cx_scaleeq(&sph, (float)((1.0 * or_factorial(l - am)) / or_factorial(l + am)));
+                            cx_muleq(&sph, mpc[l * l + l + m]);
+                            cx_addeq(&sum_m, sph);
+                        }
+                        cx_scaleeq(&sum_m, (float)(1 / pow((double)rmag, (double)(l + 1))));
+                        cx_addeq(&sum_lm, sum_m);
+                    }
+                    potential[zp * ydim * xdim + yy * xdim + xx] += sum_lm.re;
+                }
+    }
+
+    if (level == limit) {
+        int nb[8];
+        box_neighbors(level, bx, by, nb);
+        for (int zp = 0; zp < zdim; zp++) {
+            if (zp != zc) {
+                v3 r = v3_make(0, 0, (zp - zc) * dz);
+                potential[zp * ydim * xdim + (int)(ncy * xdim + ncx)] += q / v3_mag(r);
+            }
+            for (int i = 0; i < 8; i++) {
+                if (nb[i] < 0) continue;
+                int ix, iy;
+                box_xy(level, (unsigned)nb[i], &ix, &iy);
+                float icx = box_centre(limit, level, ix), icy = box_centre(limit, level, iy);
+                v3 r = v3_make((icx - ncx) * dx, (icy - ncy) * dy, (zp - zc) * dz);
+                potential[zp * ydim * xdim + (int)(icy * xdim + icx)] += q / v3_mag(r);
+            }
+        }
+    }
+}
+
+/* The BFS walk over levels lev_lo..lev_hi, every `stride`-th box, of charge
+ * layer zc.  nthreads == 1 keeps the reference's serial order; more threads
+ * split the boxes (each with its own potential buffer, added at the end),
+ * which changes only the order of the float additions (used to time it). */
+static int or_fmm_levels(const float *charge, float *potential, int limit, int P, int xdim, int ydim, int zdim,
+                         float dx, float dy, float dz, int zc, int lev_lo, int lev_hi, int stride, int nthreads) {
+    const int K = (P + 1) * (P + 1);
+    const size_t n = (size_t)zdim * ydim * xdim;
+    if (nthreads <= 1) {
+        cx *mpc = (cx *)malloc(sizeof(cx) * K);
+        if (!mpc) return 1;
+        for (int level = lev_lo; level <= lev_hi; level++) {
+            unsigned nbox = 1u << (2 * level);
+            for (unsigned b = 0; b < nbox; b += (unsigned)stride)
+                or_fmm_box(charge, potential, limit, P, xdim, ydim, zdim, dx, dy, dz, zc, level, b, mpc);
+        }
+        free(mpc);
+        return 0;
+    }
+    int fail = 0;
+#pragma omp parallel num_threads(nthreads)
+    {
+        cx *mpc = (cx *)malloc(sizeof(cx) * K);
+        float *pot = (float *)calloc(n, sizeof(float));
+        if (!mpc || !pot) {
+#pragma omp atomic write
+            fail = 1;
+        }
+        for (int level = lev_lo; level <= lev_hi; level++) {
+            long nbox = 1L << (2 * level);
+#pragma omp for schedule(dynamic, 1)
+            for (long b = 0; b < nbox; b += stride)
+                if (mpc && pot)
+                    or_fmm_box(charge, pot, limit, P, xdim, ydim, zdim, dx, dy, dz, zc, level, (unsigned)b, mpc);
+        }
+#pragma omp critical
+        if (pot)
+            for (size_t i = 0; i < n; i++) potential[i] += pot[i];
+        free(mpc);
+        free(pot);
+    }
+    return fail;
 }
 
 /* fmm_calc.cu:295-408.  Requires xdim == ydim == 2^logN (fmm_calc.cu:305).
@@ -498,84 +537,27 @@ int or_fmm_calc(const float *charge, float *potential, int xdim, int ydim, int z
                 float dx, float dy, float dz, int P) {
     unsigned logN = (unsigned)ceil(log2f((float)(xdim * ydim)) / log2f(4));
     if (xdim != ydim || (1 << logN) != xdim || P < 0 || P > 4) return 1;
-    cx *mpc = (cx *)malloc(sizeof(cx) * (P + 1) * (P + 1));
-    if (!mpc) return 1;
     for (int i = 0; i < zdim * ydim * xdim; i++) potential[i] = 0;
     for (int zc = 0; zc < zdim; zc++)
-        or_fmm_bfs(charge + zc * ydim * xdim, potential, (int)logN, P, xdim, ydim, zdim, dx, dy, dz, zc, mpc);
-    free(mpc);
+        if (or_fmm_levels(charge + zc * ydim * xdim, potential, (int)logN, P, xdim, ydim, zdim, dx, dy, dz, zc, 2,
+                          (int)logN, 1, 1))
+            return 1;
     return 0;
 }
 
-/* Per-level, per-source-box timing sample of the tree-code (one charge layer,
- * a stride of boxes per level) for the bounded CPU baseline.  Adds the
- * contributions of every `stride`-th box of layer zc into potential. */
+/* The same walk restricted to one charge layer zc, levels lev_lo..lev_hi and
+ * every `stride`-th box, on `nthreads` threads, added into potential: the
+ * bounded sample of the CPU baseline (stride 1 over every level and layer in
+ * one thread is or_fmm_calc exactly). */
 int or_fmm_bfs_sample(const float *charge, float *potential, int xdim, int ydim, int zdim,
-                      float dx, float dy, float dz, int P, int zc, int level, int stride) {
+                      float dx, float dy, float dz, int P, int zc, int lev_lo, int lev_hi, int stride,
+                      int nthreads) {
     unsigned logN = (unsigned)ceil(log2f((float)(xdim * ydim)) / log2f(4));
-    if (xdim != ydim || (1 << logN) != xdim || P < 0 || P > 4 || level < 2 || level > (int)logN) return 1;
-    /* reuse or_fmm_bfs's body through a one-level view: simplest is to run the
-     * level loop for one level and a box stride */
-    const int K = (P + 1) * (P + 1);
-    cx *mpc = (cx *)malloc(sizeof(cx) * K);
-    int limit = (int)logN;
-    const float *chg = charge + (size_t)zc * ydim * xdim;
-    unsigned nbox = 1u << (2 * level);
-    float width = (float)pow(2, limit - level);
-    for (unsigned b = 0; b < nbox; b += (unsigned)stride) {
-        int bx, by;
-        box_xy(level, b, &bx, &by);
-        float ncx = box_centre(limit, level, bx), ncy = box_centre(limit, level, by);
-        memset(mpc, 0, sizeof(cx) * K);
-        float charge_found = 0;
-        for (int yy = (int)ceilf(ncy - width / 2); yy <= (int)floorf(ncy + width / 2); yy++)
-            for (int xx = (int)ceilf(ncx - width / 2); xx <= (int)floorf(ncx + width / 2); xx++) {
-                float q = chg[yy * xdim + xx] * (dx * dy * dz);
-                if (q != 0) {
-                    charge_found = 1;
-                    float rmag = sqrtf(((xx - ncx) * dx) * ((xx - ncx) * dx) + ((yy - ncy) * dy) * ((yy - ncy) * dy));
-                    float rang = atan2f((yy - ncy) * dy, (xx - ncx) * dx);
-                    for (int l = 0; l <= P; l++)
-                        for (int m = -l; m <= l; m++) {
-                            cx sph = or_sph(l, m, (float)(M_PI / 2), rang);
-                            sph.im = -sph.im;
-                            cx_scaleeq(&sph, (float)((double)q * pow((double)rmag, (double)l)));
-                            cx_addeq(&mpc[l * l + l + m], sph);
-                        }
-                }
-            }
-        if (!charge_found) continue;
-        int il[27];
-        box_interactions(level, bx, by, il);
-        for (int i = 0; i < 27; i++) {
-            if (il[i] < 0) continue;
-            int ix, iy;
-            box_xy(level, (unsigned)il[i], &ix, &iy);
-            float icx = box_centre(limit, level, ix), icy = box_centre(limit, level, iy);
-            for (int yy = (int)ceilf(icy - width / 2); yy <= (int)floorf(icy + width / 2); yy++)
-                for (int xx = (int)ceilf(icx - width / 2); xx <= (int)floorf(icx + width / 2); xx++)
-                    for (int zp = 0; zp < zdim; zp++) {
-                        v3 r = v3_make((xx - ncx) * dx, (yy - ncy) * dy, (zp - zc) * dz);
-                        float colat = v3_colat(r), azim = v3_azim(r), rmag = v3_mag(r);
-                        cx sum_lm = cx_make(0, 0);
-                        for (int l = 0; l <= P; l++) {
-                            cx sum_m = cx_make(0, 0);
-                            for (int m = -l; m <= l; m++) {
-                                cx sph = or_sph(l, m, colat, azim);
-                                int am = abs(m);
-                                cx_scaleeq(&sph, (float)((1.0 * or_factorial(l - am)) / or_factorial(l + am)));
-                                cx_muleq(&sph, mpc[l * l + l + m]);
-                                cx_addeq(&sum_m, sph);
-                            }
-                            cx_scaleeq(&sum_m, (float)(1 / pow((double)rmag, (double)(l + 1))));
-                            cx_addeq(&sum_lm, sum_m);
-                        }
-                        potential[zp * ydim * xdim + yy * xdim + xx] += sum_lm.re;
-                    }
-        }
-    }
-    free(mpc);
-    return 0;
+    if (xdim != ydim || (1 << logN) != xdim || P < 0 || P > 4 || lev_lo < 2 || lev_hi > (int)logN ||
+        lev_lo > lev_hi || stride < 1 || zc < 0 || zc >= zdim)
+        return 1;
+    return or_fmm_levels(charge + (size_t)zc * ydim * xdim, potential, (int)logN, P, xdim, ydim, zdim, dx, dy, dz,
+                         zc, lev_lo, lev_hi, stride, nthreads);
 }
 
 /* ------------------------------------------------------------------------ */

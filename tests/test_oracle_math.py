@@ -250,3 +250,28 @@ def test_refgpu_order_accuracy_and_closed_forms(orc):
     one = np.zeros(24, F)
     one[7] = 1e14
     assert orc.calc_potential_exact_refgpu(one, (4, 3, 2), DX)[7] == 0.0
+
+
+def test_treecode_sample_is_the_full_walk(orc):
+    """The CPU baseline's timed sample and the full tree-code share one body:
+    stride 1 over every level of every charge layer in one thread adds up to
+    or_fmm_calc bit for bit; several threads change only the order of the
+    float additions."""
+    rng = np.random.default_rng(17)
+    dims, d, P = (32, 32, 3), DX, 3
+    q = (rng.normal(size=int(np.prod(dims))) * 1e14).astype(F)
+    full = orc.fmm_calc(q, dims, d, P)
+    L = orc.lib()
+    for nthreads, exact in ((1, True), (4, False)):
+        acc = np.zeros_like(q)
+        for zc in range(dims[2]):
+            assert L.or_fmm_bfs_sample(q, acc, *dims, *d, P, zc, 2, 5, 1, nthreads) == 0
+        if exact:
+            assert np.array_equal(acc, full)
+        else:
+            assert np.allclose(acc, full, rtol=1e-5, atol=1e-6 * np.abs(full).max())
+    # a stride samples every stride-th box of the requested levels only
+    part = np.zeros_like(q)
+    assert L.or_fmm_bfs_sample(q, part, *dims, *d, P, 0, 3, 3, 2, 1) == 0
+    assert np.any(part != 0) and not np.array_equal(part, full)
+    assert L.or_fmm_bfs_sample(q, part, *dims, *d, P, 0, 1, 3, 1, 1) == 1    # levels start at 2

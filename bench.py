@@ -62,6 +62,32 @@ def m2l_flops_per_eval(side, zd, P, w, m2l):
     return blocks * 2 * K * K, blocks * 2 * KP * KP * products
 
 
+def fmm_pass_bytes(side, zd, P, w):
+    """Algorithmic HBM bytes of the multipole passes of one evaluation, with
+    K = (P+1)(P+2)/2 useful coefficients per box and layer (the stored
+    vectors are padded to 32; the per-source-layer M2L partial sums are this
+    design's extra traffic, not algorithmic):
+      upward   -- read every charge once (4 B/cell), write the multipoles of
+                  the subtree levels R..leaf;
+      m2m      -- the coarse levels 2..R-1 from level R (read level R, write
+                  the coarser levels);
+      downward -- read one copy of the locals of every level, read and write
+                  the potential (8 B/cell) with L2P fused at the leaves.
+    R = leaf level - D, D = 3 (4 on trees deeper than 8 levels below the
+    root's subtrees), as the library splits the tree."""
+    K = (P + 1) * (P + 2) // 2
+    Lleaf = int(math.log2(side)) - int(math.log2(w))
+    D = 3 if Lleaf - 3 <= 5 else 4
+    if Lleaf - D < 2:
+        D = Lleaf - 2
+    R = Lleaf - D
+    box = lambda L: (4 ** L) * zd * K * 4
+    n = side * side * zd
+    return {"upward": 4 * n + sum(box(L) for L in range(R, Lleaf + 1)),
+            "m2m": sum(box(L) for L in range(2, R + 1)),
+            "downward": sum(box(L) for L in range(2, Lleaf + 1)) + 8 * n}
+
+
 def p2p_pairs_per_eval(side, zd, w):
     """Near-field pairs: each target against its 3x3 neighbour leaves in
     every layer (self excluded), edge leaves counted with their full window."""
@@ -249,6 +275,11 @@ def kernel_rooflines(prof, side, zd, P, leaf, m2l):
         peak = BF16_PEAK_TFLOPS if m2l != "fp32" else FP32_PEAK_TFLOPS
         kernels["m2l"] = dict(bound="mfma", achieved=issued / (ms * 1e-3) / 1e12, peak=peak, unit="TFLOP/s",
                               per_launch=issued, ms=ms, useful_tflops=useful / (ms * 1e-3) / 1e12)
+    for name, by in fmm_pass_bytes(side, zd, P, leaf).items():
+        if name in prof:
+            ms, _ = prof[name]
+            kernels[name] = dict(bound="hbm", achieved=by / (ms * 1e-3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                                 per_launch=by, ms=ms)
     if "p2p" in prof:
         ms, _ = prof["p2p"]
         fl, peak = p2p_flops_per_eval(side, zd, leaf, m2l)

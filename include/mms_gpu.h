@@ -216,6 +216,38 @@ double mms_profile_ms(mms_ctx *ctx, int i);
 long mms_profile_launches(mms_ctx *ctx, int i);
 
 /* ------------------------------------------------------------------------ */
+/* Part 2b: learned far-field surrogate (no counterpart in the reference)    */
+/* ------------------------------------------------------------------------ */
+/* A small convolutional neural operator maps each layer's charge density to
+ * its potential: 3x3 conv lift (charge, material mask, layer index -> 64
+ * channels), three 3x3 convs 64 -> 64 (the second with a spectral-mixing
+ * branch over 32 x 32 real Fourier modes), 1x1 projection; BF16 weights and
+ * activations, FP32 accumulation on the matrix cores.  It may stand in for
+ * the solver at RK4 stages 2-3 while the solver runs at stages 1 and 4.     */
+
+/* Number of FP32 weights mms_surrogate_init takes, in the order: lift
+ * [64][3][3][3] + bias[64]; conv1, conv2 [64][64][3][3] + bias[64] each;
+ * mode mixing [32 ky][32 kx][64 co][64 ci]; conv3 [64][64][3][3] + bias[64];
+ * projection [64] + bias. */
+long mms_surrogate_weight_count(void);
+/* Load the network (weights NULL: a seeded untrained initialisation).  The
+ * network sees charge / charge_scale and outputs potential / potential_scale
+ * (<= 0: the film's natural units Ms/dx and Ms dx). */
+int mms_surrogate_init(mms_ctx *ctx, const float *weights, float charge_scale, float potential_scale);
+/* One inference: host arrays (synchronous) or device arrays (enqueued). */
+int mms_surrogate_potential(mms_ctx *ctx, const float *charge, float *potential);
+int mms_surrogate_potential_dev(mms_ctx *ctx, const float *rho_dev, float *phi_dev);
+/* Use in mms_steps: mode 0 off; 1 the surrogate at RK4 stages 2-3 of every
+ * step (the stage-1 judge residual is still computed and kept on the device;
+ * step graphs allowed); 2 judged: at stage 1 the surrogate and the solver
+ * both run, and stages 2-3 take the surrogate only when the relative L2
+ * residual ||phi_sur - phi|| / ||phi|| <= tol, else the solver (decided on
+ * the host each step, no step graph). */
+int mms_surrogate_mode(mms_ctx *ctx, int mode, float tol);
+/* Last stage-1 residual (-1 if none), steps judged and accepted (mode 2). */
+int mms_surrogate_stats(mms_ctx *ctx, double *last_residual, long *judged, long *accepted);
+
+/* ------------------------------------------------------------------------ */
 /* Part 3: many independent films on one GPU (no counterpart in the          */
 /* reference, which runs one film per process)                              */
 /* ------------------------------------------------------------------------ */

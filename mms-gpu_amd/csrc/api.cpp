@@ -21,8 +21,9 @@ static thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
 
 const char *kclass_name(int k) {
-    static const char *names[K_COUNT] = {"div",      "stage",   "direct",  "direct_reduce", "upward", "m2m",
-                                         "m2l",      "downward",     "p2p",     "l2p", "hfield",  "energy",        "probes", "misc"};
+    static const char *names[K_COUNT] = {"div",      "stage",  "direct", "direct_reduce", "upward", "m2m",
+                                         "m2l",      "downward", "p2p",  "l2p",           "hfield", "energy",
+                                         "probes",   "misc",     "surrogate"};
     return (k >= 0 && k < K_COUNT) ? names[k] : "?";
 }
 
@@ -118,6 +119,9 @@ static void ctx_free(mms_ctx *c) {
         if (b) (void)hipFree(b);
     c->direct.release();
     c->fmm.release();
+    c->sur.release();
+    if (c->phi_s) (void)hipFree(c->phi_s);
+    c->phi_s = nullptr;
     if (c->stream) (void)hipStreamDestroy(c->stream);
     c->stream = nullptr;
 }
@@ -136,10 +140,33 @@ int mms_ctx::one_step(float dt) {
     const int a = cur, b = (cur + 1) % 3, d = (cur + 2) % 3;
     const float *in[4] = {m32[a], m32[b], m32[d], m32[b]};
     float *out[4] = {m32[b], m32[d], m32[b], m32[a]};
+    bool accept = (sur_mode == 1);
     for (int s = 0; s < 4; s++) {
         if (p.demag) {
             if (launch_div(g, in[s], rho, stream, &prof)) return 1;
-            if (potential(rho, phi)) return 1;
+            // the learned surrogate stands in at RK4 stages 2-3; the solver
+            // runs at stages 1 and 4 and, at stage 1, judges the surrogate
+            const bool use_sur = sur_mode != 0 && (s == 1 || s == 2) && accept;
+            if (use_sur) {
+                if (surrogate_potential(sur, rho, all_magnetic ? nullptr : matl, phi, sur_rscale, sur_oscale, stream,
+                                        &prof))
+                    return 1;
+            } else if (potential(rho, phi)) {
+                return 1;
+            }
+            if (sur_mode != 0 && s == 0) {
+                if (surrogate_potential(sur, rho, all_magnetic ? nullptr : matl, phi_s, sur_rscale, sur_oscale, stream,
+                                        &prof))
+                    return 1;
+                double r = 0.0;
+                if (surrogate_residual(sur, phi_s, phi, g.n(), sur_mode == 2 ? &r : nullptr, stream)) return 1;
+                if (sur_mode == 2) {
+                    sur_last = r;
+                    sur_judged++;
+                    accept = r <= sur_tol;
+                    if (accept) sur_accepted++;
+                }
+            }
         }
         // an all-magnetic film skips the per-cell material byte
         if (launch_stage(g, mat, s, (double)dt, phi, in[s], out[s], mn64, m32[a], acc32, all_magnetic ? nullptr : matl,
@@ -235,7 +262,9 @@ extern "C" int mms_set_time(mms_ctx *c, double t) {
 extern "C" int mms_steps(mms_ctx *c, int nsteps, float dt, int use_graph) {
     MMS_CHECK(c, "null context");
     MMS_CHECK(nsteps >= 0, "negative step count");
-    if (!use_graph || c->prof.on) {
+    // the judged surrogate decides on the host after stage 1 of every step:
+    // no step graph
+    if (!use_graph || c->prof.on || c->sur_mode == 2) {
         for (int i = 0; i < nsteps; i++)
             if (c->one_step(dt)) return 1;
         c->t_host += (double)nsteps * dt;
@@ -256,6 +285,64 @@ extern "C" int mms_steps(mms_ctx *c, int nsteps, float dt, int use_graph) {
     }
     for (int i = 0; i < nsteps; i++) MMS_HIP(hipGraphLaunch(c->graph, c->stream));
     c->t_host += (double)nsteps * dt;
+    return 0;
+}
+
+// ------------------------------------------------- learned surrogate --
+extern "C" long mms_surrogate_weight_count(void) { return surrogate_weight_count(); }
+
+extern "C" int mms_surrogate_init(mms_ctx *c, const float *weights, float charge_scale, float potential_scale) {
+    MMS_CHECK(c, "null context");
+    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    // natural units of the film: charge ~ Ms / dx, potential ~ Ms dx
+    c->sur_rscale = charge_scale > 0 ? 1.0f / charge_scale : c->g.dx / c->p.Ms;
+    c->sur_oscale = potential_scale > 0 ? potential_scale : c->p.Ms * c->g.dx;
+    if (c->sur.init(c->g, weights)) return 1;
+    if (!c->phi_s) MMS_HIP(hipMalloc(&c->phi_s, sizeof(float) * c->g.n()));
+    return 0;
+}
+
+extern "C" int mms_surrogate_potential_dev(mms_ctx *c, const float *rho_dev, float *phi_dev) {
+    MMS_CHECK(c && rho_dev && phi_dev, "null argument");
+    return surrogate_potential(c->sur, rho_dev, c->all_magnetic ? nullptr : c->matl, phi_dev, c->sur_rscale,
+                               c->sur_oscale, c->stream, &c->prof);
+}
+
+extern "C" int mms_surrogate_potential(mms_ctx *c, const float *charge, float *potential) {
+    MMS_CHECK(c && charge && potential, "null argument");
+    const long n = c->g.n();
+    MMS_HIP(hipMemcpyAsync(c->rho, charge, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    if (mms_surrogate_potential_dev(c, c->rho, c->phi)) return 1;
+    MMS_HIP(hipMemcpyAsync(potential, c->phi, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    MMS_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int mms_surrogate_mode(mms_ctx *c, int mode, float tol) {
+    MMS_CHECK(c, "null context");
+    MMS_CHECK(mode >= 0 && mode <= 2, "surrogate mode must be 0, 1 or 2");
+    MMS_CHECK(mode == 0 || c->sur.d_h0, "surrogate not initialised (mms_surrogate_init)");
+    if (c->graph) { (void)hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+    c->sur_mode = mode;
+    c->sur_tol = tol;
+    c->sur_judged = c->sur_accepted = 0;
+    c->sur_last = -1.0;
+    return 0;
+}
+
+extern "C" int mms_surrogate_stats(mms_ctx *c, double *last_residual, long *judged, long *accepted) {
+    MMS_CHECK(c, "null context");
+    if (c->sur_mode == 1 && c->sur.d_part) {  // the last step's stage-1 residual, left on the device
+        std::vector<double> h(2048);
+        MMS_HIP(hipMemcpyAsync(h.data(), c->sur.d_part, sizeof(double) * 2048, hipMemcpyDeviceToHost, c->stream));
+        MMS_HIP(hipStreamSynchronize(c->stream));
+        double d2 = 0, b2 = 0;
+        for (int i = 0; i < 1024; i++) { d2 += h[2 * i]; b2 += h[2 * i + 1]; }
+        c->sur_last = b2 > 0 ? sqrt(d2 / b2) : 0.0;
+    }
+    if (last_residual) *last_residual = c->sur_last;
+    if (judged) *judged = c->sur_judged;
+    if (accepted) *accepted = c->sur_accepted;
     return 0;
 }
 

@@ -43,7 +43,7 @@ struct Grid {
 // Kernel classes timed with hipEvents when profiling is on.
 enum KClass {
     K_DIV = 0, K_STAGE, K_DIRECT, K_DIRECT_REDUCE, K_P2M, K_M2M, K_M2L, K_L2L, K_P2P_L2P, K_L2P,
-    K_HFIELD, K_ENERGY, K_PROBES, K_MISC, K_COUNT
+    K_HFIELD, K_ENERGY, K_PROBES, K_MISC, K_SURROGATE, K_COUNT
 };
 const char *kclass_name(int k);
 
@@ -144,6 +144,28 @@ struct FmmHostOps {
 int fmm_build_ops(int P, int w, double ay, FmmHostOps &ops);
 int fmm_ncoef(int P);
 
+// ------------------------------------------------- learned surrogate --
+// Inference plan of the charge -> potential network (surrogate.hip): device
+// weights (BF16), Fourier bases, activation and transform buffers.
+struct SurrogatePlan {
+    Grid g{};
+    int nmode = 0;
+    unsigned short *d_lift = nullptr, *d_w1 = nullptr, *d_w2 = nullptr, *d_w3 = nullptr, *d_wm = nullptr;
+    unsigned short *d_bx = nullptr, *d_by = nullptr;
+    float *d_blift = nullptr, *d_b1 = nullptr, *d_b2 = nullptr, *d_b3 = nullptr, *d_wp = nullptr;
+    float bp = 0.0f;
+    unsigned short *d_h0 = nullptr, *d_h1 = nullptr, *d_spec = nullptr;  // NHWC BF16 activations
+    float *d_x1 = nullptr, *d_hm = nullptr, *d_gm = nullptr, *d_x2 = nullptr;
+    double *d_part = nullptr;
+    int init(const Grid &g, const float *weights);
+    void release();
+};
+long surrogate_weight_count();
+int surrogate_potential(SurrogatePlan &sp, const float *rho, const uint8_t *matl, float *phi, float rscale,
+                        float oscale, hipStream_t s, Profiler *prof);
+// relative L2 residual ||a - b|| / ||b|| (r null: partials only, left on the device)
+int surrogate_residual(SurrogatePlan &sp, const float *a, const float *b, long n, double *r, hipStream_t s);
+
 // -------------------------------------------------------- stencil / LLG --
 // Per-film record of a batch for the stencil/stage kernels (16-byte aligned).
 struct BatchFilm { int xd, yd, zd, pad; long off; float hx, hy, hz, pad2; };
@@ -225,6 +247,14 @@ struct mms_ctx {
     mms::DirectPlan direct;
     mms::FmmPlan fmm;
     mms::Profiler prof;
+    // learned surrogate: 0 off, 1 at RK4 stages 2-3 always, 2 at stages 2-3
+    // when its residual against the solver at stage 1 is within sur_tol
+    mms::SurrogatePlan sur;
+    int sur_mode = 0;
+    float sur_tol = 1e-3f, sur_rscale = 1.0f, sur_oscale = 1.0f;
+    float *phi_s = nullptr;       // surrogate potential of the stage-1 judge
+    long sur_judged = 0, sur_accepted = 0;
+    double sur_last = -1.0;
     // graph of one RK4 step
     hipGraphExec_t graph = nullptr;
     float graph_dt = 0.0f;

@@ -66,6 +66,12 @@ SIGNATURES = {
     "mms_profile_name": (C.c_char_p, [VP, I]),
     "mms_profile_ms": (D, [VP, I]),
     "mms_profile_launches": (C.c_long, [VP, I]),
+    "mms_surrogate_weight_count": (C.c_long, []),
+    "mms_surrogate_init": (I, [VP, VP, F, F]),
+    "mms_surrogate_potential": (I, [VP, f32p, f32p]),
+    "mms_surrogate_potential_dev": (I, [VP, VP, VP]),
+    "mms_surrogate_mode": (I, [VP, I, F]),
+    "mms_surrogate_stats": (I, [VP, C.POINTER(D), C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     "mms_batch_create": (I, [C.POINTER(VP), C.POINTER(MmsParams), I, VP, VP]),
     "mms_batch_destroy": (I, [VP]),
     "mms_batch_set_m": (I, [VP, f32p]),

@@ -75,6 +75,33 @@ class Film:
     def set_time(self, t):
         check(lib().mms_set_time(self._ctx, float(t)), "mms_set_time")
 
+    # --------------------------------------------- learned surrogate --
+    def surrogate_init(self, weights=None, charge_scale=0.0, potential_scale=0.0):
+        """Load the charge -> potential network (weights: FP32 vector of
+        mms_surrogate_weight_count() values, None = seeded untrained)."""
+        w = None
+        if weights is not None:
+            self._sur_w = np.ascontiguousarray(weights, dtype=np.float32).ravel()
+            if self._sur_w.size != lib().mms_surrogate_weight_count():
+                raise ValueError("surrogate weight vector has the wrong size")
+            w = self._sur_w.ctypes.data_as(C.c_void_p)
+        check(lib().mms_surrogate_init(self._ctx, w, float(charge_scale), float(potential_scale)),
+              "mms_surrogate_init")
+
+    def surrogate_potential(self, charge):
+        out = np.zeros(self.n, np.float32)
+        check(lib().mms_surrogate_potential(self._ctx, np.ascontiguousarray(charge, np.float32).ravel(), out),
+              "mms_surrogate_potential")
+        return out
+
+    def surrogate_mode(self, mode, tol=1e-3):
+        check(lib().mms_surrogate_mode(self._ctx, int(mode), float(tol)), "mms_surrogate_mode")
+
+    def surrogate_stats(self):
+        r, j, a = C.c_double(), C.c_long(), C.c_long()
+        check(lib().mms_surrogate_stats(self._ctx, C.byref(r), C.byref(j), C.byref(a)), "mms_surrogate_stats")
+        return {"last_residual": r.value, "judged": j.value, "accepted": a.value}
+
     # ------------------------------------------------------- stepping --
     def steps(self, nsteps, dt, use_graph=True):
         check(lib().mms_steps(self._ctx, int(nsteps), float(dt), int(bool(use_graph))), "mms_steps")

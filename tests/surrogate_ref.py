@@ -1,0 +1,79 @@
+"""A plain PyTorch fp32 restatement of the surrogate network (run on the CPU)
+for the GPU kernels' numerics tests.  Weights are rounded to BF16 and
+activations are rounded to BF16 where the device stores them; sums are
+FP32.  The weight vector has the documented order of
+mms_surrogate_weight_count (include/mms_gpu.h)."""
+import math
+
+import numpy as np
+import torch
+
+C = 64
+
+
+def _bf(t):
+    return t.to(torch.bfloat16).to(torch.float32)
+
+
+def gelu(x):
+    return 0.5 * x * (1.0 + torch.tanh(0.7978845608 * (x + 0.044715 * x * x * x)))
+
+
+def unpack(w):
+    w = torch.from_numpy(np.asarray(w, np.float32))
+    o = 0
+
+    def take(n, shape):
+        nonlocal o
+        t = w[o:o + n].reshape(shape)
+        o += n
+        return t
+    p = {}
+    p["lift"] = take(C * 27, (C, 3, 3, 3))
+    p["blift"] = take(C, (C,))
+    for k in ("1", "2"):
+        p["w" + k] = take(C * C * 9, (C, C, 3, 3))
+        p["b" + k] = take(C, (C,))
+    p["mode"] = take(32 * 32 * C * C, (32 * 32, C, C))
+    p["w3"] = take(C * C * 9, (C, C, 3, 3))
+    p["b3"] = take(C, (C,))
+    p["wp"] = take(C, (C,))
+    p["bp"] = take(1, (1,))
+    assert o == w.numel()
+    return p
+
+
+def basis(n):
+    x = np.arange(n, dtype=np.float64)
+    rows = []
+    for k in range(32):
+        f = (k + 1) // 2
+        a = 2 * math.pi * f * x / n
+        rows.append(np.ones(n) if k == 0 else (np.cos(a) if k % 2 else np.sin(a)))
+    return _bf(torch.from_numpy(np.sqrt(2.0 / n) * np.stack(rows)).to(torch.float32))
+
+
+def forward(weights, rho, mat, dims, rscale, oscale):
+    """rho [zd*yd*xd] float32, mat uint8 or None -> potential [zd*yd*xd]."""
+    xd, yd, zd = dims
+    p = unpack(weights)
+    r = torch.from_numpy(np.asarray(rho, np.float32).reshape(zd, yd, xd)) * np.float32(rscale)
+    m = (torch.ones(zd, yd, xd) if mat is None
+         else torch.from_numpy((np.asarray(mat).reshape(zd, yd, xd) != 0).astype(np.float32)))
+    lidx = torch.arange(zd, dtype=torch.float32).reshape(zd, 1, 1).expand(zd, yd, xd) / zd
+    x = _bf(torch.stack([r, m, lidx], 1))                                  # [zd, 3, yd, xd]
+    conv = torch.nn.functional.conv2d
+    h0 = _bf(gelu(conv(x, _bf(p["lift"]), p["blift"], padding=1)))
+    h1 = _bf(gelu(conv(h0, _bf(p["w1"]), p["b1"], padding=1)))
+    Bx, By = basis(xd), basis(yd)                                         # [32, n]
+    hl = h1.permute(0, 2, 3, 1)                                           # [z, y, x, c]
+    X1 = torch.einsum("kx,zyxc->zykc", Bx, hl)                             # FP32
+    H = torch.einsum("ly,zykc->zlkc", By, _bf(X1))                         # [z, ky, kx, c]
+    Wm = _bf(p["mode"]).reshape(32, 32, C, C)                              # [ky, kx, co, ci]
+    G = torch.einsum("lkoi,zlki->zlko", Wm, H)
+    X2 = torch.einsum("ly,zlko->zyko", By, _bf(G))
+    S = _bf(torch.einsum("kx,zyko->zyxo", Bx, _bf(X2))).permute(0, 3, 1, 2)  # [z, c, y, x]
+    h2 = _bf(gelu(conv(h1, _bf(p["w2"]), p["b2"], padding=1) + S))
+    h3 = _bf(gelu(conv(h2, _bf(p["w3"]), p["b3"], padding=1)))
+    out = (torch.einsum("zcyx,c->zyx", h3, p["wp"]) + p["bp"]) * np.float32(oscale)
+    return out.reshape(-1).numpy()

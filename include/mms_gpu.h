@@ -104,6 +104,19 @@ int time_marching(unsigned char *material, float *M, float finaltime, float time
                   int demag, int exchange, int external, int use_fmm, int use_gpu,
                   const char *sim_name, int verbosity);
 
+/* Direct-sum potential of device arrays that allocates nothing (no
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory -- code under stream/graph capture, tight serving loops).  Same
+ * kernels and chunking as calc_potential_exact, so the same results bit for
+ * bit; charge_dev / potential_dev are device arrays of xdim*ydim*zdim
+ * floats; enqueued on `stream` (a hipStream_t, NULL = the default stream),
+ * returns once enqueued.  The chunk partial sums use a fixed 16 MiB device
+ * scratch per GPU: grids needing more (above 2,097,152 cells) are rejected
+ * with an error.  Calls that overlap on one GPU share that scratch and race.
+ * Variant of calc_potential_exact (fmm_calc.cu:412-461). */
+int mms_potential_exact_noalloc(const float *charge_dev, float *potential_dev, int xdim, int ydim, int zdim,
+                                float dx, float dy, float dz, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Part 2: resident API (no counterpart in the reference: it replaces the     */
 /* reference's function-level statics with an explicit handle)              */

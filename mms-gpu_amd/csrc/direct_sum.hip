@@ -16,6 +16,8 @@
 // r^2 > 0.  1/r is the hardware reciprocal square root (v_rsq_f32): r^2 of
 // two distinct cells is at least min(1, ay^2, az^2) in cell units, never
 // denormal, so the denormal scaling of rsqrtf is not needed.
+#include <mutex>
+
 #include "mms_internal.hpp"
 
 namespace mms {
@@ -23,6 +25,11 @@ namespace {
 
 constexpr int TB = 256;    // targets per block (one per lane)
 constexpr int TILE = 256;  // sources per LDS tile
+
+// Chunk partial sums of the entry point that allocates nothing
+// (mms_potential_exact_noalloc): 16 MiB per GPU, held until the process ends.
+constexpr long kScratchDoubles = 2L * 1024 * 1024;
+static __device__ __align__(256) double scratch[kScratchDoubles];
 
 __global__ __launch_bounds__(TB) void direct_partial_kernel(const float *__restrict__ rho, double *__restrict__ part,
                                                             int xd, int yd, int zd, float ay, float az,
@@ -267,19 +274,25 @@ int direct_batch_potential(const float *rho, float *phi, const int2 *items, int 
     return 0;
 }
 
-int DirectPlan::init(const Grid &g, int split_) {
-    release();
-    split = (split_ == 2 || split_ == 4) ? split_ : 1;
-    const long n = g.n();
+// Source chunks of the direct sum for n cells: aim for >= 1024 blocks in
+// total (4 per CU) without chunks below one tile.
+static void direct_chunks(long n, int split, long *chunk, int *nchunk) {
     const long tblocks = (n + TB - 1) / TB;
-    // aim for >= 1024 blocks in total (4 per CU) without chunks below one tile
     long want = (1024L * split + tblocks - 1) / tblocks;
     long maxchunks = (n + TILE - 1) / TILE;
     if (want > maxchunks) want = maxchunks;
     if (want < 1) want = 1;
-    chunk = (n + want - 1) / want;
-    chunk = ((chunk + TILE - 1) / TILE) * TILE;
-    nchunk = (int)((n + chunk - 1) / chunk);
+    long c = (n + want - 1) / want;
+    c = ((c + TILE - 1) / TILE) * TILE;
+    *chunk = c;
+    *nchunk = (int)((n + c - 1) / c);
+}
+
+int DirectPlan::init(const Grid &g, int split_) {
+    release();
+    split = (split_ == 2 || split_ == 4) ? split_ : 1;
+    const long n = g.n();
+    direct_chunks(n, split, &chunk, &nchunk);
     MMS_HIP(hipMalloc(&part, sizeof(double) * (size_t)n * nchunk));
     return 0;
 }
@@ -290,24 +303,50 @@ void DirectPlan::release() {
     nchunk = 0;
 }
 
-int direct_potential(const Grid &g, DirectPlan &plan, const float *rho, float *phi, hipStream_t s,
-                     Profiler *prof) {
+// the two kernels of the direct sum with the chunk partials in `part`
+static int direct_launch(const Grid &g, long chunk, int nchunk, double *part, const float *rho, float *phi,
+                         hipStream_t s, Profiler *prof) {
     const long n = g.n();
-    MMS_CHECK(plan.part != nullptr, "direct solver plan not initialised");
     {
         KTimer kt(prof, K_DIRECT);
-        dim3 grid((unsigned)((n + TB - 1) / TB), (unsigned)plan.nchunk);
-        hipLaunchKernelGGL(direct_partial_kernel, grid, dim3(TB), 0, s, rho, plan.part, g.xd, g.yd, g.zd,
-                           g.ay, g.az, n, plan.chunk);
+        dim3 grid((unsigned)((n + TB - 1) / TB), (unsigned)nchunk);
+        hipLaunchKernelGGL(direct_partial_kernel, grid, dim3(TB), 0, s, rho, part, g.xd, g.yd, g.zd, g.ay, g.az, n,
+                           chunk);
         MMS_HIP(hipGetLastError());
     }
     {
         KTimer kt(prof, K_DIRECT_REDUCE);
         const double scale = (double)(g.dx * g.dy * g.dz) / (double)g.dx;
-        hipLaunchKernelGGL(direct_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, plan.part,
-                           phi, n, plan.nchunk, scale);
+        hipLaunchKernelGGL(direct_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, phi, n,
+                           nchunk, scale);
         MMS_HIP(hipGetLastError());
     }
+    return 0;
+}
+
+int direct_potential(const Grid &g, DirectPlan &plan, const float *rho, float *phi, hipStream_t s,
+                     Profiler *prof) {
+    MMS_CHECK(plan.part != nullptr, "direct solver plan not initialised");
+    return direct_launch(g, plan.chunk, plan.nchunk, plan.part, rho, phi, s, prof);
+}
+
+// Device address of this GPU's `scratch`, looked up once per device with
+// that device current (the host-side name of a __device__ array is not a
+// device address).
+static int scratch_ptr(double **out) {
+    static std::mutex mu;
+    static double *ptr[64] = {nullptr};
+    int dev = 0;
+    MMS_HIP(hipGetDevice(&dev));
+    MMS_CHECK(dev >= 0 && dev < 64, "device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        MMS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        MMS_CHECK(p != nullptr, "no device address for the direct-sum scratch");
+        ptr[dev] = static_cast<double *>(p);
+    }
+    *out = ptr[dev];
     return 0;
 }
 
@@ -343,3 +382,35 @@ int direct_probes_f64(const Grid &g, const float *rho, const int64_t *probes, in
 }
 
 }  // namespace mms
+
+// Direct-sum potential from device arrays without any allocation: the chunk
+// partial sums live in the file-scope `scratch` array of each GPU; the
+// kernels and the chunking are those of calc_potential_exact, so the results
+// are the same bit for bit.  Calls that overlap on one GPU share `scratch`
+// and race.  Every partial is written before it is read, so nothing has to
+// be cleared between calls.
+extern "C" int mms_potential_exact_noalloc(const float *charge_dev, float *potential_dev, int xdim, int ydim,
+                                           int zdim, float dx, float dy, float dz, void *stream) {
+    using namespace mms;
+    MMS_CHECK(charge_dev && potential_dev, "null device array");
+    MMS_CHECK(xdim > 0 && ydim > 0 && zdim > 0, "grid dimensions must be positive");
+    MMS_CHECK(dx > 0 && dy > 0 && dz > 0, "cell sizes must be positive");
+    Grid g;
+    g.xd = xdim; g.yd = ydim; g.zd = zdim;
+    g.dx = dx; g.dy = dy; g.dz = dz;
+    g.ay = dy / dx; g.az = dz / dx;
+    const long n = g.n();
+    long chunk = 0;
+    int nchunk = 0;
+    direct_chunks(n, 1, &chunk, &nchunk);
+    if (n * nchunk > kScratchDoubles) {
+        set_error("mms_potential_exact_noalloc: grid of " + std::to_string(n) + " cells needs " +
+                  std::to_string(n * nchunk) + " partial sums; the scratch holds " + std::to_string(kScratchDoubles) +
+                  " (at most 2,097,152 cells)");
+        return 1;
+    }
+    double *part = nullptr;
+    if (scratch_ptr(&part)) return 1;
+    return direct_launch(g, chunk, nchunk, part, charge_dev, potential_dev, static_cast<hipStream_t>(stream),
+                         nullptr);
+}

@@ -46,6 +46,7 @@ SIGNATURES = {
                      F, F, F, F, F, I, I, I, I, I, I, I]),
     "time_marching": (I, [u8p, f32p, F, F, I, I, I, F, F, F, I, F, F, F, F, F, I, I, I, I, I,
                           C.c_char_p, I]),
+    "mms_potential_exact_noalloc": (I, [VP, VP, I, I, I, F, F, F, VP]),
     "mms_create": (I, [C.POINTER(VP), C.POINTER(MmsParams), VP]),
     "mms_destroy": (I, [VP]),
     "mms_set_m": (I, [VP, f32p]),

@@ -41,8 +41,12 @@ __device__ __forceinline__ unsigned short f2bf(float x) {  // round to nearest e
     return (unsigned short)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
-__device__ __forceinline__ float gelu(float x) {  // tanh form
-    return 0.5f * x * (1.0f + tanhf(0.7978845608f * (x + 0.044715f * x * x * x)));
+// GELU, tanh form: 0.5 x (1 + tanh(u)) = x / (1 + exp(-2u)), u = sqrt(2/pi)
+// (x + 0.044715 x^3), with the hardware exp2 and reciprocal (within a few
+// ulp of the library tanh form; large |x| gives x or -0)
+__device__ __forceinline__ float gelu(float x) {
+    const float u2 = -1.5957691216f * (x + 0.044715f * x * x * x);
+    return x * __builtin_amdgcn_rcpf(1.0f + __expf(u2));
 }
 
 constexpr int C = 64;     // hidden channels
@@ -102,13 +106,18 @@ __global__ __launch_bounds__(256) void sur_lift_kernel(const float *__restrict__
 }
 
 // ------------------------------------------------------------- conv 3x3 --
-// 64 -> 64 channels on a 16 x 16 pixel tile: the 18 x 18 x 64 input window
-// and all nine 64 x 64 weight taps sit in LDS (row stride 72 bf16); wave w
-// computes D[64 co][64 pixels of tile rows 4w..4w+3] over 18 k steps (tap,
-// 32 input channels).  EPI 0: bias (+ add) + GELU, BF16 NHWC out; EPI 1:
-// bias + GELU, then the 1x1 projection to one channel (weights wp, bias bp),
-// times oscale, FP32 potential out.
-// w: [9 taps][64 co][64 ci] bf16.
+// 64 -> 64 channels on 16 x 16 pixel tiles.  A workgroup loads the nine
+// 64 x 64 weight taps into LDS once (row stride 72 bf16) and then walks
+// tiles blockIdx.x, blockIdx.x + gridDim.x, ... of layer blockIdx.z: the
+// next tile's 18 x 18 x 64 input window is loaded into registers while the
+// current one is multiplied, then stored to LDS.  Wave w computes D[64 co]
+// [64 pixels of tile rows 4w..4w+3] over 18 k steps (tap, 32 input
+// channels).  EPI 0: bias (+ add) + GELU, BF16 NHWC out; EPI 1: bias + GELU,
+// then the 1x1 projection to one channel (weights wp, bias bp), times
+// oscale, FP32 potential out.  w: [9 taps][64 co][64 ci] bf16.
+constexpr int WCH = HS * HS * 8;                 // 16-byte chunks of a window
+constexpr int WPT = (WCH + 255) / 256;           // per thread (11)
+
 template <int EPI>
 __global__ __launch_bounds__(256) void sur_conv_kernel(const unsigned short *__restrict__ in,
                                                        const unsigned short *__restrict__ w,
@@ -120,81 +129,108 @@ __global__ __launch_bounds__(256) void sur_conv_kernel(const unsigned short *__r
     extern __shared__ __attribute__((aligned(16))) unsigned short lds[];
     unsigned short *win = lds;                 // [HS * HS][CP]
     unsigned short *wt = lds + HS * HS * CP;   // [9 * 64][CP]
-    const int z = blockIdx.z, x0 = blockIdx.x * TS, y0 = blockIdx.y * TS;
-    // input window: 324 pixels x 8 chunks of 16 bytes
-    for (int e = threadIdx.x; e < HS * HS * 8; e += 256) {
-        const int p = e >> 3, ch = e & 7, sy = p / HS, sx = p % HS, gx = x0 - 1 + sx, gy = y0 - 1 + sy;
-        uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        if (gx >= 0 && gx < xd && gy >= 0 && gy < yd)
-            v = *reinterpret_cast<const uint4 *>(in + (((long)z * yd + gy) * xd + gx) * C + 8 * ch);
-        *reinterpret_cast<uint4 *>(win + p * CP + 8 * ch) = v;
+    const int z = blockIdx.z, ntx = (xd + TS - 1) / TS, nty = (yd + TS - 1) / TS, ntile = ntx * nty;
+    // the window chunks of tile t into registers (zero outside the film)
+    auto load_win = [&](int t, uint4 (&v)[WPT]) {
+        const int x0 = (t % ntx) * TS, y0 = (t / ntx) * TS;
+#pragma unroll
+        for (int q = 0; q < WPT; q++) {
+            const int e = threadIdx.x + 256 * q, p = e >> 3, ch = e & 7, sy = p / HS, sx = p % HS;
+            const int gx = x0 - 1 + sx, gy = y0 - 1 + sy;
+            v[q] = make_uint4(0u, 0u, 0u, 0u);
+            if (e < WCH && gx >= 0 && gx < xd && gy >= 0 && gy < yd)
+                v[q] = *reinterpret_cast<const uint4 *>(in + (((long)z * yd + gy) * xd + gx) * C + 8 * ch);
+        }
+    };
+    {   // weights: 18 chunks per thread, all loads in flight before the stores
+        constexpr int NW = 9 * C * 8 / 256;
+        uint4 v[NW];
+#pragma unroll
+        for (int q = 0; q < NW; q++) {
+            const int e = threadIdx.x + 256 * q, row = e >> 3, ch = e & 7;
+            v[q] = *reinterpret_cast<const uint4 *>(w + row * C + 8 * ch);
+        }
+#pragma unroll
+        for (int q = 0; q < NW; q++) {
+            const int e = threadIdx.x + 256 * q, row = e >> 3, ch = e & 7;
+            *reinterpret_cast<uint4 *>(wt + row * CP + 8 * ch) = v[q];
+        }
     }
-    for (int e = threadIdx.x; e < 9 * C * 8; e += 256) {
-        const int row = e >> 3, ch = e & 7;
-        *reinterpret_cast<uint4 *>(wt + row * CP + 8 * ch) = *reinterpret_cast<const uint4 *>(w + row * C + 8 * ch);
-    }
-    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j = lane & 15, g = lane >> 4;
-    f32x4 acc[4][4];
+    uint4 nxt[WPT];
+    int t = blockIdx.x;
+    if (t < ntile) load_win(t, nxt);
+    for (; t < ntile; t += gridDim.x) {
+        __syncthreads();  // the previous tile's products are done with win
 #pragma unroll
-    for (int r = 0; r < 4; r++)
+        for (int q = 0; q < WPT; q++) {
+            const int e = threadIdx.x + 256 * q;
+            if (e < WCH) *reinterpret_cast<uint4 *>(win + (e >> 3) * CP + 8 * (e & 7)) = nxt[q];
+        }
+        __syncthreads();
+        if (t + (int)gridDim.x < ntile) load_win(t + gridDim.x, nxt);  // in flight during the products
+        const int x0 = (t % ntx) * TS, y0 = (t / ntx) * TS;
+        f32x4 acc[4][4];
 #pragma unroll
-        for (int c = 0; c < 4; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 1
-    for (int tap = 0; tap < 9; tap++) {
-        const int dy = tap / 3, dx = tap % 3;
+        for (int tap = 0; tap < 9; tap++) {
+            const int dy = tap / 3, dx = tap % 3;
 #pragma unroll
-        for (int kb = 0; kb < 2; kb++) {
-            bf16x8 a[4], b[4];
+            for (int kb = 0; kb < 2; kb++) {
+                bf16x8 a[4], b[4];
 #pragma unroll
-            for (int r = 0; r < 4; r++)
-                a[r] = *reinterpret_cast<const bf16x8 *>(wt + (tap * C + 16 * r + j) * CP + 32 * kb + 8 * g);
+                for (int r = 0; r < 4; r++)
+                    a[r] = *reinterpret_cast<const bf16x8 *>(wt + (tap * C + 16 * r + j) * CP + 32 * kb + 8 * g);
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const int py = 4 * wave + c, px = j;
-                b[c] = *reinterpret_cast<const bf16x8 *>(win + ((py + dy) * HS + px + dx) * CP + 32 * kb + 8 * g);
-            }
+                for (int c = 0; c < 4; c++) {
+                    const int py = 4 * wave + c, px = j;
+                    b[c] = *reinterpret_cast<const bf16x8 *>(win + ((py + dy) * HS + px + dx) * CP + 32 * kb + 8 * g);
+                }
 #pragma unroll
-            for (int r = 0; r < 4; r++)
+                for (int r = 0; r < 4; r++)
 #pragma unroll
-                for (int c = 0; c < 4; c++) acc[r][c] = mfma_bf16(a[r], b[c], acc[r][c]);
-        }
-    }
-    // epilogue: lane (j, g) holds channels 16 r + 4 g + i of pixel (tile row 4 wave + c, column j)
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const int gx = x0 + j, gy = y0 + 4 * wave + c;
-        const bool ok = gx < xd && gy < yd;
-        const long pix = ((long)z * yd + gy) * xd + gx;
-        float proj = 0.0f;
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            float v[4];
-            uint2 ad = make_uint2(0u, 0u);
-            if (EPI == 0 && add && ok) ad = *reinterpret_cast<const uint2 *>(add + pix * C + 16 * r + 4 * g);
-            const unsigned short av[4] = {(unsigned short)(ad.x & 0xFFFF), (unsigned short)(ad.x >> 16),
-                                          (unsigned short)(ad.y & 0xFFFF), (unsigned short)(ad.y >> 16)};
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int co = 16 * r + 4 * g + i;
-                float t = acc[r][c][i] + bias[co];
-                if (EPI == 0) t += bf2f(av[i]);
-                v[i] = gelu(t);
-                if (EPI == 1) proj += bf2f(f2bf(v[i])) * wp[co];
-            }
-            if (EPI == 0 && ok) {
-                unsigned short s[4];
-#pragma unroll
-                for (int i = 0; i < 4; i++) s[i] = f2bf(v[i]);
-                *reinterpret_cast<uint2 *>(out + pix * C + 16 * r + 4 * g) =
-                    make_uint2(s[0] | (unsigned)s[1] << 16, s[2] | (unsigned)s[3] << 16);
+                    for (int c = 0; c < 4; c++) acc[r][c] = mfma_bf16(a[r], b[c], acc[r][c]);
             }
         }
-        if (EPI == 1) {
-            // the 64 channels of pixel j are spread over lanes j, j + 16, j + 32, j + 48
-            proj += __shfl_xor(proj, 16);
-            proj += __shfl_xor(proj, 32);
-            if (g == 0 && ok) phi[pix] = (proj + bp) * oscale;
+        // epilogue: lane (j, g) holds channels 16 r + 4 g + i of pixel (tile row 4 wave + c, column j)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int gx = x0 + j, gy = y0 + 4 * wave + c;
+            const bool ok = gx < xd && gy < yd;
+            const long pix = ((long)z * yd + gy) * xd + gx;
+            float proj = 0.0f;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                float v[4];
+                uint2 ad = make_uint2(0u, 0u);
+                if (EPI == 0 && add && ok) ad = *reinterpret_cast<const uint2 *>(add + pix * C + 16 * r + 4 * g);
+                const unsigned short av[4] = {(unsigned short)(ad.x & 0xFFFF), (unsigned short)(ad.x >> 16),
+                                              (unsigned short)(ad.y & 0xFFFF), (unsigned short)(ad.y >> 16)};
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int co = 16 * r + 4 * g + i;
+                    float tv = acc[r][c][i] + bias[co];
+                    if (EPI == 0) tv += bf2f(av[i]);
+                    v[i] = gelu(tv);
+                    if (EPI == 1) proj += bf2f(f2bf(v[i])) * wp[co];
+                }
+                if (EPI == 0 && ok) {
+                    unsigned short sv[4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) sv[i] = f2bf(v[i]);
+                    *reinterpret_cast<uint2 *>(out + pix * C + 16 * r + 4 * g) =
+                        make_uint2(sv[0] | (unsigned)sv[1] << 16, sv[2] | (unsigned)sv[3] << 16);
+                }
+            }
+            if (EPI == 1) {
+                // the 64 channels of pixel j are spread over lanes j, j + 16, j + 32, j + 48
+                proj += __shfl_xor(proj, 16);
+                proj += __shfl_xor(proj, 32);
+                if (g == 0 && ok) phi[pix] = (proj + bp) * oscale;
+            }
         }
     }
 }
@@ -466,10 +502,15 @@ int surrogate_potential(SurrogatePlan &sp, const float *rho, const uint8_t *matl
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)conv_lds));
         attr = 1;
     }
+    // convolutions: one resident workgroup per CU per layer slice walks the tiles
+    const int ntile = ((X + TS - 1) / TS) * ((Y + TS - 1) / TS);
+    int per_layer = (256 + Z - 1) / Z;
+    if (per_layer > ntile) per_layer = ntile;
+    dim3 cgrid(per_layer, 1, Z);
     KTimer kt(prof, K_SURROGATE);
     hipLaunchKernelGGL(sur_lift_kernel, grid, blk, 0, s, rho, matl, sp.d_lift, sp.d_blift, sp.d_h0, X, Y, Z, rscale);
     MMS_HIP(hipGetLastError());
-    hipLaunchKernelGGL((sur_conv_kernel<0>), grid, blk, conv_lds, s, sp.d_h0, sp.d_w1, sp.d_b1,
+    hipLaunchKernelGGL((sur_conv_kernel<0>), cgrid, blk, conv_lds, s, sp.d_h0, sp.d_w1, sp.d_b1,
                        (const unsigned short *)nullptr, sp.d_h1, (const float *)nullptr, 0.0f, (float *)nullptr, 0.0f,
                        X, Y);
     MMS_HIP(hipGetLastError());
@@ -508,10 +549,10 @@ int surrogate_potential(SurrogatePlan &sp, const float *rho, const uint8_t *matl
         a.M = X; a.N = C; a.K = 32;
         if (gemm<unsigned short, float, unsigned short>(a, Z * Y, s)) return 1;
     }
-    hipLaunchKernelGGL((sur_conv_kernel<0>), grid, blk, conv_lds, s, sp.d_h1, sp.d_w2, sp.d_b2, sp.d_spec, sp.d_h0,
+    hipLaunchKernelGGL((sur_conv_kernel<0>), cgrid, blk, conv_lds, s, sp.d_h1, sp.d_w2, sp.d_b2, sp.d_spec, sp.d_h0,
                        (const float *)nullptr, 0.0f, (float *)nullptr, 0.0f, X, Y);
     MMS_HIP(hipGetLastError());
-    hipLaunchKernelGGL((sur_conv_kernel<1>), grid, blk, conv_lds, s, sp.d_h0, sp.d_w3, sp.d_b3,
+    hipLaunchKernelGGL((sur_conv_kernel<1>), cgrid, blk, conv_lds, s, sp.d_h0, sp.d_w3, sp.d_b3,
                        (const unsigned short *)nullptr, (unsigned short *)nullptr, sp.d_wp, sp.bp, phi, oscale, X, Y);
     MMS_HIP(hipGetLastError());
     (void)npix;

@@ -232,11 +232,13 @@ long mms_profile_launches(mms_ctx *ctx, int i);
 /* Part 2b: learned far-field surrogate (no counterpart in the reference)    */
 /* ------------------------------------------------------------------------ */
 /* A small convolutional neural operator maps each layer's charge density to
- * its potential: 3x3 conv lift (charge, material mask, layer index -> 64
- * channels), three 3x3 convs 64 -> 64 (the second with a spectral-mixing
- * branch over 32 x 32 real Fourier modes), 1x1 projection; BF16 weights and
- * activations, FP32 accumulation on the matrix cores.  It may stand in for
- * the solver at RK4 stages 2-3 while the solver runs at stages 1 and 4.     */
+ * its potential: 3x3 conv lift (3 input planes: the layer's own charge, the
+ * summed charges of the layers at distance 1, the summed charges of the
+ * layers at distance >= 2 -> 64 channels), three 3x3 convs 64 -> 64 (the
+ * second with a spectral-mixing branch over 32 x 32 real Fourier modes), 1x1
+ * projection; BF16 weights and activations, FP32 accumulation on the matrix
+ * cores.  It may stand in for the solver at RK4 stages 2-3 while the solver
+ * runs at stages 1 and 4.                                                    */
 
 /* Number of FP32 weights mms_surrogate_init takes, in the order: lift
  * [64][3][3][3] + bias[64]; conv1, conv2 [64][64][3][3] + bias[64] each;
@@ -259,6 +261,46 @@ int mms_surrogate_potential_dev(mms_ctx *ctx, const float *rho_dev, float *phi_d
 int mms_surrogate_mode(mms_ctx *ctx, int mode, float tol);
 /* Last stage-1 residual (-1 if none), steps judged and accepted (mode 2). */
 int mms_surrogate_stats(mms_ctx *ctx, double *last_residual, long *judged, long *accepted);
+
+/* Training (FP32 parameters, BF16 matrix-core forward and backward passes,
+ * Adam with betas 0.9 / 0.999 and eps 1e-8).  The loss of a batch of films is
+ * the mean over its films of ||phi_sur - phi||^2 / ||phi||^2. */
+typedef struct mms_sur_trainer mms_sur_trainer;
+/* Film state `index` of a seeded family (SplitMix64 from seed and index): a
+ * disc, rectangle or ring mask (the same in every layer) holding a vortex
+ * (chirality, polarity +-1, core radius 2 cells) or uniform in-plane domains
+ * of 8, 16 or 32 cells, with 2% noise; |M| = Ms on the mask, 0 elsewhere.  M
+ * host AoS [n][3]; material (may be NULL) one byte per cell; kind (may be
+ * NULL) = 2 * shape (0 disc, 1 rectangle, 2 ring) + state (0 vortex, 1
+ * domains).  Host only. */
+int mms_surrogate_state(int xdim, int ydim, int zdim, float Ms, unsigned long long seed, long index, float *M,
+                        unsigned char *material, int *kind);
+/* A trainer for films of xdim x ydim x zdim cubic cells of side dx, batches
+ * of `batch` films, charge and potential in the natural units Ms / dx and
+ * Ms dx; weights NULL: the seeded initialisation of mms_surrogate_init. */
+int mms_surrogate_trainer_create(mms_sur_trainer **t, int xdim, int ydim, int zdim, float dx, float Ms, int batch,
+                                 const float *weights);
+int mms_surrogate_trainer_destroy(mms_sur_trainer *t);
+/* Append npairs training pairs: the states mms_surrogate_state(seed, i) for
+ * i = pairs already held, ..., their charge (the divergence of
+ * Hfield/rk4_step) and potential (FMM of order P; P <= 0: the direct sum),
+ * computed on the device. */
+int mms_surrogate_trainer_generate(mms_sur_trainer *t, int npairs, unsigned long long seed, int P);
+/* Append npairs pairs from host arrays [npairs][n]. */
+int mms_surrogate_trainer_add(mms_sur_trainer *t, int npairs, const float *charge, const float *potential);
+long mms_surrogate_trainer_pairs(mms_sur_trainer *t);
+int mms_surrogate_trainer_get_pair(mms_sur_trainer *t, long i, float *charge, float *potential);
+/* Gradient of the loss of the batch of pairs idx[0..batch) (API weight order,
+ * mms_surrogate_weight_count floats; may be NULL) and the loss, no update. */
+int mms_surrogate_trainer_gradient(mms_sur_trainer *t, const long *idx, float *grad, double *loss);
+/* One Adam step at rate lr on the batch idx; loss before the step. */
+int mms_surrogate_trainer_step(mms_sur_trainer *t, const long *idx, float lr, double *loss);
+/* Relative L2 residual ||phi_sur - phi|| / ||phi|| of each film of the batch idx. */
+int mms_surrogate_trainer_eval(mms_sur_trainer *t, const long *idx, double *residual);
+/* The network's potential of the batch idx, host [batch][n]. */
+int mms_surrogate_trainer_predict(mms_sur_trainer *t, const long *idx, float *potential);
+/* Current weights in API order (for mms_surrogate_init). */
+int mms_surrogate_trainer_weights(mms_sur_trainer *t, float *weights);
 
 /* ------------------------------------------------------------------------ */
 /* Part 3: many independent films on one GPU (no counterpart in the          */

@@ -148,16 +148,12 @@ int mms_ctx::one_step(float dt) {
             // runs at stages 1 and 4 and, at stage 1, judges the surrogate
             const bool use_sur = sur_mode != 0 && (s == 1 || s == 2) && accept;
             if (use_sur) {
-                if (surrogate_potential(sur, rho, all_magnetic ? nullptr : matl, phi, sur_rscale, sur_oscale, stream,
-                                        &prof))
-                    return 1;
+                if (surrogate_potential(sur, rho, phi, sur_rscale, sur_oscale, stream, &prof)) return 1;
             } else if (potential(rho, phi)) {
                 return 1;
             }
             if (sur_mode != 0 && s == 0) {
-                if (surrogate_potential(sur, rho, all_magnetic ? nullptr : matl, phi_s, sur_rscale, sur_oscale, stream,
-                                        &prof))
-                    return 1;
+                if (surrogate_potential(sur, rho, phi_s, sur_rscale, sur_oscale, stream, &prof)) return 1;
                 double r = 0.0;
                 if (surrogate_residual(sur, phi_s, phi, g.n(), sur_mode == 2 ? &r : nullptr, stream)) return 1;
                 if (sur_mode == 2) {
@@ -297,15 +293,14 @@ extern "C" int mms_surrogate_init(mms_ctx *c, const float *weights, float charge
     // natural units of the film: charge ~ Ms / dx, potential ~ Ms dx
     c->sur_rscale = charge_scale > 0 ? 1.0f / charge_scale : c->g.dx / c->p.Ms;
     c->sur_oscale = potential_scale > 0 ? potential_scale : c->p.Ms * c->g.dx;
-    if (c->sur.init(c->g, weights)) return 1;
+    if (c->sur.init(c->g, c->g.zd, weights, false, c->stream)) return 1;
     if (!c->phi_s) MMS_HIP(hipMalloc(&c->phi_s, sizeof(float) * c->g.n()));
     return 0;
 }
 
 extern "C" int mms_surrogate_potential_dev(mms_ctx *c, const float *rho_dev, float *phi_dev) {
     MMS_CHECK(c && rho_dev && phi_dev, "null argument");
-    return surrogate_potential(c->sur, rho_dev, c->all_magnetic ? nullptr : c->matl, phi_dev, c->sur_rscale,
-                               c->sur_oscale, c->stream, &c->prof);
+    return surrogate_potential(c->sur, rho_dev, phi_dev, c->sur_rscale, c->sur_oscale, c->stream, &c->prof);
 }
 
 extern "C" int mms_surrogate_potential(mms_ctx *c, const float *charge, float *potential) {

@@ -145,24 +145,49 @@ int fmm_build_ops(int P, int w, double ay, FmmHostOps &ops);
 int fmm_ncoef(int P);
 
 // ------------------------------------------------- learned surrogate --
-// Inference plan of the charge -> potential network (surrogate.hip): device
-// weights (BF16), Fourier bases, activation and transform buffers.
+// Plan of the charge -> potential network (surrogate.hip): FP32 parameters in
+// device layout with their BF16 copies, Fourier bases, activation and
+// transform buffers; with `train` also the saved pre-activations, gradient
+// buffers and the flipped, transposed conv taps of the backward pass.  The
+// grid's zd layers are zd / zf films of zf layers each.
 struct SurrogatePlan {
+    static constexpr int kPartBlocks = 256;  // workgroups of the weight-gradient kernels
     Grid g{};
+    int zf = 1;
     int nmode = 0;
-    unsigned short *d_lift = nullptr, *d_w1 = nullptr, *d_w2 = nullptr, *d_w3 = nullptr, *d_wm = nullptr;
-    unsigned short *d_bx = nullptr, *d_by = nullptr;
-    float *d_blift = nullptr, *d_b1 = nullptr, *d_b2 = nullptr, *d_b3 = nullptr, *d_wp = nullptr;
-    float bp = 0.0f;
-    unsigned short *d_h0 = nullptr, *d_h1 = nullptr, *d_spec = nullptr;  // NHWC BF16 activations
-    float *d_x1 = nullptr, *d_hm = nullptr, *d_gm = nullptr, *d_x2 = nullptr;
+    bool train = false;
+    float *d_param = nullptr;
+    unsigned short *d_lift = nullptr, *d_wm = nullptr, *d_bx = nullptr, *d_by = nullptr;
+    unsigned short *d_w[3] = {nullptr, nullptr, nullptr}, *d_wt[3] = {nullptr, nullptr, nullptr};
+    unsigned short *d_h0 = nullptr, *d_h1 = nullptr, *d_h2 = nullptr, *d_spec = nullptr;  // NHWC BF16
+    unsigned short *d_z[4] = {nullptr, nullptr, nullptr, nullptr};                       // pre-activations
+    unsigned short *d_dza = nullptr, *d_dzb = nullptr;                                     // gradients
+    float *d_x1 = nullptr, *d_hm = nullptr, *d_gm = nullptr, *d_x2 = nullptr, *d_dh = nullptr;
+    float *d_wpart = nullptr, *d_hpart = nullptr, *d_hsum = nullptr;
     double *d_part = nullptr;
-    int init(const Grid &g, const float *weights);
+    int init(const Grid &g, int zf, const float *weights, bool train, hipStream_t s);
+    int pack(hipStream_t s);
     void release();
 };
 long surrogate_weight_count();
-int surrogate_potential(SurrogatePlan &sp, const float *rho, const uint8_t *matl, float *phi, float rscale,
-                        float oscale, hipStream_t s, Profiler *prof);
+long surrogate_param_count();
+void surrogate_api_to_param(const float *api, float *param);
+void surrogate_param_to_api(const float *param, float *api);
+// forward pass (with sp.train the pre-activations are kept for surrogate_backward)
+int surrogate_potential(SurrogatePlan &sp, const float *rho, float *phi, float rscale, float oscale, hipStream_t s,
+                        Profiler *prof);
+// per-film losses sum (phi_s - phi_t)^2 inv_norm[f] into the head partials
+// (sp.d_hpart[f][64 blocks][66], entry 65); with grad also the projection's gradient
+int surrogate_losses(SurrogatePlan &sp, const float *phi_s, const float *phi_t, const float *inv_norm, float gscale,
+                     float oscale, bool grad, hipStream_t s);
+// gradient (device layout) of gscale / 2 * sum_f loss_f after surrogate_potential on the same rho;
+// sp.d_hsum[65] receives sum_f loss_f
+int surrogate_backward(SurrogatePlan &sp, const float *rho, const float *phi_s, const float *phi_t,
+                       const float *inv_norm, float gscale, float rscale, float oscale, float *grad, hipStream_t s);
+// Adam step `step` (from 1) at rate lr on sp.d_param with moments m, v; refreshes the BF16 copies
+int surrogate_adam(SurrogatePlan &sp, float *m, float *v, const float *grad, long step, float lr, hipStream_t s);
+// sum of x^2 over n values (synchronous)
+int surrogate_sqnorm(SurrogatePlan &sp, const float *x, long n, double *out, hipStream_t s);
 // relative L2 residual ||a - b|| / ||b|| (r null: partials only, left on the device)
 int surrogate_residual(SurrogatePlan &sp, const float *a, const float *b, long n, double *r, hipStream_t s);
 

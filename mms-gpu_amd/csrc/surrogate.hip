@@ -1,9 +1,12 @@
-// surrogate.hip -- inference of the learned far-field surrogate on gfx950.
+// surrogate.hip -- the learned far-field surrogate on gfx950: inference and
+// training kernels.
 //
-// The network maps the charge density of every film layer to its scalar
-// potential, layer by layer in the plane (the layer index is an input):
-//   lift      3x3 conv, 3 -> 64 channels (charge / charge scale, material
-//             mask, layer index / zd) + bias, GELU
+// The network maps a film's charge density to its scalar potential, layer by
+// layer in the plane; the layers see each other through the input planes:
+//   lift      3x3 conv, 3 -> 64 channels + bias, GELU; the input planes of
+//             layer z are its own charge, the summed charges of the layers at
+//             distance 1 and the summed charges of the layers at distance
+//             >= 2 of the same film, each times the charge scale
 //   conv1     3x3 conv 64 -> 64 + bias, GELU
 //   conv2     3x3 conv 64 -> 64 + bias + spectral branch, GELU, where the
 //             spectral branch of conv1's output h is
@@ -17,6 +20,15 @@
 // Activations are NHWC BF16 [z][y][x][64]; weights BF16; every product runs
 // on the matrix cores (v_mfma_f32_16x16x32_bf16) with FP32 accumulation.
 // Cells outside the film read as zero (zero padding).
+//
+// Training keeps the FP32 parameters and saves the BF16 pre-activations of
+// the forward pass.  The backward data pass of a 3x3 conv is the same conv
+// kernel with flipped, transposed weights W'[tap][ci][co] = W[8 - tap][co][ci]
+// and an epilogue that multiplies by GELU'(pre-activation); the weight
+// gradient dW[tap][co][ci] = sum_p dz[p][co] a[p + off(tap)][ci] is a GEMM
+// over pixels on the matrix cores; the spectral branch's backward pass is the
+// same transform chain with the transposed mode mixing; Adam updates the FP32
+// parameters, which are then rounded to the BF16 device copies.
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
@@ -41,12 +53,27 @@ __device__ __forceinline__ unsigned short f2bf(float x) {  // round to nearest e
     return (unsigned short)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
-// GELU, tanh form: 0.5 x (1 + tanh(u)) = x / (1 + exp(-2u)), u = sqrt(2/pi)
-// (x + 0.044715 x^3), with the hardware exp2 and reciprocal (within a few
-// ulp of the library tanh form; large |x| gives x or -0)
-__device__ __forceinline__ float gelu(float x) {
+// GELU, tanh form: 0.5 x (1 + tanh(u)) = x s, s = 1 / (1 + exp(-2u)), u =
+// sqrt(2/pi) (x + 0.044715 x^3), with the hardware exp2 and reciprocal
+// (within a few ulp of the library tanh form; large |x| gives x or -0)
+__device__ __forceinline__ float gelu_s(float x) {
     const float u2 = -1.5957691216f * (x + 0.044715f * x * x * x);
-    return x * __builtin_amdgcn_rcpf(1.0f + __expf(u2));
+    return __builtin_amdgcn_rcpf(1.0f + __expf(u2));
+}
+__device__ __forceinline__ float gelu(float x) { return x * gelu_s(x); }
+// d/dx of the same form: s + 2 x s (1 - s) sqrt(2/pi) (1 + 3 * 0.044715 x^2)
+__device__ __forceinline__ float gelu_grad(float x) {
+    const float s = gelu_s(x);
+    return s + 2.0f * x * s * (1.0f - s) * 0.7978845608f * (1.0f + 0.134145f * x * x);
+}
+__device__ __forceinline__ uint2 pack4(const float (&v)[4]) {
+    return make_uint2(f2bf(v[0]) | (unsigned)f2bf(v[1]) << 16, f2bf(v[2]) | (unsigned)f2bf(v[3]) << 16);
+}
+__device__ __forceinline__ void unpack4(uint2 u, float (&v)[4]) {
+    v[0] = bf2f((unsigned short)(u.x & 0xFFFF));
+    v[1] = bf2f((unsigned short)(u.x >> 16));
+    v[2] = bf2f((unsigned short)(u.y & 0xFFFF));
+    v[3] = bf2f((unsigned short)(u.y >> 16));
 }
 
 constexpr int C = 64;     // hidden channels
@@ -54,26 +81,62 @@ constexpr int CP = 72;    // LDS pixel / weight-row stride in bf16 (16-byte alig
 constexpr int TS = 16;    // output tile side (pixels)
 constexpr int HS = TS + 2;
 
-// --------------------------------------------------------------- lift --
-// 3x3 conv of the three input planes (charge * rscale, mask, layer index)
-// to 64 channels: one MFMA k step of 32 (27 taps x planes, 5 zero) per
-// 16 x 16 output tile; wave w takes tile rows 4w .. 4w + 3 (64 pixels).
-// wl: [64 co][32 k] bf16 (k = tap * 3 + plane), bias [64].
-__global__ __launch_bounds__(256) void sur_lift_kernel(const float *__restrict__ rho, const uint8_t *__restrict__ matl,
-                                                       const unsigned short *__restrict__ wl,
-                                                       const float *__restrict__ bias, unsigned short *__restrict__ out,
-                                                       int xd, int yd, int zd, float rscale) {
-    __shared__ float in[3][HS][HS];
-    const int z = blockIdx.z, x0 = blockIdx.x * TS, y0 = blockIdx.y * TS;
+// FP32 parameters in device layout (surrogate_param_count):
+//   lift [64 co][32 k] (k = tap * 3 + plane, 27 used) + bias [64]
+//   conv1, conv2 [9 tap][64 co][64 ci] + bias [64] each
+//   mode mixing [1024 mode][64 co][64 ci]
+//   conv3 [9][64][64] + bias [64], projection [64] + bias [1]
+constexpr long CONVN = 9L * C * C;
+constexpr long OFF_L = 0, OFF_BL = OFF_L + 32L * C, OFF_W1 = OFF_BL + C, OFF_W2 = OFF_W1 + CONVN + C,
+               OFF_WM = OFF_W2 + CONVN + C, OFF_W3 = OFF_WM + 1024L * C * C, OFF_WP = OFF_W3 + CONVN + C,
+               OFF_BP = OFF_WP + C, NPARAM = OFF_BP + 1;
+constexpr long OFF_W[3] = {OFF_W1, OFF_W2, OFF_W3};
+
+// input planes of layer z at (gx, gy): own charge, layers at distance 1,
+// layers at distance >= 2 of the film of zf layers (times rscale)
+__device__ __forceinline__ void input_planes(const float *__restrict__ rho, int z, int zf, long plane, long c2,
+                                             float rscale, float (&v)[3]) {
+    const int f0 = (z / zf) * zf, zl = z - f0;
+    float p0 = 0.0f, p1 = 0.0f, p2 = 0.0f;
+    for (int k = 0; k < zf; k++) {
+        const float r = rho[(long)(f0 + k) * plane + c2];
+        const int d = k > zl ? k - zl : zl - k;
+        if (d == 0) p0 = r;
+        else if (d == 1) p1 += r;
+        else p2 += r;
+    }
+    v[0] = p0 * rscale;
+    v[1] = p1 * rscale;
+    v[2] = p2 * rscale;
+}
+
+// the 18 x 18 window of the three input planes around the tile (x0, y0) of layer z
+__device__ __forceinline__ void stage_planes(const float *__restrict__ rho, float (*in)[HS][HS], int z, int zf,
+                                             int x0, int y0, int xd, int yd, float rscale) {
     const long plane = (long)xd * yd;
     for (int e = threadIdx.x; e < HS * HS; e += 256) {
         const int sy = e / HS, sx = e % HS, gx = x0 - 1 + sx, gy = y0 - 1 + sy;
-        const bool ok = gx >= 0 && gx < xd && gy >= 0 && gy < yd;
-        const long c = (long)z * plane + (long)gy * xd + gx;
-        in[0][sy][sx] = ok ? rho[c] * rscale : 0.0f;
-        in[1][sy][sx] = ok ? (matl ? (matl[c] ? 1.0f : 0.0f) : 1.0f) : 0.0f;
-        in[2][sy][sx] = ok ? (float)z / (float)zd : 0.0f;
+        float v[3] = {0.0f, 0.0f, 0.0f};
+        if (gx >= 0 && gx < xd && gy >= 0 && gy < yd) input_planes(rho, z, zf, plane, (long)gy * xd + gx, rscale, v);
+        in[0][sy][sx] = v[0];
+        in[1][sy][sx] = v[1];
+        in[2][sy][sx] = v[2];
     }
+}
+
+// --------------------------------------------------------------- lift --
+// 3x3 conv of the three input planes to 64 channels: one MFMA k step of 32
+// (27 taps x planes, 5 zero) per 16 x 16 output tile; wave w takes tile rows
+// 4w .. 4w + 3 (64 pixels).  wl: [64 co][32 k] bf16 (k = tap * 3 + plane),
+// bias [64]; zsave (or null): BF16 pre-activations.
+__global__ __launch_bounds__(256) void sur_lift_kernel(const float *__restrict__ rho,
+                                                       const unsigned short *__restrict__ wl,
+                                                       const float *__restrict__ bias, unsigned short *__restrict__ out,
+                                                       unsigned short *__restrict__ zsave, int xd, int yd, int zf,
+                                                       float rscale) {
+    __shared__ float in[3][HS][HS];
+    const int z = blockIdx.z, x0 = blockIdx.x * TS, y0 = blockIdx.y * TS;
+    stage_planes(rho, in, z, zf, x0, y0, xd, yd, rscale);
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j = lane & 15, g = lane >> 4;
     bf16x8 a[4];
@@ -85,22 +148,27 @@ __global__ __launch_bounds__(256) void sur_lift_kernel(const float *__restrict__
         u16x8 bv;
 #pragma unroll
         for (int t = 0; t < 8; t++) {
-            const int k = 8 * g + t, tap = k / 3, pl = k % 3;
+            const int k = 8 * g + t, kc = k < 27 ? k : 0, tap = kc / 3, pl = kc % 3;  // kc: reads stay in the window
             bv[t] = (k < 27) ? f2bf(in[pl][py + tap / 3][px + tap % 3]) : (unsigned short)0;
         }
         const bf16x8 b = __builtin_bit_cast(bf16x8, bv);
         const int gx = x0 + px, gy = y0 + py;
         const bool ok = gx < xd && gy < yd;
-        unsigned short *o = out + (((long)z * yd + gy) * xd + gx) * C;
+        const long o = (((long)z * yd + gy) * xd + gx) * C;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            f32x4 acc = mfma_bf16(a[r], b, f32x4{0.f, 0.f, 0.f, 0.f});
+            const f32x4 acc = mfma_bf16(a[r], b, f32x4{0.f, 0.f, 0.f, 0.f});
             // D row 4 g + i of row tile r = output channel 16 r + 4 g + i of pixel j
-            unsigned short v[4];
+            float zv[4], v[4];
 #pragma unroll
-            for (int i = 0; i < 4; i++) v[i] = f2bf(gelu(acc[i] + bias[16 * r + 4 * g + i]));
-            if (ok) *reinterpret_cast<uint2 *>(o + 16 * r + 4 * g) = make_uint2(v[0] | (unsigned)v[1] << 16,
-                                                                              v[2] | (unsigned)v[3] << 16);
+            for (int i = 0; i < 4; i++) {
+                zv[i] = acc[i] + bias[16 * r + 4 * g + i];
+                v[i] = gelu(zv[i]);
+            }
+            if (ok) {
+                *reinterpret_cast<uint2 *>(out + o + 16 * r + 4 * g) = pack4(v);
+                if (zsave) *reinterpret_cast<uint2 *>(zsave + o + 16 * r + 4 * g) = pack4(zv);
+            }
         }
     }
 }
@@ -112,23 +180,33 @@ __global__ __launch_bounds__(256) void sur_lift_kernel(const float *__restrict__
 // next tile's 18 x 18 x 64 input window is loaded into registers while the
 // current one is multiplied, then stored to LDS.  Wave w computes D[64 co]
 // [64 pixels of tile rows 4w..4w+3] over 18 k steps (tap, 32 input
-// channels).  EPI 0: bias (+ add) + GELU, BF16 NHWC out; EPI 1: bias + GELU,
-// then the 1x1 projection to one channel (weights wp, bias bp), times
-// oscale, FP32 potential out.  w: [9 taps][64 co][64 ci] bf16.
+// channels).  w: [9 taps][64 co][64 ci] bf16.  Epilogues:
+//   EPI 0: bias (+ add) + GELU, BF16 NHWC out (zsave: pre-activation)
+//   EPI 1: bias + GELU, then the 1x1 projection to one channel (weights wp,
+//          bias *bp), times oscale, FP32 potential out (zsave as EPI 0)
+//   EPI 2: backward data pass: (sum + add) * GELU'(zin), BF16 NHWC out
+struct ConvArgs {
+    const unsigned short *in, *w;
+    const float *bias;
+    const unsigned short *add;
+    unsigned short *out, *zsave;
+    const unsigned short *zin;
+    const float *wp, *bp;
+    float *phi;
+    float oscale;
+    int xd, yd;
+};
 constexpr int WCH = HS * HS * 8;                 // 16-byte chunks of a window
 constexpr int WPT = (WCH + 255) / 256;           // per thread (11)
+constexpr size_t CONV_LDS = sizeof(unsigned short) * ((size_t)HS * HS * CP + 9 * C * CP);
 
 template <int EPI>
-__global__ __launch_bounds__(256) void sur_conv_kernel(const unsigned short *__restrict__ in,
-                                                       const unsigned short *__restrict__ w,
-                                                       const float *__restrict__ bias,
-                                                       const unsigned short *__restrict__ add,
-                                                       unsigned short *__restrict__ out, const float *__restrict__ wp,
-                                                       float bp, float *__restrict__ phi, float oscale, int xd,
-                                                       int yd) {
+__global__ __launch_bounds__(256) void sur_conv_kernel(ConvArgs A) {
     extern __shared__ __attribute__((aligned(16))) unsigned short lds[];
     unsigned short *win = lds;                 // [HS * HS][CP]
     unsigned short *wt = lds + HS * HS * CP;   // [9 * 64][CP]
+    const int xd = A.xd, yd = A.yd;
+    const unsigned short *__restrict__ in = A.in;
     const int z = blockIdx.z, ntx = (xd + TS - 1) / TS, nty = (yd + TS - 1) / TS, ntile = ntx * nty;
     // the window chunks of tile t into registers (zero outside the film)
     auto load_win = [&](int t, uint4 (&v)[WPT]) {
@@ -148,7 +226,7 @@ __global__ __launch_bounds__(256) void sur_conv_kernel(const unsigned short *__r
 #pragma unroll
         for (int q = 0; q < NW; q++) {
             const int e = threadIdx.x + 256 * q, row = e >> 3, ch = e & 7;
-            v[q] = *reinterpret_cast<const uint4 *>(w + row * C + 8 * ch);
+            v[q] = *reinterpret_cast<const uint4 *>(A.w + row * C + 8 * ch);
         }
 #pragma unroll
         for (int q = 0; q < NW; q++) {
@@ -204,34 +282,278 @@ __global__ __launch_bounds__(256) void sur_conv_kernel(const unsigned short *__r
             float proj = 0.0f;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                float v[4];
-                uint2 ad = make_uint2(0u, 0u);
-                if (EPI == 0 && add && ok) ad = *reinterpret_cast<const uint2 *>(add + pix * C + 16 * r + 4 * g);
-                const unsigned short av[4] = {(unsigned short)(ad.x & 0xFFFF), (unsigned short)(ad.x >> 16),
-                                              (unsigned short)(ad.y & 0xFFFF), (unsigned short)(ad.y >> 16)};
+                const long o = pix * C + 16 * r + 4 * g;
+                float ad[4] = {0.f, 0.f, 0.f, 0.f}, zz[4] = {0.f, 0.f, 0.f, 0.f}, v[4];
+                if (EPI != 1 && A.add && ok) unpack4(*reinterpret_cast<const uint2 *>(A.add + o), ad);
+                if (EPI == 2) {
+                    if (ok) unpack4(*reinterpret_cast<const uint2 *>(A.zin + o), zz);
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int co = 16 * r + 4 * g + i;
-                    float tv = acc[r][c][i] + bias[co];
-                    if (EPI == 0) tv += bf2f(av[i]);
-                    v[i] = gelu(tv);
-                    if (EPI == 1) proj += bf2f(f2bf(v[i])) * wp[co];
-                }
-                if (EPI == 0 && ok) {
-                    unsigned short sv[4];
+                    for (int i = 0; i < 4; i++) v[i] = ok ? (acc[r][c][i] + ad[i]) * gelu_grad(zz[i]) : 0.0f;
+                } else {
 #pragma unroll
-                    for (int i = 0; i < 4; i++) sv[i] = f2bf(v[i]);
-                    *reinterpret_cast<uint2 *>(out + pix * C + 16 * r + 4 * g) =
-                        make_uint2(sv[0] | (unsigned)sv[1] << 16, sv[2] | (unsigned)sv[3] << 16);
+                    for (int i = 0; i < 4; i++) {
+                        const int co = 16 * r + 4 * g + i;
+                        zz[i] = acc[r][c][i] + A.bias[co] + ad[i];
+                        v[i] = gelu(zz[i]);
+                        if (EPI == 1) proj += bf2f(f2bf(v[i])) * A.wp[co];
+                    }
+                    if (A.zsave && ok) *reinterpret_cast<uint2 *>(A.zsave + o) = pack4(zz);
                 }
+                if (EPI != 1 && ok) *reinterpret_cast<uint2 *>(A.out + o) = pack4(v);
             }
             if (EPI == 1) {
                 // the 64 channels of pixel j are spread over lanes j, j + 16, j + 32, j + 48
                 proj += __shfl_xor(proj, 16);
                 proj += __shfl_xor(proj, 32);
-                if (g == 0 && ok) phi[pix] = (proj + bp) * oscale;
+                if (g == 0 && ok) A.phi[pix] = (proj + A.bp[0]) * A.oscale;
             }
         }
+    }
+}
+
+// ------------------------------------------------- weight gradients --
+// The gradient tile dz (16 x 16 pixels x 64 channels, NHWC) transposed into
+// LDS as dzT[co][pixel] (row stride DZS): thread (row, half, channel block)
+// loads 8 pixels x 8 channels and stores 8 channel rows of 8 pixels.
+constexpr int DZS = 256 + 8;
+// element k of 8 packed BF16 values
+__device__ __forceinline__ unsigned bf_elem(const uint4 &v, int k) {
+    const unsigned w = (k >> 1) == 0 ? v.x : (k >> 1) == 1 ? v.y : (k >> 1) == 2 ? v.z : v.w;
+    return (k & 1) ? (w >> 16) : (w & 0xFFFFu);
+}
+// element k of each of v[0..7], packed (the k-th column of an 8 x 8 BF16 block)
+__device__ __forceinline__ uint4 column8(const uint4 (&v)[8], int k) {
+    uint4 o;
+    o.x = bf_elem(v[0], k) | bf_elem(v[1], k) << 16;
+    o.y = bf_elem(v[2], k) | bf_elem(v[3], k) << 16;
+    o.z = bf_elem(v[4], k) | bf_elem(v[5], k) << 16;
+    o.w = bf_elem(v[6], k) | bf_elem(v[7], k) << 16;
+    return o;
+}
+__device__ __forceinline__ void stage_dzT(const unsigned short *__restrict__ dz, unsigned short *dzT, int z, int x0,
+                                          int y0, int xd, int yd) {
+    const int py = threadIdx.x >> 4, q = (threadIdx.x >> 3) & 1, cb = threadIdx.x & 7;
+    uint4 v[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int gx = x0 + 8 * q + i, gy = y0 + py;
+        v[i] = make_uint4(0u, 0u, 0u, 0u);
+        if (gx < xd && gy < yd) v[i] = *reinterpret_cast<const uint4 *>(dz + (((long)z * yd + gy) * xd + gx) * C + 8 * cb);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) *reinterpret_cast<uint4 *>(dzT + (8 * cb + k) * DZS + py * 16 + 8 * q) = column8(v, k);
+}
+
+// bias gradient partial of thread t: channel t / 4, pixels 64 (t % 4) .. + 63 of dzT
+__device__ __forceinline__ float dzT_rowsum(const unsigned short *dzT) {
+    const int co = threadIdx.x >> 2, qq = threadIdx.x & 3;
+    float s = 0.0f;
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(dzT + co * DZS + 64 * qq + 8 * u);
+        const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int i = 0; i < 4; i++) s += bf2f((unsigned short)(w[i] & 0xFFFF)) + bf2f((unsigned short)(w[i] >> 16));
+    }
+    return s;
+}
+
+// dW[tap][co][ci] = sum_p dz[p][co] a[p + off(tap)][ci] (off(tap) = (tap / 3 - 1,
+// tap % 3 - 1), a zero outside the film) and db[co] = sum_p dz[p][co] as MFMA
+// GEMMs over pixels: M = co, N = ci, K = 32 pixels of two tile rows per k step
+// (lane group g: row 2 kk + g / 2, columns 8 (g % 2) .. + 7).  The input window
+// is held transposed in three copies shifted by dx (xc[dx][ci][wy][x] =
+// a(wy, x + dx)), so that every operand fragment is one 16-byte LDS read.
+// Wave w owns input channels 16 w .. 16 w + 15 for all taps and output
+// channels (36 accumulator tiles).  Persistent: the workgroup walks tiles
+// blockIdx.x, + gridDim.x, ... over all layers and writes its partial sums
+// to part[blockIdx.x][9 * 64 * 64 + 64].
+constexpr int XS = HS * 16 + 8;   // xc row stride per channel (296)
+constexpr size_t WGRAD_LDS = sizeof(unsigned short) * ((size_t)C * DZS + 3 * C * XS);
+constexpr long WGRAD_PART = CONVN + C;
+
+__global__ __launch_bounds__(256) void sur_wgrad_kernel(const unsigned short *__restrict__ dz,
+                                                        const unsigned short *__restrict__ a, float *__restrict__ part,
+                                                        int xd, int yd, int nz) {
+    extern __shared__ __attribute__((aligned(16))) unsigned short lds[];
+    unsigned short *dzT = lds;               // [64 co][DZS]
+    unsigned short *xc = lds + C * DZS;      // [3 dx][64 ci][XS]
+    const int ntx = (xd + TS - 1) / TS, nty = (yd + TS - 1) / TS, npl = ntx * nty, ntile = npl * nz;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j = lane & 15, g = lane >> 4;
+    f32x4 acc[9][4];
+#pragma unroll
+    for (int tap = 0; tap < 9; tap++)
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++) acc[tap][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float dbs = 0.0f;
+    for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
+        const int z = t / npl, tl = t % npl, x0 = (tl % ntx) * TS, y0 = (tl / ntx) * TS;
+        __syncthreads();  // the previous tile's products are done with the LDS
+        stage_dzT(dz, dzT, z, x0, y0, xd, yd);
+        for (int item = threadIdx.x; item < 3 * HS * 16; item += 256) {
+            const int cb = item & 7, q = (item >> 3) & 1, wy = (item >> 4) % HS, dx = (item >> 4) / HS;
+            const int gy = y0 - 1 + wy;
+            uint4 v[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int gx = x0 - 1 + 8 * q + i + dx;
+                v[i] = make_uint4(0u, 0u, 0u, 0u);
+                if (gx >= 0 && gx < xd && gy >= 0 && gy < yd)
+                    v[i] = *reinterpret_cast<const uint4 *>(a + (((long)z * yd + gy) * xd + gx) * C + 8 * cb);
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                *reinterpret_cast<uint4 *>(xc + (dx * C + 8 * cb + k) * XS + wy * 16 + 8 * q) = column8(v, k);
+        }
+        __syncthreads();
+        dbs += dzT_rowsum(dzT);
+#pragma unroll 1
+        for (int kk = 0; kk < 8; kk++) {
+            const int row = 2 * kk + (g >> 1), px0 = 8 * (g & 1);
+            bf16x8 A[4];
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++)
+                A[ct] = *reinterpret_cast<const bf16x8 *>(dzT + (16 * ct + j) * DZS + row * 16 + px0);
+#pragma unroll
+            for (int tap = 0; tap < 9; tap++) {
+                const int dy = tap / 3, dx = tap % 3;
+                const bf16x8 B = *reinterpret_cast<const bf16x8 *>(xc + (dx * C + 16 * wave + j) * XS + (row + dy) * 16 + px0);
+#pragma unroll
+                for (int ct = 0; ct < 4; ct++) acc[tap][ct] = mfma_bf16(A[ct], B, acc[tap][ct]);
+            }
+        }
+    }
+    // D row 4 g + i of tile (tap, ct) = co 16 ct + 4 g + i; column j = ci 16 wave + j
+    float *pp = part + (long)blockIdx.x * WGRAD_PART;
+#pragma unroll
+    for (int tap = 0; tap < 9; tap++)
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) pp[(long)tap * C * C + (16 * ct + 4 * g + i) * C + 16 * wave + j] = acc[tap][ct][i];
+    dbs += __shfl_xor(dbs, 1);
+    dbs += __shfl_xor(dbs, 2);
+    if ((threadIdx.x & 3) == 0) pp[CONVN + (threadIdx.x >> 2)] = dbs;
+}
+
+// Lift weight gradient dWl[co][k] = sum_p dz0[p][co] x[p + off(tap)][plane]
+// (k = tap * 3 + plane < 27, x the BF16-rounded input planes), dbl[co] =
+// sum_p dz0[p][co]; M = co (wave w: 16 w .. + 15), N = k (two tiles), K =
+// pixels as in sur_wgrad_kernel.  part[blockIdx.x][64 * 32 + 64].
+constexpr long LGRAD_PART = 32L * C + C;
+__global__ __launch_bounds__(256) void sur_lift_wgrad_kernel(const float *__restrict__ rho,
+                                                             const unsigned short *__restrict__ dz,
+                                                             float *__restrict__ part, int xd, int yd, int nz, int zf,
+                                                             float rscale) {
+    __shared__ __attribute__((aligned(16))) unsigned short dzT[C * DZS];
+    __shared__ float in[3][HS][HS];
+    const int ntx = (xd + TS - 1) / TS, nty = (yd + TS - 1) / TS, npl = ntx * nty, ntile = npl * nz;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j = lane & 15, g = lane >> 4;
+    f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+    float dbs = 0.0f;
+    for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
+        const int z = t / npl, tl = t % npl, x0 = (tl % ntx) * TS, y0 = (tl / ntx) * TS;
+        __syncthreads();
+        stage_dzT(dz, dzT, z, x0, y0, xd, yd);
+        stage_planes(rho, in, z, zf, x0, y0, xd, yd, rscale);
+        __syncthreads();
+        dbs += dzT_rowsum(dzT);
+#pragma unroll 1
+        for (int kk = 0; kk < 8; kk++) {
+            const int row = 2 * kk + (g >> 1), px0 = 8 * (g & 1);
+            const bf16x8 A = *reinterpret_cast<const bf16x8 *>(dzT + (16 * wave + j) * DZS + row * 16 + px0);
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++) {
+                const int k = 16 * nt + j, kc = k < 27 ? k : 0, tap = kc / 3, pl = kc % 3, dy = tap / 3, dx = tap % 3;
+                u16x8 bv;
+#pragma unroll
+                for (int u = 0; u < 8; u++) bv[u] = (k < 27) ? f2bf(in[pl][row + dy][px0 + u + dx]) : (unsigned short)0;
+                acc[nt] = mfma_bf16(A, __builtin_bit_cast(bf16x8, bv), acc[nt]);
+            }
+        }
+    }
+    float *pp = part + (long)blockIdx.x * LGRAD_PART;
+#pragma unroll
+    for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) pp[(16 * wave + 4 * g + i) * 32 + 16 * nt + j] = acc[nt][i];
+    dbs += __shfl_xor(dbs, 1);
+    dbs += __shfl_xor(dbs, 2);
+    if ((threadIdx.x & 3) == 0) pp[32 * C + (threadIdx.x >> 2)] = dbs;
+}
+
+// out[i] = sum_b part[b * stride + i], i < len (fixed order: deterministic)
+__global__ __launch_bounds__(256) void sur_sum_kernel(const float *__restrict__ part, long stride, int nb, long len,
+                                                      float *__restrict__ out) {
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= len) return;
+    float s = 0.0f;
+    for (int b = 0; b < nb; b++) s += part[(long)b * stride + i];
+    out[i] = s;
+}
+
+// ---------------------------------------------------- loss and head --
+// Per film f of npf cells: loss_f = sum (phi_s - phi_t)^2 inv_norm[f]; with dz3
+// non-null also dphi = gscale (phi_s - phi_t) inv_norm[f], dproj = dphi
+// oscale, dz3 = dproj wp GELU'(z3) and the partials of dwp = sum dproj
+// GELU(z3) (BF16-rounded as the forward projection reads it) and dbp = sum
+// dproj.  Grid (HB, films); thread = (pixel slot t / 8, channel block t % 8).
+// part[(f * HB + blk) * 66 + (0..63 dwp, 64 dbp, 65 loss)].
+constexpr int HB = 64, HPART = 66;
+__global__ __launch_bounds__(256) void sur_head_kernel(const float *__restrict__ phs, const float *__restrict__ pht,
+                                                       const float *__restrict__ inv_norm, float gscale, float oscale,
+                                                       const unsigned short *__restrict__ z3,
+                                                       const float *__restrict__ wp, unsigned short *__restrict__ dz3,
+                                                       long npf, float *__restrict__ part) {
+    __shared__ float sw[32][C + 1];
+    __shared__ float sl[32], sb[32];
+    const int f = blockIdx.y, ch = threadIdx.x & 7, slot = threadIdx.x >> 3;
+    const long base = (long)f * npf;
+    const float inv = inv_norm[f];
+    float dw[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, wv[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) wv[k] = wp[8 * ch + k];
+    float ls = 0.0f, db = 0.0f;
+    for (long p = (long)blockIdx.x * 32 + slot; p < npf; p += (long)HB * 32) {
+        const float d = phs[base + p] - pht[base + p];
+        ls += d * d * inv;
+        if (!dz3) continue;
+        const float dproj = gscale * d * inv * oscale;
+        db += dproj;
+        const uint4 zv = *reinterpret_cast<const uint4 *>(z3 + (base + p) * C + 8 * ch);
+        const unsigned zw[4] = {zv.x, zv.y, zv.z, zv.w};
+        unsigned ow[4];
+#pragma unroll
+        for (int h = 0; h < 4; h++) {
+            unsigned short o2[2];
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const int k = 2 * h + e;
+                const float zz = bf2f((unsigned short)(e ? zw[h] >> 16 : zw[h] & 0xFFFF));
+                dw[k] += dproj * bf2f(f2bf(gelu(zz)));
+                o2[e] = f2bf(dproj * wv[k] * gelu_grad(zz));
+            }
+            ow[h] = o2[0] | (unsigned)o2[1] << 16;
+        }
+        *reinterpret_cast<uint4 *>(dz3 + (base + p) * C + 8 * ch) = make_uint4(ow[0], ow[1], ow[2], ow[3]);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) sw[slot][8 * ch + k] = dw[k];
+    if (ch == 0) { sl[slot] = ls; sb[slot] = db; }
+    __syncthreads();
+    float *pp = part + ((long)f * HB + blockIdx.x) * HPART;
+    if (threadIdx.x < C) {
+        float s = 0.0f;
+        for (int q = 0; q < 32; q++) s += sw[q][threadIdx.x];
+        pp[threadIdx.x] = s;
+    } else if (threadIdx.x == C) {
+        float s = 0.0f;
+        for (int q = 0; q < 32; q++) s += sb[q];
+        pp[C] = s;
+    } else if (threadIdx.x == C + 1) {
+        float s = 0.0f;
+        for (int q = 0; q < 32; q++) s += sl[q];
+        pp[C + 1] = s;
     }
 }
 
@@ -310,17 +632,41 @@ __global__ __launch_bounds__(256) void sur_gemm_kernel(GemmArgs g) {
             }
 }
 
-// per-mode channel mixing: G[z][mode][co] = sum_ci H[z][mode][ci] Wm[mode][co][ci]
-// (one workgroup of 64 threads per (mode, layer); FP32 in and out, BF16 weights)
+// per-mode channel mixing (one workgroup of 64 threads per (mode, layer); FP32
+// in and out, BF16 weights):
+//   forward     G[z][mode][co] = sum_ci H[z][mode][ci] Wm[mode][co][ci]
+//   transposed  G[z][mode][ci] = sum_co H[z][mode][co] Wm[mode][co][ci]
+template <bool TR>
 __global__ __launch_bounds__(64) void sur_mix_kernel(const float *__restrict__ H, const unsigned short *__restrict__ wm,
                                                      float *__restrict__ G, int nmode) {
-    const int mode = blockIdx.x, z = blockIdx.y, co = threadIdx.x;
+    const int mode = blockIdx.x, z = blockIdx.y, o = threadIdx.x;
     const float *h = H + ((long)z * nmode + mode) * C;
-    const unsigned short *wr = wm + ((long)mode * C + co) * C;
+    const unsigned short *wb = wm + (long)mode * C * C;
     float s = 0.0f;
 #pragma unroll 8
-    for (int ci = 0; ci < C; ci++) s += bf2f(wr[ci]) * h[ci];
-    G[((long)z * nmode + mode) * C + co] = s;
+    for (int k = 0; k < C; k++) s += bf2f(TR ? wb[k * C + o] : wb[o * C + k]) * h[k];
+    G[((long)z * nmode + mode) * C + o] = s;
+}
+
+// dWm[mode][co][ci] = sum_z dG[z][mode][co] H[z][mode][ci]; one workgroup per mode
+__global__ __launch_bounds__(256) void sur_mix_wgrad_kernel(const float *__restrict__ dG, const float *__restrict__ H,
+                                                            float *__restrict__ dW, int nmode, int nz) {
+    __shared__ float sg[C], sh[C];
+    const int mode = blockIdx.x, ci = threadIdx.x & 63, co0 = threadIdx.x >> 6;
+    float acc[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) acc[u] = 0.0f;
+    for (int z = 0; z < nz; z++) {
+        __syncthreads();
+        if (threadIdx.x < C) sg[threadIdx.x] = dG[((long)z * nmode + mode) * C + threadIdx.x];
+        else if (threadIdx.x < 2 * C) sh[threadIdx.x - C] = H[((long)z * nmode + mode) * C + threadIdx.x - C];
+        __syncthreads();
+        const float hv = sh[ci];
+#pragma unroll
+        for (int u = 0; u < 16; u++) acc[u] += sg[co0 + 4 * u] * hv;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u++) dW[((long)mode * C + co0 + 4 * u) * C + ci] = acc[u];
 }
 
 // relative L2 residual partials of a against b: [nblk][2] = (sum (a-b)^2, sum b^2)
@@ -341,6 +687,34 @@ __global__ __launch_bounds__(256) void sur_residual_kernel(const float *__restri
         __syncthreads();
     }
     if (threadIdx.x == 0) { part[2 * blockIdx.x] = s0[0]; part[2 * blockIdx.x + 1] = s1[0]; }
+}
+
+// --------------------------------------------- parameters and Adam --
+// BF16 copy of n FP32 values
+__global__ __launch_bounds__(256) void sur_to_bf_kernel(const float *__restrict__ src, unsigned short *__restrict__ dst,
+                                                        long n) {
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i < n) dst[i] = f2bf(src[i]);
+}
+// flipped, transposed BF16 taps of a conv: wt[tap][ci][co] = w[8 - tap][co][ci]
+__global__ __launch_bounds__(256) void sur_flipT_kernel(const float *__restrict__ w, unsigned short *__restrict__ wt) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= CONVN) return;
+    const int tap = i / (C * C), r = i % (C * C), ci = r / C, co = r % C;
+    wt[i] = f2bf(w[((8 - tap) * C + co) * C + ci]);
+}
+// Adam (bias-corrected; c1 = 1 / (1 - b1^t), c2 = 1 / (1 - b2^t))
+__global__ __launch_bounds__(256) void sur_adam_kernel(float *__restrict__ w, float *__restrict__ m,
+                                                       float *__restrict__ v, const float *__restrict__ g, long n,
+                                                       float lr, float b1, float b2, float c1, float c2, float eps) {
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
 }
 
 template <typename TA, typename TB, typename TC>
@@ -371,10 +745,80 @@ unsigned short host_bf(float x) {
     return (unsigned short)(u >> 16);
 }
 
+int launch_wgrad(SurrogatePlan &sp, const unsigned short *dz, const unsigned short *a, float *grad, hipStream_t s) {
+    const Grid &g = sp.g;
+    const int ntile = ((g.xd + TS - 1) / TS) * ((g.yd + TS - 1) / TS) * g.zd;
+    const int nb = ntile < SurrogatePlan::kPartBlocks ? ntile : SurrogatePlan::kPartBlocks;
+    hipLaunchKernelGGL(sur_wgrad_kernel, dim3(nb), dim3(256), WGRAD_LDS, s, dz, a, sp.d_wpart, g.xd, g.yd, g.zd);
+    MMS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(sur_sum_kernel, dim3((unsigned)((WGRAD_PART + 255) / 256)), dim3(256), 0, s, sp.d_wpart,
+                       WGRAD_PART, nb, WGRAD_PART, grad);
+    MMS_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_conv(int epi, const SurrogatePlan &sp, const ConvArgs &a, hipStream_t s) {
+    const Grid &g = sp.g;
+    const int ntile = ((g.xd + TS - 1) / TS) * ((g.yd + TS - 1) / TS);
+    // one resident workgroup per CU per layer slice walks the tiles
+    int per_layer = (256 + g.zd - 1) / g.zd;
+    if (per_layer > ntile) per_layer = ntile;
+    dim3 grid(per_layer, 1, g.zd);
+    if (epi == 0) hipLaunchKernelGGL((sur_conv_kernel<0>), grid, dim3(256), CONV_LDS, s, a);
+    else if (epi == 1) hipLaunchKernelGGL((sur_conv_kernel<1>), grid, dim3(256), CONV_LDS, s, a);
+    else hipLaunchKernelGGL((sur_conv_kernel<2>), grid, dim3(256), CONV_LDS, s, a);
+    MMS_HIP(hipGetLastError());
+    return 0;
+}
+
+// The spectral branch's transform chain of the BF16 NHWC array `in`: X1 = Bx in,
+// hbuf = By X1, gbuf = mix(hbuf) (transposed mixing when tr), X2 = By^T gbuf,
+// out = Bx^T X2 (BF16 NHWC).
+int spectral(SurrogatePlan &sp, const unsigned short *in, float *hbuf, float *gbuf, bool tr, unsigned short *out,
+             hipStream_t s) {
+    const int X = sp.g.xd, Y = sp.g.yd, Z = sp.g.zd;
+    {
+        GemmArgs a{};  // X1[z][y][kx][c] = sum_x Bx[kx][x] in[z][y][x][c]
+        a.A = sp.d_bx; a.sam = X; a.sak = 1; a.sab = 0;
+        a.B = in; a.sbk = C; a.sbn = 1; a.sbb = (long)X * C;
+        a.C = sp.d_x1; a.scm = C; a.scn = 1; a.scb = 32L * C;
+        a.M = 32; a.N = C; a.K = X;
+        if (gemm<unsigned short, unsigned short, float>(a, Z * Y, s)) return 1;
+    }
+    {
+        GemmArgs a{};  // H[z][ky][(kx, c)] = sum_y By[ky][y] X1[z][y][(kx, c)]
+        a.A = sp.d_by; a.sam = Y; a.sak = 1; a.sab = 0;
+        a.B = sp.d_x1; a.sbk = 32L * C; a.sbn = 1; a.sbb = (long)Y * 32 * C;
+        a.C = hbuf; a.scm = 32L * C; a.scn = 1; a.scb = (long)sp.nmode * C;
+        a.M = 32; a.N = 32 * C; a.K = Y;
+        if (gemm<unsigned short, float, float>(a, Z, s)) return 1;
+    }
+    if (tr) hipLaunchKernelGGL((sur_mix_kernel<true>), dim3(sp.nmode, Z), dim3(64), 0, s, hbuf, sp.d_wm, gbuf, sp.nmode);
+    else hipLaunchKernelGGL((sur_mix_kernel<false>), dim3(sp.nmode, Z), dim3(64), 0, s, hbuf, sp.d_wm, gbuf, sp.nmode);
+    MMS_HIP(hipGetLastError());
+    {
+        GemmArgs a{};  // X2[z][y][(kx, c)] = sum_ky By[ky][y] G[z][ky][(kx, c)]
+        a.A = sp.d_by; a.sam = 1; a.sak = Y; a.sab = 0;
+        a.B = gbuf; a.sbk = 32L * C; a.sbn = 1; a.sbb = (long)sp.nmode * C;
+        a.C = sp.d_x2; a.scm = 32L * C; a.scn = 1; a.scb = (long)Y * 32 * C;
+        a.M = Y; a.N = 32 * C; a.K = 32;
+        if (gemm<unsigned short, float, float>(a, Z, s)) return 1;
+    }
+    {
+        GemmArgs a{};  // out[z][y][x][c] = sum_kx Bx[kx][x] X2[z][y][kx][c]  (BF16 NHWC)
+        a.A = sp.d_bx; a.sam = 1; a.sak = X; a.sab = 0;
+        a.B = sp.d_x2; a.sbk = C; a.sbn = 1; a.sbb = 32L * C;
+        a.C = out; a.scm = C; a.scn = 1; a.scb = (long)X * C;
+        a.M = X; a.N = C; a.K = 32;
+        if (gemm<unsigned short, float, unsigned short>(a, Z * Y, s)) return 1;
+    }
+    return 0;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------ host plan --
-// Weight vector layout (FP32, in this order; surrogate_weight_count):
+// Weight vector layout of the API (FP32, in this order; surrogate_weight_count):
 //   lift   [64 co][3 planes][3 ky][3 kx] + bias [64]
 //   conv1  [64 co][64 ci][3][3] + bias [64]
 //   conv2  [64 co][64 ci][3][3] + bias [64]
@@ -384,11 +828,66 @@ unsigned short host_bf(float x) {
 long surrogate_weight_count() {
     return (long)C * 27 + C + 3L * (C * C * 9 + C) + 32L * 32 * C * C + C + 1;
 }
+long surrogate_param_count() { return NPARAM; }
 
-int SurrogatePlan::init(const Grid &g_, const float *weights) {
+// API order -> device layout (the lift's k = tap * 3 + plane padded to 32,
+// conv taps outermost); the other blocks keep their order
+void surrogate_api_to_param(const float *api, float *prm) {
+    std::fill(prm, prm + NPARAM, 0.0f);
+    long o = 0;
+    for (int co = 0; co < C; co++)
+        for (int pl = 0; pl < 3; pl++)
+            for (int tap = 0; tap < 9; tap++) prm[OFF_L + co * 32 + tap * 3 + pl] = api[o + (co * 3 + pl) * 9 + tap];
+    o += C * 27;
+    for (int co = 0; co < C; co++) prm[OFF_BL + co] = api[o + co];
+    o += C;
+    auto conv = [&](long off) {
+        for (int co = 0; co < C; co++)
+            for (int ci = 0; ci < C; ci++)
+                for (int tap = 0; tap < 9; tap++) prm[off + ((long)tap * C + co) * C + ci] = api[o + (co * C + ci) * 9 + tap];
+        o += CONVN;
+        for (int co = 0; co < C; co++) prm[off + CONVN + co] = api[o + co];
+        o += C;
+    };
+    conv(OFF_W1);
+    conv(OFF_W2);
+    std::copy(api + o, api + o + 1024L * C * C, prm + OFF_WM);
+    o += 1024L * C * C;
+    conv(OFF_W3);
+    std::copy(api + o, api + o + C + 1, prm + OFF_WP);
+}
+
+void surrogate_param_to_api(const float *prm, float *api) {
+    long o = 0;
+    for (int co = 0; co < C; co++)
+        for (int pl = 0; pl < 3; pl++)
+            for (int tap = 0; tap < 9; tap++) api[o + (co * 3 + pl) * 9 + tap] = prm[OFF_L + co * 32 + tap * 3 + pl];
+    o += C * 27;
+    for (int co = 0; co < C; co++) api[o + co] = prm[OFF_BL + co];
+    o += C;
+    auto conv = [&](long off) {
+        for (int co = 0; co < C; co++)
+            for (int ci = 0; ci < C; ci++)
+                for (int tap = 0; tap < 9; tap++) api[o + (co * C + ci) * 9 + tap] = prm[off + ((long)tap * C + co) * C + ci];
+        o += CONVN;
+        for (int co = 0; co < C; co++) api[o + co] = prm[off + CONVN + co];
+        o += C;
+    };
+    conv(OFF_W1);
+    conv(OFF_W2);
+    std::copy(prm + OFF_WM, prm + OFF_WM + 1024L * C * C, api + o);
+    o += 1024L * C * C;
+    conv(OFF_W3);
+    std::copy(prm + OFF_WP, prm + OFF_WP + C + 1, api + o);
+}
+
+int SurrogatePlan::init(const Grid &g_, int zf_, const float *weights, bool train_, hipStream_t s) {
     release();
+    MMS_CHECK(zf_ >= 1 && g_.zd % zf_ == 0, "surrogate: layers are not a whole number of films");
     g = g_;
-    const long n = g.n(), npix = n;
+    zf = zf_;
+    train = train_;
+    const long npix = g.n();
     nmode = 32 * 32;
     std::vector<float> wv((size_t)surrogate_weight_count());
     if (weights) {
@@ -409,34 +908,16 @@ int SurrogatePlan::init(const Grid &g_, const float *weights) {
         for (int i = 0; i < C; i++) wv[o++] = r.sym(sqrt(3.0 / C));
         wv[o++] = 0.0f;
     }
-    size_t o = 0;
-    // lift: [co][k = tap * 3 + plane], k padded to 32
-    std::vector<unsigned short> lift((size_t)C * 32, 0);
-    for (int co = 0; co < C; co++)
-        for (int pl = 0; pl < 3; pl++)
-            for (int tap = 0; tap < 9; tap++) lift[co * 32 + tap * 3 + pl] = host_bf(wv[o + (co * 3 + pl) * 9 + tap]);
-    o += C * 27;
-    std::vector<float> bl(wv.begin() + o, wv.begin() + o + C);
-    o += C;
-    auto conv = [&](std::vector<unsigned short> &w, std::vector<float> &b) {
-        w.assign((size_t)9 * C * C, 0);  // [tap][co][ci]
-        for (int co = 0; co < C; co++)
-            for (int ci = 0; ci < C; ci++)
-                for (int tap = 0; tap < 9; tap++) w[((size_t)tap * C + co) * C + ci] = host_bf(wv[o + (co * C + ci) * 9 + tap]);
-        o += (size_t)C * C * 9;
-        b.assign(wv.begin() + o, wv.begin() + o + C);
-        o += C;
-    };
-    std::vector<unsigned short> w1, w2, w3, wm((size_t)nmode * C * C);
-    std::vector<float> b1, b2, b3;
-    conv(w1, b1);
-    conv(w2, b2);
-    for (size_t i = 0; i < wm.size(); i++) wm[i] = host_bf(wv[o + i]);  // [mode][co][ci]
-    o += wm.size();
-    conv(w3, b3);
-    std::vector<float> wpv(wv.begin() + o, wv.begin() + o + C);
-    o += C;
-    bp = wv[o++];
+    std::vector<float> prm((size_t)NPARAM);
+    surrogate_api_to_param(wv.data(), prm.data());
+    MMS_HIP(hipMalloc(&d_param, sizeof(float) * NPARAM));
+    MMS_HIP(hipMemcpyAsync(d_param, prm.data(), sizeof(float) * NPARAM, hipMemcpyHostToDevice, s));
+    MMS_HIP(hipMalloc(&d_lift, sizeof(unsigned short) * 32 * C));
+    for (int l = 0; l < 3; l++) {
+        MMS_HIP(hipMalloc(&d_w[l], sizeof(unsigned short) * CONVN));
+        if (train) MMS_HIP(hipMalloc(&d_wt[l], sizeof(unsigned short) * CONVN));
+    }
+    MMS_HIP(hipMalloc(&d_wm, sizeof(unsigned short) * nmode * C * C));
     // real Fourier bases (BF16): Bx[k][x] over the padded width, By[k][y]
     // rows scaled by sqrt(2 / N), so that Bx Bx^T is close to the identity
     auto basis = [](int N) {
@@ -452,110 +933,175 @@ int SurrogatePlan::init(const Grid &g_, const float *weights) {
         return b;
     };
     std::vector<unsigned short> bx = basis(g.xd), by = basis(g.yd);
-    auto up = [](void **dst, const void *src, size_t bytes) -> int {
-        MMS_HIP(hipMalloc(dst, bytes));
-        MMS_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
-        return 0;
-    };
-    if (up((void **)&d_lift, lift.data(), lift.size() * 2) || up((void **)&d_blift, bl.data(), bl.size() * 4) ||
-        up((void **)&d_w1, w1.data(), w1.size() * 2) || up((void **)&d_b1, b1.data(), b1.size() * 4) ||
-        up((void **)&d_w2, w2.data(), w2.size() * 2) || up((void **)&d_b2, b2.data(), b2.size() * 4) ||
-        up((void **)&d_w3, w3.data(), w3.size() * 2) || up((void **)&d_b3, b3.data(), b3.size() * 4) ||
-        up((void **)&d_wm, wm.data(), wm.size() * 2) || up((void **)&d_wp, wpv.data(), wpv.size() * 4) ||
-        up((void **)&d_bx, bx.data(), bx.size() * 2) || up((void **)&d_by, by.data(), by.size() * 2))
-        return 1;
-    MMS_HIP(hipMalloc(&d_h0, sizeof(unsigned short) * npix * C));
-    MMS_HIP(hipMalloc(&d_h1, sizeof(unsigned short) * npix * C));
-    MMS_HIP(hipMalloc(&d_spec, sizeof(unsigned short) * npix * C));
+    MMS_HIP(hipMalloc(&d_bx, bx.size() * 2));
+    MMS_HIP(hipMalloc(&d_by, by.size() * 2));
+    MMS_HIP(hipMemcpyAsync(d_bx, bx.data(), bx.size() * 2, hipMemcpyHostToDevice, s));
+    MMS_HIP(hipMemcpyAsync(d_by, by.data(), by.size() * 2, hipMemcpyHostToDevice, s));
+    const size_t act = sizeof(unsigned short) * npix * C;
+    MMS_HIP(hipMalloc(&d_h0, act));
+    MMS_HIP(hipMalloc(&d_h1, act));
+    MMS_HIP(hipMalloc(&d_spec, act));
+    d_h2 = d_h0;  // inference: conv2's output overwrites the lift's
     MMS_HIP(hipMalloc(&d_x1, sizeof(float) * (size_t)g.zd * g.yd * 32 * C));
     MMS_HIP(hipMalloc(&d_hm, sizeof(float) * (size_t)g.zd * nmode * C));
     MMS_HIP(hipMalloc(&d_gm, sizeof(float) * (size_t)g.zd * nmode * C));
     MMS_HIP(hipMalloc(&d_x2, sizeof(float) * (size_t)g.zd * g.yd * 32 * C));
     MMS_HIP(hipMalloc(&d_part, sizeof(double) * 2 * 1024));
+    if (train) {
+        MMS_HIP(hipMalloc(&d_h2, act));
+        for (int l = 0; l < 4; l++) MMS_HIP(hipMalloc(&d_z[l], act));
+        MMS_HIP(hipMalloc(&d_dza, act));
+        MMS_HIP(hipMalloc(&d_dzb, act));
+        MMS_HIP(hipMalloc(&d_dh, sizeof(float) * (size_t)g.zd * nmode * C));
+        MMS_HIP(hipMalloc(&d_wpart, sizeof(float) * kPartBlocks * WGRAD_PART));
+        MMS_HIP(hipMalloc(&d_hpart, sizeof(float) * (size_t)(g.zd / zf) * HB * HPART));
+        MMS_HIP(hipMalloc(&d_hsum, sizeof(float) * HPART));
+    }
+    static bool attr = false;
+    if (!attr) {
+        for (const void *k : {reinterpret_cast<const void *>(&sur_conv_kernel<0>),
+                              reinterpret_cast<const void *>(&sur_conv_kernel<1>),
+                              reinterpret_cast<const void *>(&sur_conv_kernel<2>)})
+            MMS_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)CONV_LDS));
+        MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&sur_wgrad_kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)WGRAD_LDS));
+        attr = true;
+    }
+    if (pack(s)) return 1;
+    MMS_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+// BF16 device copies of the FP32 parameters
+int SurrogatePlan::pack(hipStream_t s) {
+    auto bf = [&](long off, unsigned short *dst, long n) -> int {
+        hipLaunchKernelGGL(sur_to_bf_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_param + off, dst, n);
+        MMS_HIP(hipGetLastError());
+        return 0;
+    };
+    if (bf(OFF_L, d_lift, 32L * C) || bf(OFF_WM, d_wm, (long)nmode * C * C)) return 1;
+    for (int l = 0; l < 3; l++) {
+        if (bf(OFF_W[l], d_w[l], CONVN)) return 1;
+        if (d_wt[l]) {
+            hipLaunchKernelGGL(sur_flipT_kernel, dim3((unsigned)((CONVN + 255) / 256)), dim3(256), 0, s,
+                               d_param + OFF_W[l], d_wt[l]);
+            MMS_HIP(hipGetLastError());
+        }
+    }
     return 0;
 }
 
 void SurrogatePlan::release() {
-    void *bufs[] = {d_lift, d_blift, d_w1, d_b1, d_w2, d_b2, d_w3, d_b3, d_wm, d_wp, d_bx, d_by,
-                    d_h0,   d_h1,    d_spec, d_x1, d_hm, d_gm, d_x2, d_part};
+    if (d_h2 == d_h0) d_h2 = nullptr;
+    void *bufs[] = {d_param, d_lift, d_w[0], d_w[1], d_w[2], d_wt[0], d_wt[1], d_wt[2], d_wm, d_bx, d_by,
+                    d_h0,    d_h1,   d_h2,   d_spec, d_z[0], d_z[1],  d_z[2],  d_z[3],  d_dza, d_dzb,
+                    d_x1,    d_hm,   d_gm,   d_x2,   d_dh,   d_wpart, d_hpart, d_hsum,  d_part};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    d_lift = d_w1 = d_w2 = d_w3 = d_wm = d_bx = d_by = nullptr;
-    d_h0 = d_h1 = d_spec = nullptr;
-    d_blift = d_b1 = d_b2 = d_b3 = d_wp = d_x1 = d_hm = d_gm = d_x2 = nullptr;
+    d_param = d_x1 = d_hm = d_gm = d_x2 = d_dh = d_wpart = d_hpart = d_hsum = nullptr;
+    d_lift = d_wm = d_bx = d_by = d_h0 = d_h1 = d_h2 = d_spec = d_dza = d_dzb = nullptr;
+    for (int l = 0; l < 3; l++) d_w[l] = d_wt[l] = nullptr;
+    for (int l = 0; l < 4; l++) d_z[l] = nullptr;
     d_part = nullptr;
 }
 
-int surrogate_potential(SurrogatePlan &sp, const float *rho, const uint8_t *matl, float *phi, float rscale,
-                        float oscale, hipStream_t s, Profiler *prof) {
+int surrogate_potential(SurrogatePlan &sp, const float *rho, float *phi, float rscale, float oscale, hipStream_t s,
+                        Profiler *prof) {
     const Grid &g = sp.g;
     MMS_CHECK(sp.d_h0 != nullptr, "surrogate not initialised");
     const int X = g.xd, Y = g.yd, Z = g.zd;
-    const long npix = g.n();
-    dim3 grid((X + TS - 1) / TS, (Y + TS - 1) / TS, Z), blk(256);
-    const size_t conv_lds = sizeof(unsigned short) * ((size_t)HS * HS * CP + 9 * C * CP);
-    static int attr = 0;
-    if (!attr) {
-        MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&sur_conv_kernel<0>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)conv_lds));
-        MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&sur_conv_kernel<1>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)conv_lds));
-        attr = 1;
-    }
-    // convolutions: one resident workgroup per CU per layer slice walks the tiles
-    const int ntile = ((X + TS - 1) / TS) * ((Y + TS - 1) / TS);
-    int per_layer = (256 + Z - 1) / Z;
-    if (per_layer > ntile) per_layer = ntile;
-    dim3 cgrid(per_layer, 1, Z);
+    const float *P = sp.d_param;
     KTimer kt(prof, K_SURROGATE);
-    hipLaunchKernelGGL(sur_lift_kernel, grid, blk, 0, s, rho, matl, sp.d_lift, sp.d_blift, sp.d_h0, X, Y, Z, rscale);
+    hipLaunchKernelGGL(sur_lift_kernel, dim3((X + TS - 1) / TS, (Y + TS - 1) / TS, Z), dim3(256), 0, s, rho, sp.d_lift,
+                       P + OFF_BL, sp.d_h0, sp.d_z[0], X, Y, sp.zf, rscale);
     MMS_HIP(hipGetLastError());
-    hipLaunchKernelGGL((sur_conv_kernel<0>), cgrid, blk, conv_lds, s, sp.d_h0, sp.d_w1, sp.d_b1,
-                       (const unsigned short *)nullptr, sp.d_h1, (const float *)nullptr, 0.0f, (float *)nullptr, 0.0f,
-                       X, Y);
+    ConvArgs a{};
+    a.xd = X; a.yd = Y;
+    a.in = sp.d_h0; a.w = sp.d_w[0]; a.bias = P + OFF_W1 + CONVN; a.out = sp.d_h1; a.zsave = sp.d_z[1];
+    if (launch_conv(0, sp, a, s)) return 1;
+    // spectral branch of h1
+    if (spectral(sp, sp.d_h1, sp.d_hm, sp.d_gm, false, sp.d_spec, s)) return 1;
+    a.in = sp.d_h1; a.w = sp.d_w[1]; a.bias = P + OFF_W2 + CONVN; a.add = sp.d_spec; a.out = sp.d_h2;
+    a.zsave = sp.d_z[2];
+    if (launch_conv(0, sp, a, s)) return 1;
+    a = ConvArgs{};
+    a.xd = X; a.yd = Y;
+    a.in = sp.d_h2; a.w = sp.d_w[2]; a.bias = P + OFF_W3 + CONVN; a.zsave = sp.d_z[3];
+    a.wp = P + OFF_WP; a.bp = P + OFF_BP; a.phi = phi; a.oscale = oscale;
+    return launch_conv(1, sp, a, s);
+}
+
+int surrogate_losses(SurrogatePlan &sp, const float *phi_s, const float *phi_t, const float *inv_norm, float gscale,
+                     float oscale, bool grad, hipStream_t s) {
+    const int nf = sp.g.zd / sp.zf;
+    hipLaunchKernelGGL(sur_head_kernel, dim3(HB, nf), dim3(256), 0, s, phi_s, phi_t, inv_norm, gscale, oscale,
+                       sp.d_z[3], sp.d_param + OFF_WP, grad ? sp.d_dza : (unsigned short *)nullptr, sp.g.n() / nf,
+                       sp.d_hpart);
     MMS_HIP(hipGetLastError());
-    // spectral branch of h1: x transform, y transform, mixing, inverse y, inverse x
-    {
-        GemmArgs a{};  // X1[z][y][kx][c] = sum_x Bx[kx][x] h1[z][y][x][c]
-        a.A = sp.d_bx; a.sam = X; a.sak = 1; a.sab = 0;
-        a.B = sp.d_h1; a.sbk = C; a.sbn = 1; a.sbb = (long)X * C;
-        a.C = sp.d_x1; a.scm = C; a.scn = 1; a.scb = 32L * C;
-        a.M = 32; a.N = C; a.K = X;
-        if (gemm<unsigned short, unsigned short, float>(a, Z * Y, s)) return 1;
-    }
-    {
-        GemmArgs a{};  // H[z][ky][(kx, c)] = sum_y By[ky][y] X1[z][y][(kx, c)]
-        a.A = sp.d_by; a.sam = Y; a.sak = 1; a.sab = 0;
-        a.B = sp.d_x1; a.sbk = 32L * C; a.sbn = 1; a.sbb = (long)Y * 32 * C;
-        a.C = sp.d_hm; a.scm = 32L * C; a.scn = 1; a.scb = (long)sp.nmode * C;
-        a.M = 32; a.N = 32 * C; a.K = Y;
-        if (gemm<unsigned short, float, float>(a, Z, s)) return 1;
-    }
-    hipLaunchKernelGGL(sur_mix_kernel, dim3(sp.nmode, Z), dim3(64), 0, s, sp.d_hm, sp.d_wm, sp.d_gm, sp.nmode);
+    return 0;
+}
+
+int surrogate_backward(SurrogatePlan &sp, const float *rho, const float *phi_s, const float *phi_t,
+                       const float *inv_norm, float gscale, float rscale, float oscale, float *grad, hipStream_t s) {
+    MMS_CHECK(sp.train, "surrogate plan without training buffers");
+    const Grid &g = sp.g;
+    const int nf = g.zd / sp.zf;
+    // loss and projection: dz3 -> d_dza, dwp / dbp / loss partials -> d_hsum
+    if (surrogate_losses(sp, phi_s, phi_t, inv_norm, gscale, oscale, true, s)) return 1;
+    hipLaunchKernelGGL(sur_sum_kernel, dim3(1), dim3(256), 0, s, sp.d_hpart, (long)HPART, nf * HB, (long)HPART,
+                       sp.d_hsum);
     MMS_HIP(hipGetLastError());
-    {
-        GemmArgs a{};  // X2[z][y][(kx, c)] = sum_ky By[ky][y] G[z][ky][(kx, c)]
-        a.A = sp.d_by; a.sam = 1; a.sak = Y; a.sab = 0;
-        a.B = sp.d_gm; a.sbk = 32L * C; a.sbn = 1; a.sbb = (long)sp.nmode * C;
-        a.C = sp.d_x2; a.scm = 32L * C; a.scn = 1; a.scb = (long)Y * 32 * C;
-        a.M = Y; a.N = 32 * C; a.K = 32;
-        if (gemm<unsigned short, float, float>(a, Z, s)) return 1;
-    }
-    {
-        GemmArgs a{};  // S[z][y][x][c] = sum_kx Bx[kx][x] X2[z][y][kx][c]  (BF16 NHWC)
-        a.A = sp.d_bx; a.sam = 1; a.sak = X; a.sab = 0;
-        a.B = sp.d_x2; a.sbk = C; a.sbn = 1; a.sbb = 32L * C;
-        a.C = sp.d_spec; a.scm = C; a.scn = 1; a.scb = (long)X * C;
-        a.M = X; a.N = C; a.K = 32;
-        if (gemm<unsigned short, float, unsigned short>(a, Z * Y, s)) return 1;
-    }
-    hipLaunchKernelGGL((sur_conv_kernel<0>), cgrid, blk, conv_lds, s, sp.d_h1, sp.d_w2, sp.d_b2, sp.d_spec, sp.d_h0,
-                       (const float *)nullptr, 0.0f, (float *)nullptr, 0.0f, X, Y);
+    MMS_HIP(hipMemcpyAsync(grad + OFF_WP, sp.d_hsum, sizeof(float) * (C + 1), hipMemcpyDeviceToDevice, s));
+    // conv3: dW3 from (dz3, h2); dz2 = convT(dz3) GELU'(z2)
+    if (launch_wgrad(sp, sp.d_dza, sp.d_h2, grad + OFF_W3, s)) return 1;
+    ConvArgs a{};
+    a.xd = g.xd; a.yd = g.yd;
+    a.in = sp.d_dza; a.w = sp.d_wt[2]; a.zin = sp.d_z[2]; a.out = sp.d_dzb;
+    if (launch_conv(2, sp, a, s)) return 1;
+    // conv2: dW2 from (dz2, h1); the spectral branch: dWm and its data gradient
+    if (launch_wgrad(sp, sp.d_dzb, sp.d_h1, grad + OFF_W2, s)) return 1;
+    if (spectral(sp, sp.d_dzb, sp.d_gm, sp.d_dh, true, sp.d_spec, s)) return 1;
+    hipLaunchKernelGGL(sur_mix_wgrad_kernel, dim3(sp.nmode), dim3(256), 0, s, sp.d_gm, sp.d_hm, grad + OFF_WM,
+                       sp.nmode, g.zd);
     MMS_HIP(hipGetLastError());
-    hipLaunchKernelGGL((sur_conv_kernel<1>), cgrid, blk, conv_lds, s, sp.d_h0, sp.d_w3, sp.d_b3,
-                       (const unsigned short *)nullptr, (unsigned short *)nullptr, sp.d_wp, sp.bp, phi, oscale, X, Y);
+    // dz1 = (convT(dz2) + spectral^T(dz2)) GELU'(z1)
+    a.in = sp.d_dzb; a.w = sp.d_wt[1]; a.add = sp.d_spec; a.zin = sp.d_z[1]; a.out = sp.d_dza;
+    if (launch_conv(2, sp, a, s)) return 1;
+    // conv1: dW1 from (dz1, h0); dz0 = convT(dz1) GELU'(z0)
+    if (launch_wgrad(sp, sp.d_dza, sp.d_h0, grad + OFF_W1, s)) return 1;
+    a.in = sp.d_dza; a.w = sp.d_wt[0]; a.add = nullptr; a.zin = sp.d_z[0]; a.out = sp.d_dzb;
+    if (launch_conv(2, sp, a, s)) return 1;
+    // lift
+    const int ntile = ((g.xd + TS - 1) / TS) * ((g.yd + TS - 1) / TS) * g.zd;
+    const int nb = ntile < SurrogatePlan::kPartBlocks ? ntile : SurrogatePlan::kPartBlocks;
+    hipLaunchKernelGGL(sur_lift_wgrad_kernel, dim3(nb), dim3(256), 0, s, rho, sp.d_dzb, sp.d_wpart, g.xd, g.yd, g.zd,
+                       sp.zf, rscale);
     MMS_HIP(hipGetLastError());
-    (void)npix;
+    hipLaunchKernelGGL(sur_sum_kernel, dim3((unsigned)((LGRAD_PART + 255) / 256)), dim3(256), 0, s, sp.d_wpart,
+                       LGRAD_PART, nb, LGRAD_PART, grad + OFF_L);
+    MMS_HIP(hipGetLastError());
+    return 0;
+}
+
+int surrogate_adam(SurrogatePlan &sp, float *m, float *v, const float *grad, long step, float lr, hipStream_t s) {
+    const double b1 = 0.9, b2 = 0.999;
+    const float c1 = (float)(1.0 / (1.0 - pow(b1, (double)step))), c2 = (float)(1.0 / (1.0 - pow(b2, (double)step)));
+    hipLaunchKernelGGL(sur_adam_kernel, dim3((unsigned)((NPARAM + 255) / 256)), dim3(256), 0, s, sp.d_param, m, v,
+                       grad, NPARAM, lr, (float)b1, (float)b2, c1, c2, 1e-8f);
+    MMS_HIP(hipGetLastError());
+    return sp.pack(s);
+}
+
+int surrogate_sqnorm(SurrogatePlan &sp, const float *x, long n, double *out, hipStream_t s) {
+    // the residual kernel of x against itself: its second partials are sum x^2
+    hipLaunchKernelGGL(sur_residual_kernel, dim3(1024), dim3(256), 0, s, x, x, n, sp.d_part);
+    MMS_HIP(hipGetLastError());
+    std::vector<double> h(2048);
+    MMS_HIP(hipMemcpyAsync(h.data(), sp.d_part, sizeof(double) * 2048, hipMemcpyDeviceToHost, s));
+    MMS_HIP(hipStreamSynchronize(s));
+    double b2 = 0;
+    for (int i = 0; i < 1024; i++) b2 += h[2 * i + 1];
+    *out = b2;
     return 0;
 }
 

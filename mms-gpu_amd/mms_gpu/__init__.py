@@ -31,6 +31,7 @@ from .film import Film, SOLVER_DIRECT, SOLVER_FMM, M2L_FP32, M2L_BF16, M2L_BF16X
 from .film import M2L_FP16X2_BF16X3NEAR, M2L_FP16X2_FP16X2NEAR, SOLVER_DIRECT_REFGPU  # noqa: F401
 from .film import EXT_NONE, EXT_RAMP, EXT_CONST  # noqa: F401
 from .batch import FilmBatch  # noqa: F401
+from .surrogate import SurrogateTrainer, surrogate_state  # noqa: F401
 
 
 def _f32(a, name):

@@ -73,6 +73,18 @@ SIGNATURES = {
     "mms_surrogate_potential_dev": (I, [VP, VP, VP]),
     "mms_surrogate_mode": (I, [VP, I, F]),
     "mms_surrogate_stats": (I, [VP, C.POINTER(D), C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+    "mms_surrogate_state": (I, [I, I, I, F, C.c_ulonglong, C.c_long, f32p, VP, C.POINTER(I)]),
+    "mms_surrogate_trainer_create": (I, [C.POINTER(VP), I, I, I, F, F, I, VP]),
+    "mms_surrogate_trainer_destroy": (I, [VP]),
+    "mms_surrogate_trainer_generate": (I, [VP, I, C.c_ulonglong, I]),
+    "mms_surrogate_trainer_add": (I, [VP, I, f32p, f32p]),
+    "mms_surrogate_trainer_pairs": (C.c_long, [VP]),
+    "mms_surrogate_trainer_get_pair": (I, [VP, C.c_long, VP, VP]),
+    "mms_surrogate_trainer_gradient": (I, [VP, i64p, VP, C.POINTER(D)]),
+    "mms_surrogate_trainer_step": (I, [VP, i64p, F, C.POINTER(D)]),
+    "mms_surrogate_trainer_eval": (I, [VP, i64p, f64p]),
+    "mms_surrogate_trainer_predict": (I, [VP, i64p, f32p]),
+    "mms_surrogate_trainer_weights": (I, [VP, f32p]),
     "mms_batch_create": (I, [C.POINTER(VP), C.POINTER(MmsParams), I, VP, VP]),
     "mms_batch_destroy": (I, [VP]),
     "mms_batch_set_m": (I, [VP, f32p]),

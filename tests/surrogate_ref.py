@@ -53,15 +53,24 @@ def basis(n):
     return _bf(torch.from_numpy(np.sqrt(2.0 / n) * np.stack(rows)).to(torch.float32))
 
 
-def forward(weights, rho, mat, dims, rscale, oscale):
-    """rho [zd*yd*xd] float32, mat uint8 or None -> potential [zd*yd*xd]."""
-    xd, yd, zd = dims
-    p = unpack(weights)
-    r = torch.from_numpy(np.asarray(rho, np.float32).reshape(zd, yd, xd)) * np.float32(rscale)
-    m = (torch.ones(zd, yd, xd) if mat is None
-         else torch.from_numpy((np.asarray(mat).reshape(zd, yd, xd) != 0).astype(np.float32)))
-    lidx = torch.arange(zd, dtype=torch.float32).reshape(zd, 1, 1).expand(zd, yd, xd) / zd
-    x = _bf(torch.stack([r, m, lidx], 1))                                  # [zd, 3, yd, xd]
+def planes(rho, zf, rscale):
+    """Input planes of every layer of films of zf layers: [Z, 3, y, x] (own
+    charge, layers at distance 1, layers at distance >= 2), times rscale."""
+    Z = rho.shape[0]
+    out = torch.zeros((Z, 3) + tuple(rho.shape[1:]), dtype=torch.float32)
+    for z in range(Z):
+        f0 = (z // zf) * zf
+        for k in range(zf):
+            d = abs(k - (z - f0))
+            out[z, min(d, 2)] += rho[f0 + k]
+    return out * np.float32(rscale)
+
+
+def forward_t(p, rho, zf, rscale, oscale):
+    """Differentiable forward pass: p the unpacked parameters (tensors), rho
+    [Z, y, x] float32 -> potential [Z, y, x]."""
+    Z, yd, xd = rho.shape
+    x = _bf(planes(rho, zf, rscale))
     conv = torch.nn.functional.conv2d
     h0 = _bf(gelu(conv(x, _bf(p["lift"]), p["blift"], padding=1)))
     h1 = _bf(gelu(conv(h0, _bf(p["w1"]), p["b1"], padding=1)))
@@ -75,5 +84,45 @@ def forward(weights, rho, mat, dims, rscale, oscale):
     S = _bf(torch.einsum("kx,zyko->zyxo", Bx, _bf(X2))).permute(0, 3, 1, 2)  # [z, c, y, x]
     h2 = _bf(gelu(conv(h1, _bf(p["w2"]), p["b2"], padding=1) + S))
     h3 = _bf(gelu(conv(h2, _bf(p["w3"]), p["b3"], padding=1)))
-    out = (torch.einsum("zcyx,c->zyx", h3, p["wp"]) + p["bp"]) * np.float32(oscale)
+    return (torch.einsum("zcyx,c->zyx", h3, p["wp"]) + p["bp"]) * np.float32(oscale)
+
+
+def forward(weights, rho, mat, dims, rscale, oscale):
+    """rho [zd*yd*xd] float32 of one film -> potential [zd*yd*xd] (mat is
+    not an input of the network)."""
+    xd, yd, zd = dims
+    p = unpack(weights)
+    r = torch.from_numpy(np.asarray(rho, np.float32).reshape(zd, yd, xd))
+    with torch.no_grad():
+        out = forward_t(p, r, zd, rscale, oscale)
     return out.reshape(-1).numpy()
+
+
+BLOCKS = ("lift", "blift", "w1", "b1", "w2", "b2", "mode", "w3", "b3", "wp", "bp")
+
+
+def loss_and_grad(weights, rho, phi, dims, nfilm, rscale, oscale):
+    """Loss mean_f ||phi_s - phi||^2 / ||phi||^2 over nfilm films of dims
+    (rho, phi [nfilm * n]) and its gradient in the API weight order."""
+    xd, yd, zd = dims
+    p = {k: v.clone().requires_grad_(True) for k, v in unpack(weights).items()}
+    r = torch.from_numpy(np.asarray(rho, np.float32).reshape(nfilm * zd, yd, xd))
+    t = torch.from_numpy(np.asarray(phi, np.float32).reshape(nfilm, -1))
+    out = forward_t(p, r, zd, rscale, oscale).reshape(nfilm, -1)
+    inv = torch.from_numpy(np.array([1.0 / np.sum(np.asarray(phi, np.float64).reshape(nfilm, -1)[f] ** 2)
+                                     for f in range(nfilm)], np.float32))
+    loss = (((out - t) ** 2).sum(1) * inv).mean()
+    loss.backward()
+    g = np.concatenate([p[k].grad.reshape(-1).numpy() for k in BLOCKS]).astype(np.float32)
+    return float(loss.detach()), g
+
+
+def block_slices():
+    """(name, slice) of each block of the API weight vector."""
+    sizes = {"lift": C * 27, "blift": C, "w1": C * C * 9, "b1": C, "w2": C * C * 9, "b2": C,
+             "mode": 32 * 32 * C * C, "w3": C * C * 9, "b3": C, "wp": C, "bp": 1}
+    o, out = 0, []
+    for k in BLOCKS:
+        out.append((k, slice(o, o + sizes[k])))
+        o += sizes[k]
+    return out

@@ -324,15 +324,54 @@ def surrogate_flops_per_eval(side, zd):
     return 3 * conv + lift + 2 * 64 * n + spec
 
 
-def surrogate_timing(mg, wl, w, args, plain_sps):
+def surrogate_training(mg, wl, args):
+    """Training of the surrogate on its own workload: args.sur_pairs seeded
+    128 x 128 x 3 film states (disc / rectangle / ring masks, vortex or domain
+    states) with their FMM (P = 6) potentials generated on the GPU, the last
+    512 held out; args.sur_steps Adam steps of 8 films (rate 1e-3, 3e-4 over
+    the last third) over a seeded shuffle.  Returns (weights, dict)."""
+    dims, B, hold = (128, 128, 3), 8, 512
+    t = mg.SurrogateTrainer(*dims, wl.CELL, wl.MS, B)
+    t0 = time.perf_counter()
+    t.generate(args.sur_pairs, 2010, P=6)
+    gen_s = time.perf_counter() - t0
+    ntrain = args.sur_pairs - hold
+    rng = np.random.default_rng(2010)
+    held = [np.arange(ntrain + B * k, ntrain + B * k + B) for k in range(hold // B)]
+    before = float(np.mean([t.eval(b) for b in held]))
+    order = np.zeros(0, np.int64)
+    losses = []
+    t0 = time.perf_counter()
+    for s in range(args.sur_steps):
+        if order.size < B:
+            order = np.concatenate([order, rng.permutation(ntrain)])
+        idx, order = order[:B], order[B:]
+        losses.append(t.step(idx, 1e-3 if s < 2 * args.sur_steps // 3 else 3e-4))
+    train_s = time.perf_counter() - t0
+    after = [float(np.mean(t.eval(b))) for b in held]
+    w = t.weights()
+    t.close()
+    out = {"workload": "128x128x3 seeded states, FMM P=6 potentials", "pairs": args.sur_pairs,
+           "held_out": hold, "batch": B, "steps": args.sur_steps, "generate_s": round(gen_s, 2),
+           "pairs_generated_per_s": round(args.sur_pairs / gen_s, 1),
+           "ms_per_step": round(1e3 * train_s / max(1, args.sur_steps), 3),
+           "films_per_s": round(B * args.sur_steps / train_s, 1),
+           "train_loss_last_100": round(float(np.mean(losses[-100:])), 5) if losses else None,
+           "held_out_residual_before": round(before, 4), "held_out_residual": round(float(np.mean(after)), 4)}
+    log(f"surrogate training: {args.sur_pairs} pairs in {gen_s:.1f} s; {args.sur_steps} steps at "
+        f"{out['ms_per_step']} ms; held-out residual {before:.3f} -> {out['held_out_residual']:.4f}")
+    return w, out
+
+
+def surrogate_timing(mg, wl, w, args, plain_sps, weights=None):
     """The learned surrogate on the same film: one inference timed with HIP
     events beside one solver evaluation; RK4 steps/s with the surrogate at
     stages 2-3 (and the stage-1 judge evaluation) in every step; the judged
     protocol (stages 2-3 take the surrogate only when its stage-1 residual
-    is within 1e-3, else the solver); the residual of the (untrained)
-    network.  Returns a dict for the JSON line."""
+    is within 1e-3, else the solver); the residual of the network (trained
+    when weights are given).  Returns a dict for the JSON line."""
     f = make_film(mg, wl, w, args.P, args.m2l, args.leaf)
-    f.surrogate_init()
+    f.surrogate_init(weights)
     ch = wl.charge_of(w.M, w.dims, w.d)
     f.profile_begin()
     for _ in range(3):
@@ -349,7 +388,7 @@ def surrogate_timing(mg, wl, w, args, plain_sps):
     st = f.surrogate_stats()
     f.close()
     fl = surrogate_flops_per_eval(w.xdim, w.zdim)
-    out = {"trained": False, "inference_ms": round(sur_ms, 4), "solver_eval_ms": round(fmm_ms, 4),
+    out = {"trained": weights is not None, "inference_ms": round(sur_ms, 4), "solver_eval_ms": round(fmm_ms, 4),
            "inference_tflops": round(fl / (sur_ms * 1e-3) / 1e12, 2), "inference_flops": fl,
            "inference_frac_bf16_peak": round(fl / (sur_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4),
            "steps_per_s_stages23": round(sps23, 3), "ratio_stages23": round(sps23 / plain_sps, 4),
@@ -465,7 +504,9 @@ def main():
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the timing of the other M2L precisions")
     ap.add_argument("--graph", type=int, default=1)
-    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate timing")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate legs")
+    ap.add_argument("--sur-pairs", type=int, default=20000, help="surrogate training pairs (128x128x3)")
+    ap.add_argument("--sur-steps", type=int, default=2500, help="surrogate Adam steps of 8 films (0: untrained)")
     ap.add_argument("--workload", default="c3", choices=["c3", "c5"],
                     help="c3: the 1024x1024x4 film (default); c5: the 512-film sweep")
     ap.add_argument("--serial", action="store_true",
@@ -563,7 +604,11 @@ def main():
 
     sur = None
     if not args.no_surrogate:
-        sur = surrogate_timing(mg, wl, w, args, steps_per_s)
+        weights, training = None, None
+        if args.sur_steps > 0:
+            weights, training = surrogate_training(mg, wl, args)
+        sur = surrogate_timing(mg, wl, w, args, steps_per_s, weights)
+        sur["training"] = training
 
     cpu = None
     if not args.no_cpu:
@@ -596,10 +641,11 @@ def main():
                    "precision_split": "FP32 fields and stage M, FP64 RK4 combine and renormalisation"},
         "surrogate_ratio": sur["ratio_judged"] if sur else None,
         "surrogate_note": ("the metric's ratio: surrogate+FMM steps/s (stage-1 judge, fallback to the FMM when "
-                           "the residual exceeds 1e-3) over plain-FMM steps/s; the network is not trained, so "
-                           "the judge rejects it and every step falls back; `surrogate.ratio_stages23` is the "
-                           "ratio with the surrogate used at stages 2-3 unconditionally (an upper bound on any "
-                           "trained network of this size); `value` is the plain GPU FMM form"),
+                           "the residual exceeds 1e-3) over plain-FMM steps/s, with the network trained in this "
+                           "run (`surrogate.training`); the judge rejects it when its residual on this film is "
+                           "above 1e-3 and the step falls back; `surrogate.ratio_stages23` is the ratio with the "
+                           "surrogate used at stages 2-3 unconditionally (an upper bound on any network of this "
+                           "size, whatever its accuracy); `value` is the plain GPU FMM form"),
         "surrogate": sur,
         "roofline": roof,
         "kernels": others,

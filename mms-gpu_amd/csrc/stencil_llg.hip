@@ -10,8 +10,9 @@
 // Layout: M stage values are FP32 SoA planes [3][n] (x plane, y plane, z plane),
 // the RK4 base M_n and the slope accumulator are FP64 SoA [3][n].  Each thread
 // owns one (x,y) column and walks z, keeping the z-1/z/z+1 values in registers;
-// in-plane neighbours come from the caches (blocks are 64 x 4 columns, so the
-// y-neighbour rows of a block are resident in L1/L2).
+// in-plane neighbours come from the caches (blocks are 64 x 4 columns, or
+// 256 x 8 for the four-cells-per-thread stage kernel, so the y-neighbour rows
+// of a block are resident in L1/L2).
 // Out-of-grid neighbours read as 0 (the reference's zero padding: smeared
 // surface charge, Dirichlet-zero exchange).
 #include <cstdlib>
@@ -22,6 +23,10 @@ namespace mms {
 namespace {
 
 constexpr int BX = 64, BY = 4;
+// rows per block of the four-cells-per-thread stage kernel (measured on the
+// 1024 x 1024 x 4 film: 2, 4, 8 and 16 rows of 64 threads gave 54.9, 54.9,
+// 51.6 and 58.6 us per launch; 128 x 4 and 32 x 16 threads 53.1 and 52.9)
+constexpr int BY4 = 8;
 
 // XCD-aware tile order for a 1D grid of 8 * ceil(ntiles / 8) workgroups.
 // Workgroup i runs on XCD i mod 8 (round-robin dispatch); it takes logical
@@ -546,11 +551,11 @@ __device__ __forceinline__ void stage_column4(const StageArgs &A, const float *_
 }
 
 template <int ZD, int ST>
-__global__ __launch_bounds__(BX * BY) void stage4_kernel(StageArgs A) {
-    const int ntx = (A.xd + 4 * BX - 1) / (4 * BX), nty = (A.yd + BY - 1) / BY;
+__global__ __launch_bounds__(BX * BY4) void stage4_kernel(StageArgs A) {
+    const int ntx = (A.xd + 4 * BX - 1) / (4 * BX), nty = (A.yd + BY4 - 1) / BY4;
     int tx, ty;
     if (!xcd_tile(ntx, ntx * nty, tx, ty)) return;
-    const int x = 4 * (tx * BX + threadIdx.x), y = ty * BY + threadIdx.y;
+    const int x = 4 * (tx * BX + threadIdx.x), y = ty * BY4 + threadIdx.y;
     if (x >= A.xd || y >= A.yd) return;
     stage_column4<ZD, ST>(A, A.m_in, A.m_out, A.mn, A.acc, A.phi, A.matl, A.m_base, x, y, A.xd, A.yd, A.zd,
                           (long)A.xd * A.yd * A.zd);
@@ -792,12 +797,12 @@ int launch_stage_ex(const Grid &g, const Material &mat, int stage, double dt, co
     const char *one = std::getenv("MMS_GPU_STAGE_ONE_CELL");
     if (plain && !(one && one[0] == '1') && g.xd % 4 == 0 && (al & 15) == 0 &&
         (reinterpret_cast<uintptr_t>(matl) & 3) == 0) {
-        const long ntiles = (long)((g.xd + 4 * BX - 1) / (4 * BX)) * ((g.yd + BY - 1) / BY);
-        dim3 grid4((unsigned)(8 * ((ntiles + 7) / 8)));
+        const long ntiles = (long)((g.xd + 4 * BX - 1) / (4 * BX)) * ((g.yd + BY4 - 1) / BY4);
+        dim3 grid4((unsigned)(8 * ((ntiles + 7) / 8))), block4(BX, BY4);
         switch (g.zd) {
-        case 3: launch_stage4_zd<3>(grid4, block, s, A); break;
-        case 4: launch_stage4_zd<4>(grid4, block, s, A); break;
-        default: launch_stage4_zd<0>(grid4, block, s, A);
+        case 3: launch_stage4_zd<3>(grid4, block4, s, A); break;
+        case 4: launch_stage4_zd<4>(grid4, block4, s, A); break;
+        default: launch_stage4_zd<0>(grid4, block4, s, A);
         }
         MMS_HIP(hipGetLastError());
         return 0;

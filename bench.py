@@ -51,15 +51,17 @@ def m2l_flops_per_eval(side, zd, P, w, m2l):
     blocks.  Real embedding: only coefficients with n + m even are kept, K =
     (P+1)(P+2)/2 reals per box.  Returns (useful, issued): useful counts
     2 K^2 per block product; issued counts the MFMA work actually run, blocks
-    padded to KP = 32 ceil(K / 32) and three BF16 products per block in the
-    split (bf16x3) form."""
+    padded to KP = 16 (K <= 16) or 32 ceil(K / 32), three BF16 products per
+    block in the split (bf16x3) form, two in the FP16 hi/lo forms; the 16-bit
+    forms of 16-slot blocks run the 27 offsets as 14 pairs (one zero block)."""
     K = (P + 1) * (P + 2) // 2
-    KP = 32 * ((K + 31) // 32)
+    KP = 16 if K <= 16 else 32 * ((K + 31) // 32)
     Lleaf = int(math.log2(side)) - int(math.log2(w))
     blocks = sum((4 ** L) * zd * 27 * zd for L in range(2, Lleaf + 1))
     products = {"bf16x3": 3, "bf16x3_f16near": 3, "fp16x2": 2, "fp16x2_bf16x3near": 2,
                 "fp16x2_f16x2near": 2}.get(m2l, 1)
-    return blocks * 2 * K * K, blocks * 2 * KP * KP * products
+    issued_blocks = blocks * 28 / 27 if (KP == 16 and m2l != "fp32") else blocks
+    return blocks * 2 * K * K, issued_blocks * 2 * KP * KP * products
 
 
 def fmm_pass_bytes(side, zd, P, w):

@@ -60,7 +60,8 @@ __device__ __forceinline__ f32x4 mfma32(f16x8 a, f16x8 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
-template <int K> struct KPad { static constexpr int KP = ((K + 31) / 32) * 32; };
+// coefficient slots per box: 16 when K <= 16 (P <= 4), else K rounded up to 32
+template <int K> struct KPad { static constexpr int KP = K <= 16 ? 16 : ((K + 31) / 32) * 32; };
 template <int P> struct NC { static constexpr int K = (P + 1) * (P + 2) / 2; };
 
 __device__ inline unsigned short f2bf(float x) {  // round to nearest even
@@ -587,11 +588,15 @@ __global__ __launch_bounds__(256) void down_coarse_kernel(const float *__restric
 // px + 2 py: the 6 x 6 block [-2-px, 3-px] x [-2-py, 3-py] of source boxes
 // minus the 3 x 3 neighbourhood (children of the parent's neighbours minus
 // own neighbours, Box.cu:121-145).
-__constant__ int c_m2l_offs[4][27] = {  // 32-bit entries: wave-uniform scalar loads
-    {17, 18, 19, 20, 21, 22, 33, 37, 38, 49, 53, 54, 65, 69, 70, 81, 82, 83, 84, 85, 86, 97, 98, 99, 100, 101, 102},
-    {16, 17, 18, 19, 20, 21, 32, 33, 37, 48, 49, 53, 64, 65, 69, 80, 81, 82, 83, 84, 85, 96, 97, 98, 99, 100, 101},
-    {1, 2, 3, 4, 5, 6, 17, 18, 19, 20, 21, 22, 33, 37, 38, 49, 53, 54, 65, 69, 70, 81, 82, 83, 84, 85, 86},
-    {0, 1, 2, 3, 4, 5, 16, 17, 18, 19, 20, 21, 32, 33, 37, 48, 49, 53, 64, 65, 69, 80, 81, 82, 83, 84, 85}};
+// The 28th entry repeats the 27th: the second offset of the last pair in the
+// 16-slot form, whose operator block is zero.
+#define MMS_M2L_OFFSETS                                                                                           \
+    {{17, 18, 19, 20, 21, 22, 33, 37, 38, 49, 53, 54, 65, 69, 70, 81, 82, 83, 84, 85, 86, 97, 98, 99, 100, 101, 102, 102}, \
+     {16, 17, 18, 19, 20, 21, 32, 33, 37, 48, 49, 53, 64, 65, 69, 80, 81, 82, 83, 84, 85, 96, 97, 98, 99, 100, 101, 101}, \
+     {1, 2, 3, 4, 5, 6, 17, 18, 19, 20, 21, 22, 33, 37, 38, 49, 53, 54, 65, 69, 70, 81, 82, 83, 84, 85, 86, 86},           \
+     {0, 1, 2, 3, 4, 5, 16, 17, 18, 19, 20, 21, 32, 33, 37, 48, 49, 53, 64, 65, 69, 80, 81, 82, 83, 84, 85, 85}}
+__constant__ int c_m2l_offs[4][28] = MMS_M2L_OFFSETS;  // 32-bit entries: wave-uniform scalar loads
+const int h_m2l_offs[4][28] = MMS_M2L_OFFSETS;          // the same table for the host's operator layout
 
 template <int KP, int MODE, int NC> struct M2LCfg {
     static constexpr int G = KP / 4;    // slots per lane group (lanes 16g..16g+15)
@@ -666,12 +671,28 @@ __global__ __launch_bounds__(256 * ZP) void m2l_mfma_kernel(const float *__restr
     // operator fragments of the first three offsets, in flight while the window is staged
     constexpr bool F16M = (MODE == MMS_M2L_FP16 || MODE == MMS_M2L_FP16X2);  // scaled FP16 source window
     constexpr int NP = (MODE == MMS_M2L_BF16X3 || MODE == MMS_M2L_FP16X2) ? 2 : 1;  // operator planes (hi, lo)
-    constexpr int NK = (MODE == MMS_M2L_FP32) ? G / 4 : G / 8;  // 16-byte A loads per row tile and plane
+    // 16-slot blocks with 16-bit operands: the 32-deep k step of one MFMA takes
+    // two offsets (lane groups 0-1 the first, 2-3 the second): 14 offset pairs
+    // (the last with a zero second block) in place of 27 offsets
+    constexpr bool PAIR = (KP == 16 && MODE != MMS_M2L_FP32);
+    constexpr int NU = PAIR ? 14 : 27, RG = PAIR ? 2 : 3;  // product units; operator blocks in flight
+    constexpr int NK = (MODE == MMS_M2L_FP32) ? G / 4 : (PAIR ? 1 : G / 8);  // 16-byte A loads per row tile and plane
     typedef typename std::conditional<F16M, f16x8, bf16x8>::type BV;  // 8 x 16-bit operand
     typedef typename std::conditional<MODE == MMS_M2L_FP32, f32x4, BV>::type AV;
-    AV abuf[3][NP][RT][NK];
-    const long lo_base = (long)RT * 64 * G * 49 * (2 * zd - 1);  // operator lo plane offset (elements)
-    auto loadA = [&](AV (&a)[NP][RT][NK], int code) {
+    AV abuf[RG][NP][RT][NK];
+    // operator lo plane offset (elements); the pair form's fragments are
+    // [dz][child parity][pair][64 lanes][8]
+    const long lo_base = PAIR ? (long)(2 * zd - 1) * 4 * 14 * 64 * 8 : (long)RT * 64 * G * 49 * (2 * zd - 1);
+    auto loadA = [&](AV (&a)[NP][RT][NK], int u) {
+        if constexpr (PAIR) {
+            const long fb = (((long)dzi * 4 + wave) * 14 + u) * 64 * 8;
+#pragma unroll
+            for (int q = 0; q < NP; q++)
+                a[q][0][0] = *reinterpret_cast<const AV *>(reinterpret_cast<const unsigned short *>(Tfrag) + q * lo_base +
+                                                            fb + lane * 8);
+            return;
+        }
+        const int code = offs[u];
         const long fbase = ((long)dzi * 49 + (code >> 4) * 7 + (code & 15)) * RT;
 #pragma unroll
         for (int q = 0; q < NP; q++)
@@ -688,7 +709,7 @@ __global__ __launch_bounds__(256 * ZP) void m2l_mfma_kernel(const float *__restr
                 }
     };
 #pragma unroll
-    for (int k = 0; k < 3; k++) loadA(abuf[k], offs[k]);
+    for (int k = 0; k < RG; k++) loadA(abuf[k], k);
 
     // stage the source window of layer zs (zero outside the grid), in
     // batches of 8 loads per thread issued before any of them is used
@@ -813,13 +834,23 @@ __global__ __launch_bounds__(256 * ZP) void m2l_mfma_kernel(const float *__restr
     __syncthreads();
     if (zt >= zd) return;  // padding waves of the last layer group (after the only barrier)
 
-    // 27 offsets, three operator blocks in flight (ring of registers)
+    // 27 offsets (14 offset pairs), RG operator blocks in flight (ring of registers)
 #pragma unroll 1
-    for (int ob = 0; ob < 27; ob += 3) {
+    for (int ob = 0; ob < NU; ob += RG) {
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const int code = offs[ob + k];
-            const int woff = ((code >> 4) - 3) * WX + ((code & 15) - 3);
+        for (int k = 0; k < RG; k++) {
+            // window offset of the source box; the pair form: lane groups 0-1
+            // read the first offset's box, 2-3 the second's, at slots 8 (g % 2)
+            int woff, slot;
+            if constexpr (PAIR) {
+                const int ca = offs[2 * (ob + k)], cb = offs[2 * (ob + k) + 1], code = (g < 2) ? ca : cb;
+                woff = ((code >> 4) - 3) * WX + ((code & 15) - 3);
+                slot = 8 * (g & 1);
+            } else {
+                const int code = offs[ob + k];
+                woff = ((code >> 4) - 3) * WX + ((code & 15) - 3);
+                slot = g * G;
+            }
             if constexpr (MODE == MMS_M2L_FP32) {
                 const float *lds = reinterpret_cast<const float *>(smem);
 #pragma unroll
@@ -827,7 +858,7 @@ __global__ __launch_bounds__(256 * ZP) void m2l_mfma_kernel(const float *__restr
                     f32x4 b[NC];
 #pragma unroll
                     for (int c = 0; c < NC; c++)
-                        b[c] = *reinterpret_cast<const f32x4 *>(lds + (tbox[c] + woff) * SB + g * G + 4 * u4);
+                        b[c] = *reinterpret_cast<const f32x4 *>(lds + (tbox[c] + woff) * SB + slot + 4 * u4);
 #pragma unroll
                     for (int u = 0; u < 4; u++)
 #pragma unroll
@@ -843,7 +874,7 @@ __global__ __launch_bounds__(256 * ZP) void m2l_mfma_kernel(const float *__restr
                     BV bh[NC];
 #pragma unroll
                     for (int c = 0; c < NC; c++)
-                        bh[c] = *reinterpret_cast<const BV *>(hi + (tbox[c] + woff) * SB + g * G + 8 * sk);
+                        bh[c] = *reinterpret_cast<const BV *>(hi + (tbox[c] + woff) * SB + slot + 8 * sk);
 #pragma unroll
                     for (int r = 0; r < RT; r++)
 #pragma unroll
@@ -860,7 +891,7 @@ __global__ __launch_bounds__(256 * ZP) void m2l_mfma_kernel(const float *__restr
                         bf16x8 bl[NC];
 #pragma unroll
                         for (int c = 0; c < NC; c++)
-                            bl[c] = *reinterpret_cast<const bf16x8 *>(hi + WX * WY * SB + (tbox[c] + woff) * SB + g * G + 8 * sk);
+                            bl[c] = *reinterpret_cast<const bf16x8 *>(hi + WX * WY * SB + (tbox[c] + woff) * SB + slot + 8 * sk);
 #pragma unroll
                         for (int r = 0; r < RT; r++)
 #pragma unroll
@@ -874,7 +905,7 @@ __global__ __launch_bounds__(256 * ZP) void m2l_mfma_kernel(const float *__restr
                     }
                 }
             }
-            if (ob + 3 < 27) loadA(abuf[k], offs[ob + 3 + k]);
+            if (ob + RG < NU) loadA(abuf[k], ob + RG + k);
         }
     }
     // D tile row 4g + q (q = 0..3), column j16: slots 16r + 4g .. +3 of target j16
@@ -1357,9 +1388,9 @@ __global__ __launch_bounds__(256) void p2p_16bit_kernel(const float *__restrict_
 template <int KP, int MODE, int NC>
 int launch_m2l(const FmmPlan &pl, hipStream_t s) {
     typedef M2LCfg<KP, MODE, NC> C;
-    // target layers per staged source window (two for 32-coefficient blocks;
-    // 64-coefficient blocks need the registers of a 256-thread workgroup)
-    constexpr int ZP = (KP == 32) ? 2 : 1;
+    // target layers per staged source window (two for 16- and 32-slot blocks;
+    // 64-slot blocks need the registers of a 256-thread workgroup)
+    constexpr int ZP = (KP <= 32) ? 2 : 1;
     static int attr_done = 0;
     if (!attr_done) {
         MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&m2l_mfma_kernel<KP, MODE, NC, ZP>),
@@ -1685,7 +1716,7 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
     const bool f16_ops = (m2l_mode == MMS_M2L_FP16 || m2l_mode == MMS_M2L_FP16X2);  // equilibrated FP16 operator
     m2l_cols = (cols_ == 8) ? 8 : 4;
     K = fmm_ncoef(P);
-    KP = ((K + 31) / 32) * 32;
+    KP = K <= 16 ? 16 : ((K + 31) / 32) * 32;
     const int side = g.xd > g.yd ? g.xd : g.yd;
     Lmax = 0;
     while ((1 << Lmax) < side) Lmax++;
@@ -1755,26 +1786,34 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
                 }
         if (upload((void **)&d_l2pf, l2pf.data(), sizeof(float) * l2pf.size())) return 1;
     }
-    // M2L fragments per level: [dz][49][blk]
+    // M2L fragments per level: [dz][49][blk]; 16-bit operands of 16-slot
+    // blocks: offset pairs [dz][child parity][14 pairs][64 lanes][8] (lane
+    // (m, g) holds row m, columns 8 (g % 2) .. + 7 of the pair's first block
+    // for g < 2, of its second for g >= 2)
     const size_t nblk = (size_t)(2 * zd - 1) * 49;
+    const bool pair = (KP == 16 && m2l_mode != MMS_M2L_FP32);
+    const size_t pair_elems = (size_t)(2 * zd - 1) * 4 * 14 * 64 * 8;
+    const size_t nplane = (m2l_mode == MMS_M2L_BF16X3 || m2l_mode == MMS_M2L_FP16X2) ? 2 : 1;
     m2l_level_bytes = (m2l_mode == MMS_M2L_FP32) ? nblk * blk * sizeof(float)
-                                                 : nblk * blk * sizeof(unsigned short) *
-                                                       ((m2l_mode == MMS_M2L_BF16X3 || m2l_mode == MMS_M2L_FP16X2) ? 2 : 1);
+                                                 : (pair ? pair_elems : nblk * blk) * sizeof(unsigned short) * nplane;
+    std::vector<std::vector<double>> pair_T;  // pair form: every level's operator blocks [dz][49][K][K]
     std::vector<char> all(m2l_level_bytes * (Lleaf - 1));
     std::vector<float> lv(nblk * blk);
     // fragment element i = (blk, rt, lane, u) holds Op[16 rt + lane % 16][G (lane / 16) + u]
     auto frow = [&](size_t i) { return (int)(16 * ((i / (64 * G)) % RT) + (i / G) % 16); };
     auto fcol = [&](size_t i) { return (int)(G * ((i / G) % 64 / 16) + i % G); };
     std::vector<float> fp16_levels;  // FP16: every level's FP32 fragments, scaled once all are known
+    std::vector<double> fp16_a, fp16_b;  // FP16: the operator's row and column factors
     for (int L = 2; L <= Lleaf; L++) {
         std::vector<double> T = ops.m2l_level(L, Lmax, zd, (double)g.ay, (double)g.az);
         for (size_t b = 0; b < nblk; b++) frag(&T[b * K * K], lv, b * blk);
         char *dst = all.data() + (size_t)(L - 2) * m2l_level_bytes;
+        if (pair) pair_T.push_back(T);
         if (m2l_mode == MMS_M2L_FP32) {
             memcpy(dst, lv.data(), m2l_level_bytes);
         } else if (f16_ops) {
             fp16_levels.insert(fp16_levels.end(), lv.begin(), lv.end());
-        } else {
+        } else if (!pair) {
             unsigned short *h = reinterpret_cast<unsigned short *>(dst);
             for (size_t i = 0; i < lv.size(); i++) {
                 h[i] = host_bf16(lv[i]);
@@ -1810,7 +1849,7 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
         // FP16X2: each level's lo plane (v - hi, FP16) follows its hi plane
         unsigned short *h = reinterpret_cast<unsigned short *>(all.data());
         const size_t per = lv.size(), np = (m2l_mode == MMS_M2L_FP16X2) ? 2 : 1;
-        for (size_t i = 0; i < fp16_levels.size(); i++) {
+        for (size_t i = 0; i < (pair ? 0 : fp16_levels.size()); i++) {
             const float v = (float)(fp16_levels[i] * a[frow(i)] * b[fcol(i)]);
             MMS_CHECK(std::fabs(v) < 65504.0f, "FP16 M2L operator out of range");
             const _Float16 hv = (_Float16)v;
@@ -1822,6 +1861,40 @@ int FmmPlan::init(const Grid &g, int P_, int w_, int precision_, int cols_, int 
             }
         }
         if (upload((void **)&d_m2l_scale, sc.data(), sizeof(float) * sc.size())) return 1;
+        fp16_a.assign(a.begin(), a.end());
+        fp16_b.assign(b.begin(), b.end());
+    }
+    if (pair) {
+        for (int L = 2; L <= Lleaf; L++) {
+            const std::vector<double> &T = pair_T[L - 2];
+            unsigned short *h = reinterpret_cast<unsigned short *>(all.data() + (size_t)(L - 2) * m2l_level_bytes);
+            for (int dzi = 0; dzi < 2 * zd - 1; dzi++)
+                for (int p = 0; p < 4; p++)
+                    for (int u = 0; u < 14; u++)
+                        for (int lane = 0; lane < 64; lane++)
+                            for (int t = 0; t < 8; t++) {
+                                const int m = lane % 16, gg = lane / 16, kk = 8 * (gg & 1) + t, idx = 2 * u + (gg >> 1);
+                                double v = 0.0;
+                                if (idx < 27 && m < K && kk < K) {
+                                    const int code = h_m2l_offs[p][idx], oi = (code >> 4) * 7 + (code & 15);
+                                    v = T[(((size_t)dzi * 49 + oi) * K + m) * K + kk];
+                                    if (f16_ops) v *= fp16_a[m] * fp16_b[kk];
+                                }
+                                const size_t o = ((((size_t)dzi * 4 + p) * 14 + u) * 64 + lane) * 8 + t;
+                                if (f16_ops) {
+                                    MMS_CHECK(std::fabs(v) < 65504.0, "FP16 M2L operator out of range");
+                                    const _Float16 hv = (_Float16)(float)v;
+                                    memcpy(&h[o], &hv, 2);
+                                    if (nplane == 2) {
+                                        const _Float16 lv16 = (_Float16)((float)v - (float)hv);
+                                        memcpy(&h[pair_elems + o], &lv16, 2);
+                                    }
+                                } else {
+                                    h[o] = host_bf16((float)v);
+                                    if (nplane == 2) h[pair_elems + o] = host_bf16((float)v - host_bf2f(h[o]));
+                                }
+                            }
+        }
     }
     if (upload((void **)&d_m2l, all.data(), all.size())) return 1;
     // near-field 1/r table over offsets dx, dy in [-(2w-1), 2w-1], dz in [-(zd-1), zd-1]

@@ -178,21 +178,24 @@ def test_fmm_linearity_and_sign(mg):
     assert _rel(pab, pa.astype(float) + pb) < 5e-6   # FP32 rounding of a + b and of the sums
 
 
+@pytest.mark.parametrize("P", [6, 4])
 @pytest.mark.parametrize("leaf", [4, 8])
 @pytest.mark.parametrize("mode,tol", [("fp32", 2e-6), ("bf16x3", 2e-5), ("bf16", 3e-3), ("fp16", 8e-4),
                                       ("bf16x3_f16near", 8e-4), ("fp16x2", 8e-4), ("fp16x2_bf16x3near", 8e-4),
                                       ("fp16x2_f16x2near", 8e-4)])
-def test_fmm_m2l_precision_modes(mg, mode, tol, leaf):
+def test_fmm_m2l_precision_modes(mg, mode, tol, leaf, P):
     """M2L on the matrix cores in each operand precision vs the FP64 FMM
     oracle (same P, leaf width, tree): FP32 to rounding; hi/lo BF16 split
     to ~2^-16 of the far field; plain BF16 to ~2^-8 and scaled FP16 to
     ~2^-11 of the far field.  At leaf width 8 the BF16 modes run the near
     field in the hi/lo BF16 split (three products), which stays at the
     split's ~2^-16; the FP16 mode and the BF16 split M2L with FP16 near
-    field run it in scaled FP16 (~2^-11)."""
+    field run it in scaled FP16 (~2^-11).  P = 4 (15 coefficients) takes the
+    16-slot blocks: the 16-bit forms pair two interaction offsets per 32-deep
+    MFMA k step."""
     from oracle import fmm_oracle as fo
     rng = np.random.default_rng(21)
-    xd, zd, P = 128, 4, 6
+    xd, zd = 128, 4
     q = (rng.normal(size=(zd, xd, xd)) * 1e14).astype(F)
     prec = {"fp32": mg.M2L_FP32, "bf16": mg.M2L_BF16, "bf16x3": mg.M2L_BF16X3, "fp16": mg.M2L_FP16,
             "bf16x3_f16near": mg.M2L_BF16X3_FP16NEAR, "fp16x2": mg.M2L_FP16X2,
@@ -201,7 +204,25 @@ def test_fmm_m2l_precision_modes(mg, mode, tol, leaf):
         pot = f.potential(q.ravel())
     ref = fo.FmmOracle(xd, xd, zd, P, w=leaf).potential_units(q.astype(float)).ravel() * float(D5[0]) ** 2
     err = _rel(pot, ref)
-    _say(f"m2l {mode} leaf {leaf}: rel err vs FP64 FMM {err:.2e}")
+    _say(f"m2l {mode} leaf {leaf} P {P}: rel err vs FP64 FMM {err:.2e}")
+    assert err < tol
+
+
+@pytest.mark.parametrize("P", [1, 2, 3, 4])
+@pytest.mark.parametrize("mode,tol", [("fp32", 2e-6), ("bf16x3", 2e-5), ("bf16", 3e-3), ("fp16x2", 8e-4)])
+def test_fmm_low_orders_sixteen_slots(mg, P, mode, tol):
+    """Orders 1-4 (3 to 15 coefficients) in 16-slot blocks, every M2L form
+    against the FP64 FMM oracle on a ragged three-layer film (leaf width 4)."""
+    from oracle import fmm_oracle as fo
+    rng = np.random.default_rng(40 + P)
+    xd, yd, zd = 64, 48, 3
+    q = (rng.normal(size=(zd, yd, xd)) * 1e14).astype(F)
+    prec = {"fp32": mg.M2L_FP32, "bf16x3": mg.M2L_BF16X3, "bf16": mg.M2L_BF16, "fp16x2": mg.M2L_FP16X2}[mode]
+    with mg.Film(xd, yd, zd, *D5, solver=mg.SOLVER_FMM, P=P, m2l_precision=prec, leaf_width=4) as f:
+        pot = f.potential(q.ravel())
+    ref = fo.FmmOracle(xd, yd, zd, P, w=4).potential_units(q.astype(float)).ravel() * float(D5[0]) ** 2
+    err = _rel(pot, ref)
+    _say(f"P {P} {mode}: rel err vs FP64 FMM {err:.2e}")
     assert err < tol
 
 

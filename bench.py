@@ -330,9 +330,12 @@ def surrogate_training(mg, wl, args):
     """Training of the surrogate on its own workload: args.sur_pairs seeded
     128 x 128 x 3 film states (disc / rectangle / ring masks, vortex or domain
     states) with their FMM (P = 6) potentials generated on the GPU, the last
-    512 held out; args.sur_steps Adam steps of 8 films (rate 1e-3, 3e-4 over
+    512 (at most a quarter) held out; args.sur_steps Adam steps of 8 films (rate 1e-3, 3e-4 over
     the last third) over a seeded shuffle.  Returns (weights, dict)."""
-    dims, B, hold = (128, 128, 3), 8, 512
+    dims, B = (128, 128, 3), 8
+    if args.sur_pairs < 2 * B:
+        raise SystemExit("--sur-pairs must be at least 16")
+    hold = min(512, (args.sur_pairs // 4) // B * B)  # held-out pairs, whole batches
     t = mg.SurrogateTrainer(*dims, wl.CELL, wl.MS, B)
     t0 = time.perf_counter()
     t.generate(args.sur_pairs, 2010, P=6)

@@ -152,6 +152,9 @@ int fmm_ncoef(int P);
 // grid's zd layers are zd / zf films of zf layers each.
 struct SurrogatePlan {
     static constexpr int kPartBlocks = 256;  // workgroups of the weight-gradient kernels
+    // loss / projection partials: per film kHeadBlocks blocks of kHeadPart
+    // floats (64 projection-weight gradients, the bias gradient, the loss)
+    static constexpr int kHeadBlocks = 64, kHeadPart = 66;
     Grid g{};
     int zf = 1;
     int nmode = 0;
@@ -177,11 +180,11 @@ void surrogate_param_to_api(const float *param, float *api);
 int surrogate_potential(SurrogatePlan &sp, const float *rho, float *phi, float rscale, float oscale, hipStream_t s,
                         Profiler *prof);
 // per-film losses sum (phi_s - phi_t)^2 inv_norm[f] into the head partials
-// (sp.d_hpart[f][64 blocks][66], entry 65); with grad also the projection's gradient
+// (sp.d_hpart[f][kHeadBlocks][kHeadPart], last entry); with grad also the projection's gradient
 int surrogate_losses(SurrogatePlan &sp, const float *phi_s, const float *phi_t, const float *inv_norm, float gscale,
                      float oscale, bool grad, hipStream_t s);
 // gradient (device layout) of gscale / 2 * sum_f loss_f after surrogate_potential on the same rho;
-// sp.d_hsum[65] receives sum_f loss_f
+// sp.d_hsum[kHeadPart - 1] receives sum_f loss_f
 int surrogate_backward(SurrogatePlan &sp, const float *rho, const float *phi_s, const float *phi_t,
                        const float *inv_norm, float gscale, float rscale, float oscale, float *grad, hipStream_t s);
 // Adam step `step` (from 1) at rate lr on sp.d_param with moments m, v; refreshes the BF16 copies

@@ -498,8 +498,9 @@ __global__ __launch_bounds__(256) void sur_sum_kernel(const float *__restrict__ 
 // oscale, dz3 = dproj wp GELU'(z3) and the partials of dwp = sum dproj
 // GELU(z3) (BF16-rounded as the forward projection reads it) and dbp = sum
 // dproj.  Grid (HB, films); thread = (pixel slot t / 8, channel block t % 8).
-// part[(f * HB + blk) * 66 + (0..63 dwp, 64 dbp, 65 loss)].
-constexpr int HB = 64, HPART = 66;
+// part[(f * HB + blk) * HPART + (0..63 dwp, 64 dbp, 65 loss)].
+constexpr int HB = SurrogatePlan::kHeadBlocks, HPART = SurrogatePlan::kHeadPart;
+static_assert(HPART == C + 2, "head partials: C projection-weight gradients, the bias gradient, the loss");
 __global__ __launch_bounds__(256) void sur_head_kernel(const float *__restrict__ phs, const float *__restrict__ pht,
                                                        const float *__restrict__ inv_norm, float gscale, float oscale,
                                                        const unsigned short *__restrict__ z3,

@@ -173,7 +173,8 @@ int gradient(mms_sur_trainer *t, const long *idx, double *loss) {
                            t->d_grad, t->stream))
         return 1;
     float lsum = 0;
-    MMS_HIP(hipMemcpyAsync(&lsum, t->sp.d_hsum + 65, sizeof(float), hipMemcpyDeviceToHost, t->stream));
+    MMS_HIP(hipMemcpyAsync(&lsum, t->sp.d_hsum + SurrogatePlan::kHeadPart - 1, sizeof(float), hipMemcpyDeviceToHost,
+                           t->stream));
     MMS_HIP(hipStreamSynchronize(t->stream));
     if (loss) *loss = (double)lsum / t->batch;
     return 0;
@@ -310,13 +311,13 @@ extern "C" int mms_surrogate_trainer_eval(mms_sur_trainer *t, const long *idx, d
     MMS_CHECK(t && residual, "bad argument");
     if (forward_batch(t, idx)) return 1;
     if (surrogate_losses(t->sp, t->d_bphs, t->d_bphi, t->d_binv, 0.0f, t->oscale, false, t->stream)) return 1;
-    const int nblk = 64, hp = 66;
+    const int nblk = SurrogatePlan::kHeadBlocks, hp = SurrogatePlan::kHeadPart;
     std::vector<float> h((size_t)t->batch * nblk * hp);
     MMS_HIP(hipMemcpyAsync(h.data(), t->sp.d_hpart, sizeof(float) * h.size(), hipMemcpyDeviceToHost, t->stream));
     MMS_HIP(hipStreamSynchronize(t->stream));
     for (int b = 0; b < t->batch; b++) {
         double s = 0;
-        for (int k = 0; k < nblk; k++) s += h[((size_t)b * nblk + k) * hp + 65];
+        for (int k = 0; k < nblk; k++) s += h[((size_t)b * nblk + k) * hp + hp - 1];
         residual[b] = std::sqrt(s);
     }
     return 0;

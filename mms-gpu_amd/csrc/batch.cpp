@@ -9,6 +9,7 @@
 //   field/LLG stage (tile list, per-film external field).
 // A whole RK4 step is captured once as a hipGraph and replayed.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <stdio.h>
 #include <string.h>
 #include <map>
@@ -39,6 +40,8 @@ struct mms_batch {
     int ntiles = 0;
     int2 *d_direct = nullptr;
     int ndirect_items = 0, ndirect_films = 0;
+    float *d_dtab = nullptr;  // 1/r tables of the direct films (at their cell offsets), or null
+    size_t dtab_lds = 0;      // LDS of the table kernel (largest direct film)
     std::vector<std::unique_ptr<FmmPlan>> fmm;  // one per size class
     std::vector<Grid> fmm_grid;
     std::vector<int> fmm_side;
@@ -68,6 +71,7 @@ int mms_batch::one_step(float dt) {
             launches++;
             if (ndirect_items) {
                 if (direct_batch_potential(rho, phi, d_direct, ndirect_items, d_films, gcommon.ay, gcommon.az, scale,
+                                           d_dtab, dtab_lds,
                                            stream, &prof))
                     return 1;
                 launches++;
@@ -89,7 +93,7 @@ int mms_batch::one_step(float dt) {
 static void batch_free(mms_batch *b) {
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     void *bufs[] = {b->m32[0], b->m32[1], b->m32[2], b->mn64, b->acc32, b->rho, b->phi, b->aos, b->matl,
-                    b->d_films, b->d_tiles, b->d_direct};
+                    b->d_films, b->d_tiles, b->d_direct, b->d_dtab};
     for (void *x : bufs)
         if (x) (void)hipFree(x);
     for (auto &f : b->fmm) f->release();
@@ -166,10 +170,39 @@ static int batch_build(mms_batch *b, const mms_film_spec *specs, const unsigned 
     b->ntiles = (int)tiles.size();
     MMS_HIP(hipMalloc(&b->d_tiles, sizeof(int4) * tiles.size()));
     MMS_HIP(hipMemcpy(b->d_tiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice));
+    // the largest films' items first: their workgroups are the longest, and
+    // the smaller ones fill in behind them
+    std::stable_sort(direct_items.begin(), direct_items.end(), [&](const int2 &u, const int2 &v) {
+        const long nu = (long)b->films[u.x].xd * b->films[u.x].yd, nv = (long)b->films[v.x].xd * b->films[v.x].yd;
+        return nu > nv;
+    });
     b->ndirect_items = (int)direct_items.size();
     if (b->ndirect_items) {
         MMS_HIP(hipMalloc(&b->d_direct, sizeof(int2) * direct_items.size()));
         MMS_HIP(hipMemcpy(b->d_direct, direct_items.data(), sizeof(int2) * direct_items.size(), hipMemcpyHostToDevice));
+        // 1/r tables G[|dz|][|dy|][|dx|] of the direct films (FP64, rounded
+        // once) when every one fits the table kernel's LDS
+        size_t lds = 0;
+        for (int f = 0; f < b->nfilm; f++)
+            if (b->solver[f] == 0) lds = std::max(lds, direct_table_lds(b->films[f].xd, b->films[f].yd, p.zdim));
+        if (lds <= kDirectTableLds) {
+            std::vector<float> tab((size_t)N, 0.0f);
+            const double ay = b->gcommon.ay, az = b->gcommon.az;
+            for (int f = 0; f < b->nfilm; f++) {
+                if (b->solver[f] != 0) continue;
+                const BatchFilm &bf = b->films[f];
+                float *t = tab.data() + bf.off;
+                for (int dz = 0; dz < bf.zd; dz++)
+                    for (int dy = 0; dy < bf.yd; dy++)
+                        for (int dx = 0; dx < bf.xd; dx++) {
+                            const double r2 = (double)dx * dx + (dy * ay) * (dy * ay) + (dz * az) * (dz * az);
+                            t[((long)dz * bf.yd + dy) * bf.xd + dx] = (dx | dy | dz) ? (float)(1.0 / std::sqrt(r2)) : 0.0f;
+                        }
+            }
+            MMS_HIP(hipMalloc(&b->d_dtab, sizeof(float) * N));
+            MMS_HIP(hipMemcpy(b->d_dtab, tab.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+            b->dtab_lds = lds;
+        }
     }
     for (auto &kv : classes) {
         Grid g = b->gcommon;
@@ -261,7 +294,7 @@ extern "C" int mms_batch_potential(mms_batch *b, const float *charge, float *pot
     if (b->ndirect_items) {
         const double scale = (double)(b->gcommon.dx * b->gcommon.dy * b->gcommon.dz) / (double)b->gcommon.dx;
         if (direct_batch_potential(b->rho, b->phi, b->d_direct, b->ndirect_items, b->d_films, b->gcommon.ay,
-                                   b->gcommon.az, scale, b->stream, &b->prof))
+                                   b->gcommon.az, scale, b->d_dtab, b->dtab_lds, b->stream, &b->prof))
             return 1;
     }
     for (size_t k = 0; k < b->fmm.size(); k++)

@@ -184,6 +184,72 @@ __global__ __launch_bounds__(TB) void direct_batch_kernel(const float *__restric
     if (t < n) phi[t] = (float)(acc * scale);
 }
 
+// Batched small films on a 1/r table.  On the grid, 1/r depends only on the
+// absolute cell offsets, so the film's table G[|dz|][|dy|][|dx|] (xd*yd*zd
+// floats, built on the host in FP64 and rounded once; G[0] = 0 excludes the
+// cell itself) and its charges (rows padded to a multiple of 4 with zeros)
+// sit in LDS.  Per pair: one absolute difference plus row base (the table
+// byte offset, v_sad_u32), one LDS read, one FMA; FP32 over a source row (at
+// most 256 sources, four interleaved chains), FP64 across rows.  Two threads
+// per target (the even and the odd source rows of every layer), added at the end.  The
+// padded sources read table entries past the row (or past the table, into
+// its zeroed padding and the charges), all finite, times a zero charge.
+constexpr int TT = 2 * TB;  // threads of the table kernel
+__global__ __launch_bounds__(TT) void direct_table_kernel(const float *__restrict__ rho, float *__restrict__ phi,
+                                                          const float *__restrict__ gtab,
+                                                          const int2 *__restrict__ items,
+                                                          const BatchFilm *__restrict__ films, double scale) {
+    extern __shared__ __attribute__((aligned(16))) float dlds[];
+    __shared__ double half1[TB];
+    const int2 it = items[blockIdx.x];
+    const BatchFilm f = films[it.x];
+    const int xd = f.xd, yd = f.yd, zd = f.zd, xdp = (xd + 3) & ~3;
+    const int plane = xd * yd, n = plane * zd, n4 = (n + 3) & ~3;
+    float *tab = dlds;       // [zd][yd][xd]
+    float *q = dlds + n4;    // [zd][yd][xdp]
+    rho += f.off;
+    phi += f.off;
+    gtab += f.off;
+    for (int e = threadIdx.x; e < n4; e += TT) tab[e] = e < n ? gtab[e] : 0.0f;  // padding read by padded sources
+    for (int e = threadIdx.x; e < zd * yd * xdp; e += TT) {
+        const int xs = e % xdp, r = e / xdp;
+        q[e] = xs < xd ? rho[(long)r * xd + xs] : 0.0f;
+    }
+    __syncthreads();
+    const int h = threadIdx.x / TB, t = it.y * TB + (threadIdx.x % TB);
+    double acc = 0.0;
+    if (t < n) {
+        const int xt = t % xd, yt = (t / xd) % yd, zt = t / plane;
+        const char *tb = reinterpret_cast<const char *>(tab);
+        const unsigned x4t = 4u * (unsigned)xt;
+        const int ng = xdp >> 2;  // float4 groups of a row
+        for (int zs = 0; zs < zd; zs++) {
+            const unsigned bz = 4u * (unsigned)((zt > zs ? zt - zs : zs - zt) * plane);
+            for (int ys = h; ys < yd; ys += 2) {
+                // byte offset of the table row; a source's entry is |4 xt - 4 xs| further
+                const unsigned rb = __umul24((unsigned)(yt > ys ? yt - ys : ys - yt), 4u * (unsigned)xd) + bz;
+                const float4 *qr = reinterpret_cast<const float4 *>(q + (zs * yd + ys) * xdp);
+                for (int g0 = 0; g0 < ng; g0 += 64) {  // FP32 over at most 256 sources
+                    const int g1 = (g0 + 64 < ng) ? g0 + 64 : ng;
+                    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;  // one chain per column mod 4
+                    for (int g = g0; g < g1; g++) {
+                        const float4 c = qr[g];
+                        const unsigned x4s = 16u * (unsigned)g;
+                        s0 = fmaf(c.x, *reinterpret_cast<const float *>(tb + (max(x4t, x4s) - min(x4t, x4s) + rb)), s0);
+                        s1 = fmaf(c.y, *reinterpret_cast<const float *>(tb + (max(x4t, x4s + 4) - min(x4t, x4s + 4) + rb)), s1);
+                        s2 = fmaf(c.z, *reinterpret_cast<const float *>(tb + (max(x4t, x4s + 8) - min(x4t, x4s + 8) + rb)), s2);
+                        s3 = fmaf(c.w, *reinterpret_cast<const float *>(tb + (max(x4t, x4s + 12) - min(x4t, x4s + 12) + rb)), s3);
+                    }
+                    acc += (double)((s0 + s1) + (s2 + s3));
+                }
+            }
+        }
+    }
+    if (h == 1) half1[threadIdx.x - TB] = acc;
+    __syncthreads();
+    if (h == 0 && t < n) phi[t] = (float)((acc + half1[threadIdx.x]) * scale);
+}
+
 // The summation order and float arithmetic of the reference's CUDA kernel
 // calc_potential_exact_kernel (potential_calc.cu:18-116), for callers that
 // want its results bit for bit:
@@ -266,12 +332,29 @@ int direct_refgpu_potential(const Grid &g, const float *rho, float *phi, hipStre
 }
 
 int direct_batch_potential(const float *rho, float *phi, const int2 *items, int nitems, const BatchFilm *films,
-                           float ay, float az, double scale, hipStream_t s, Profiler *prof) {
+                           float ay, float az, double scale, const float *tab, size_t tab_lds, hipStream_t s,
+                           Profiler *prof) {
     KTimer kt(prof, K_DIRECT);
     if (nitems <= 0) return 0;
-    hipLaunchKernelGGL(direct_batch_kernel, dim3(nitems), dim3(TB), 0, s, rho, phi, items, films, ay, az, scale);
+    if (tab) {
+        MMS_CHECK(tab_lds <= kDirectTableLds, "direct sum: film table exceeds LDS");
+        static bool attr = false;
+        if (!attr) {
+            MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&direct_table_kernel),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDirectTableLds));
+            attr = true;
+        }
+        hipLaunchKernelGGL(direct_table_kernel, dim3(nitems), dim3(TT), tab_lds, s, rho, phi, tab, items, films, scale);
+    } else {
+        hipLaunchKernelGGL(direct_batch_kernel, dim3(nitems), dim3(TB), 0, s, rho, phi, items, films, ay, az, scale);
+    }
     MMS_HIP(hipGetLastError());
     return 0;
+}
+
+size_t direct_table_lds(int xd, int yd, int zd) {
+    const long n = (long)xd * yd * zd;
+    return sizeof(float) * (size_t)(((n + 3) & ~3L) + (long)zd * yd * ((xd + 3) & ~3));
 }
 
 // Source chunks of the direct sum for n cells: aim for >= 1024 blocks in

@@ -239,8 +239,13 @@ int launch_stage_batch(const Grid &g, const Material &mat, int stage, double dt,
                        float *m_out, double *mn, const float *m_base, float *acc, const uint8_t *matl,
                        const int4 *tiles, int ntiles, const BatchFilm *films, long pstride, hipStream_t s,
                        Profiler *prof);
+// tab (or null: per-pair 1/r): every direct film's 1/r table G[|dz|][|dy|][|dx|] at the film's cell
+// offset; tab_lds: the largest direct film's table + padded charges (direct_table_lds) in bytes
 int direct_batch_potential(const float *rho, float *phi, const int2 *items, int nitems, const BatchFilm *films,
-                           float ay, float az, double scale, hipStream_t s, Profiler *prof);
+                           float ay, float az, double scale, const float *tab, size_t tab_lds, hipStream_t s,
+                           Profiler *prof);
+constexpr size_t kDirectTableLds = 96 * 1024;  // films whose table and charges fit take the table kernel
+size_t direct_table_lds(int xd, int yd, int zd);
 int launch_aos_from_soa(const float *soa32, float *aos, long n, hipStream_t s);
 int launch_add_time(double *t_dev, double dt, hipStream_t s);
 // zero the FP32 SoA planes [3][n] of m on non-material cells

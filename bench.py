@@ -500,8 +500,8 @@ def bench_c5(args, mg, wl):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--m2l", default="fp16x2_f16x2near", choices=PREC_NAMES,
                     help="operand form; the default is the fastest form inside the workload's precision split")
     ap.add_argument("--leaf", type=int, default=8)

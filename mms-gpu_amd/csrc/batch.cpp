@@ -10,6 +10,7 @@
 // A whole RK4 step is captured once as a hipGraph and replayed.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cstdlib>
 #include <stdio.h>
 #include <string.h>
 #include <map>
@@ -30,6 +31,12 @@ struct mms_batch {
     std::vector<BatchFilm> films;
     std::vector<int> solver;  // per film: 0 direct, 1 fmm
     hipStream_t stream = nullptr;
+    // the direct sum of the small films and each FMM size class are
+    // independent: branches 1.. run on their own streams (default priority),
+    // forked after the charge kernel and joined before the stage kernel
+    std::vector<hipStream_t> s_branch;
+    hipEvent_t ev_fork = nullptr;
+    std::vector<hipEvent_t> ev_join;
     float *m32[3] = {nullptr, nullptr, nullptr};  // state slot cur + two stage slots
     double *mn64 = nullptr;
     float *acc32 = nullptr;  // FP32 slope accumulator
@@ -69,16 +76,30 @@ int mms_batch::one_step(float dt) {
                                  gcommon.zd, stream, &prof))
                 return 1;
             launches++;
+            // branches: the FMM classes, then the direct sum; one kernel at a
+            // time inside a profiling window
+            const bool conc = !s_branch.empty() && !prof.on;
+            if (conc) {
+                MMS_HIP(hipEventRecord(ev_fork, stream));
+                for (hipStream_t sb : s_branch) MMS_HIP(hipStreamWaitEvent(sb, ev_fork, 0));
+            }
+            int br = 0;
+            auto branch = [&]() { hipStream_t sb = (conc && br > 0) ? s_branch[br - 1] : stream; br++; return sb; };
+            for (size_t k = 0; k < fmm.size(); k++) {
+                if (fmm_potential(fmm_grid[k], *fmm[k], rho, phi, branch(), &prof)) return 1;
+                launches += 2 * fmm[k]->Lleaf - 1;
+            }
             if (ndirect_items) {
                 if (direct_batch_potential(rho, phi, d_direct, ndirect_items, d_films, gcommon.ay, gcommon.az, scale,
-                                           d_dtab, dtab_lds,
-                                           stream, &prof))
+                                           d_dtab, dtab_lds, branch(), &prof))
                     return 1;
                 launches++;
             }
-            for (size_t k = 0; k < fmm.size(); k++) {
-                if (fmm_potential(fmm_grid[k], *fmm[k], rho, phi, stream, &prof)) return 1;
-                launches += 2 * fmm[k]->Lleaf - 1;
+            if (conc) {
+                for (size_t i = 0; i < s_branch.size(); i++) {
+                    MMS_HIP(hipEventRecord(ev_join[i], s_branch[i]));
+                    MMS_HIP(hipStreamWaitEvent(stream, ev_join[i], 0));
+                }
             }
         }
         if (launch_stage_batch(gcommon, mat, s, (double)dt, phi, in[s], out[s], mn64, m32[a], acc32, matl, d_tiles, ntiles,
@@ -97,6 +118,9 @@ static void batch_free(mms_batch *b) {
     for (void *x : bufs)
         if (x) (void)hipFree(x);
     for (auto &f : b->fmm) f->release();
+    for (hipStream_t sb : b->s_branch) (void)hipStreamDestroy(sb);
+    for (hipEvent_t e : b->ev_join) (void)hipEventDestroy(e);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
@@ -215,6 +239,20 @@ static int batch_build(mms_batch *b, const mms_film_spec *specs, const unsigned 
         b->fmm.push_back(std::move(plan));
         b->fmm_grid.push_back(g);
         b->fmm_side.push_back(kv.first);
+    }
+    // one stream per solver branch beyond the first (MMS_GPU_SERIAL=1: all on one)
+    const int nbranch = (int)b->fmm.size() + (b->ndirect_items ? 1 : 0);
+    const char *serial = std::getenv("MMS_GPU_SERIAL");
+    if (nbranch > 1 && !(serial && serial[0] == '1')) {
+        MMS_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+        for (int i = 1; i < nbranch; i++) {
+            hipStream_t sb;
+            hipEvent_t e;
+            MMS_HIP(hipStreamCreateWithFlags(&sb, hipStreamNonBlocking));
+            b->s_branch.push_back(sb);
+            MMS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            b->ev_join.push_back(e);
+        }
     }
     return 0;
 }

@@ -442,8 +442,9 @@ def bench_c5(args, mg, wl):
     ncell = int(info["cells"])
     kernels = {}
     if "stage" in prof:
+        # the films carry material masks: every stage reads the material byte
         ms, _ = prof["stage"]
-        by = (77 + 3 * 101) / 4 * ncell
+        by = sum(stage_bytes_per_cell(st, False) for st in range(4)) / 4 * ncell
         kernels["stage"] = dict(bound="hbm", achieved=by / (ms * 1e-3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s", ms=ms)
     if "direct" in prof:
         # the small films' direct sum reads 1/r from a per-film table: per

@@ -140,8 +140,7 @@ static int batch_build(mms_batch *b, const mms_film_spec *specs, const unsigned 
         bf.hx = sp.hext[0]; bf.hy = sp.hext[1]; bf.hz = sp.hext[2]; bf.pad2 = 0;
         const long n = (long)sp.xdim * sp.ydim * p.zdim;
         off += n;
-        for (int ty = 0; ty < (sp.ydim + STENCIL_TILE_Y - 1) / STENCIL_TILE_Y; ty++)
-            for (int tx = 0; tx < (sp.xdim + STENCIL_TILE_X - 1) / STENCIL_TILE_X; tx++) tiles.push_back({f, tx, ty, 0});
+        for (long c = 0; c < (long)sp.xdim * sp.ydim; c += STENCIL_TILE_COLUMNS) tiles.push_back({f, (int)c, 0, 0});
         int side = 16;
         while (side < sp.xdim || side < sp.ydim) side <<= 1;
         if (p.solver == MMS_SOLVER_DIRECT || n < kDirectMaxCells) {
@@ -181,7 +180,7 @@ static int batch_build(mms_batch *b, const mms_film_spec *specs, const unsigned 
     {
         // XCD-aware order: workgroup i runs on XCD i mod 8, so list slot i
         // holds logical tile (i mod 8) * per + i / 8 and each XCD walks a
-        // contiguous run of tiles (whole films, tile rows in order); padding
+        // contiguous run of tiles (whole films, column runs in order); padding
         // slots carry film -1
         const long nt = (long)tiles.size(), per = (nt + 7) / 8;
         std::vector<int4> order(8 * per);

@@ -230,9 +230,10 @@ int launch_stage_ex(const Grid &g, const Material &mat, int stage, double dt, co
 // Block partials [nblk][5] of (sum Mx, My, Mz, |M|, count) over material cells
 int launch_mavg(const float *m, const uint8_t *matl, long n, double *partial, int nblk, hipStream_t s);
 int launch_soa_from_aos(const float *aos, float *soa32, double *soa64, long n, hipStream_t s);
-// Batched films (tile list of (film, tile x, tile y) over 64 x 4 column tiles;
-// component planes pstride cells apart; per-film constant external field)
-constexpr int STENCIL_TILE_X = 64, STENCIL_TILE_Y = 4;
+// Batched films (tile list of (film, first column): 256 consecutive (x, y)
+// columns of a film in row-major order; component planes pstride cells apart;
+// per-film constant external field)
+constexpr int STENCIL_TILE_COLUMNS = 256;
 int launch_div_batch(const float *m, float *rho, const int4 *tiles, int ntiles, const BatchFilm *films, long pstride,
                      float dx, float dy, float dz, int zd, hipStream_t s, Profiler *prof);
 int launch_stage_batch(const Grid &g, const Material &mat, int stage, double dt, const float *phi, const float *m_in,

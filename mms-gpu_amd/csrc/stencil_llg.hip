@@ -43,6 +43,16 @@ __device__ __forceinline__ bool xcd_tile(int ntx, int ntiles, int &tx, int &ty) 
     return true;
 }
 
+// Batched films: a tile is 256 consecutive (x, y) columns of one film in
+// row-major order, starting at column t.y, so only each film's last tile has
+// idle threads; out-of-film threads get x = xd (no work).
+__device__ __forceinline__ void batch_column(const int4 &t, const BatchFilm &f, int &x, int &y) {
+    const int col = t.y + (int)(threadIdx.y * BX + threadIdx.x);
+    const bool ok = col < f.xd * f.yd;
+    y = ok ? col / f.xd : 0;
+    x = ok ? col - y * f.xd : f.xd;
+}
+
 template <int ZD>
 __device__ __forceinline__ void div_column(const float *__restrict__ m, float *__restrict__ rho, int x, int y, int xd,
                                            int yd, int zd_rt, long n, float dx, float dy, float dz) {
@@ -120,8 +130,9 @@ __global__ __launch_bounds__(BX * BY) void div_batch_kernel(const float *__restr
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;  // padding slot of the XCD-ordered tile list
     const BatchFilm f = films[t.x];
-    div_column<ZD>(m + f.off, rho + f.off, t.y * BX + threadIdx.x, t.z * BY + threadIdx.y, f.xd, f.yd, f.zd, pstride,
-                   dx, dy, dz);
+    int x, y;
+    batch_column(t, f, x, y);
+    div_column<ZD>(m + f.off, rho + f.off, x, y, f.xd, f.yd, f.zd, pstride, dx, dy, dz);
 }
 
 
@@ -569,8 +580,9 @@ __global__ __launch_bounds__(BX * BY) void stage_batch_kernel(StageArgs A, const
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;  // padding slot of the XCD-ordered tile list
     const BatchFilm f = films[t.x];
-    stage_column<ZD, ST>(A, t.y * BX + threadIdx.x, t.z * BY + threadIdx.y, f.xd, f.yd, f.zd, f.off, pstride, f.hx, f.hy,
-                     f.hz);
+    int x, y;
+    batch_column(t, f, x, y);
+    stage_column<ZD, ST>(A, x, y, f.xd, f.yd, f.zd, f.off, pstride, f.hx, f.hy, f.hz);
 }
 
 __global__ __launch_bounds__(BX * BY) void hfield_kernel(StageArgs A) {

@@ -248,12 +248,29 @@ __global__ __launch_bounds__(256) void sur_conv_kernel(ConvArgs A) {
         __syncthreads();
         if (t + (int)gridDim.x < ntile) load_win(t + gridDim.x, nxt);  // in flight during the products
         const int x0 = (t % ntx) * TS, y0 = (t / ntx) * TS;
+        // the epilogue's operands (added sum, saved pre-activations) of this
+        // lane's outputs, loaded now so the products hide their latency
+        uint2 pad[4][4], pzin[4][4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int gx = x0 + j, gy = y0 + 4 * wave + c;
+            const bool ok = gx < xd && gy < yd;
+            const long pix = ((long)z * yd + gy) * xd + gx;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const long o = pix * C + 16 * r + 4 * g;
+                pad[c][r] = make_uint2(0u, 0u);
+                pzin[c][r] = make_uint2(0u, 0u);
+                if (EPI != 1 && A.add && ok) pad[c][r] = *reinterpret_cast<const uint2 *>(A.add + o);
+                if (EPI == 2 && ok) pzin[c][r] = *reinterpret_cast<const uint2 *>(A.zin + o);
+            }
+        }
         f32x4 acc[4][4];
 #pragma unroll
         for (int r = 0; r < 4; r++)
 #pragma unroll
             for (int c = 0; c < 4; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 1
+#pragma unroll
         for (int tap = 0; tap < 9; tap++) {
             const int dy = tap / 3, dx = tap % 3;
 #pragma unroll
@@ -283,10 +300,10 @@ __global__ __launch_bounds__(256) void sur_conv_kernel(ConvArgs A) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const long o = pix * C + 16 * r + 4 * g;
-                float ad[4] = {0.f, 0.f, 0.f, 0.f}, zz[4] = {0.f, 0.f, 0.f, 0.f}, v[4];
-                if (EPI != 1 && A.add && ok) unpack4(*reinterpret_cast<const uint2 *>(A.add + o), ad);
+                float ad[4], zz[4] = {0.f, 0.f, 0.f, 0.f}, v[4];
+                unpack4(pad[c][r], ad);
                 if (EPI == 2) {
-                    if (ok) unpack4(*reinterpret_cast<const uint2 *>(A.zin + o), zz);
+                    unpack4(pzin[c][r], zz);
 #pragma unroll
                     for (int i = 0; i < 4; i++) v[i] = ok ? (acc[r][c][i] + ad[i]) * gelu_grad(zz[i]) : 0.0f;
                 } else {

@@ -160,9 +160,10 @@ struct SurrogatePlan {
     int nmode = 0;
     bool train = false;
     float *d_param = nullptr;
-    unsigned short *d_lift = nullptr, *d_wm = nullptr, *d_bx = nullptr, *d_by = nullptr;
+    unsigned short *d_lift = nullptr, *d_wm = nullptr, *d_bx = nullptr, *d_by = nullptr, *d_bxt = nullptr;
     unsigned short *d_w[3] = {nullptr, nullptr, nullptr}, *d_wt[3] = {nullptr, nullptr, nullptr};
-    unsigned short *d_h0 = nullptr, *d_h1 = nullptr, *d_h2 = nullptr, *d_spec = nullptr;  // NHWC BF16
+    unsigned short *d_h0 = nullptr, *d_h1 = nullptr, *d_h2 = nullptr;  // NHWC BF16
+    unsigned short *d_x2t = nullptr;  // spectral X2^T [z][y][64][32] BF16
     unsigned short *d_z[4] = {nullptr, nullptr, nullptr, nullptr};                       // pre-activations
     unsigned short *d_dza = nullptr, *d_dzb = nullptr;                                     // gradients
     float *d_x1 = nullptr, *d_hm = nullptr, *d_gm = nullptr, *d_x2 = nullptr, *d_dh = nullptr;

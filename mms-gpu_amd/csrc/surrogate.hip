@@ -77,7 +77,7 @@ __device__ __forceinline__ void unpack4(uint2 u, float (&v)[4]) {
 }
 
 constexpr int C = 64;     // hidden channels
-constexpr int CP = 72;    // LDS pixel / weight-row stride in bf16 (16-byte aligned, spreads banks)
+constexpr int CP = 80;    // LDS pixel / weight-row stride in bf16: 160-byte rows keep the 16-byte fragment reads free of bank conflicts (144-byte rows: 4 conflict cycles per read)
 constexpr int TS = 16;    // output tile side (pixels)
 constexpr int HS = TS + 2;
 
@@ -180,15 +180,19 @@ __global__ __launch_bounds__(256) void sur_lift_kernel(const float *__restrict__
 // next tile's 18 x 18 x 64 input window is loaded into registers while the
 // current one is multiplied, then stored to LDS.  Wave w computes D[64 co]
 // [64 pixels of tile rows 4w..4w+3] over 18 k steps (tap, 32 input
-// channels).  w: [9 taps][64 co][64 ci] bf16.  Epilogues:
-//   EPI 0: bias (+ add) + GELU, BF16 NHWC out (zsave: pre-activation)
+// channels).  w: [9 taps][64 co][64 ci] bf16.  With x2t the spectral
+// branch's inverse transform is added in the same accumulators: S[co][x] =
+// sum_k X2[z][y][k][co] Bx[k][x], one 32-deep MFMA per 16 x 16 output tile
+// from X2^T [z][y][64 co][32 k] and Bx^T [x][32 k] (BF16, global memory).
+// Epilogues:
+//   EPI 0: bias + GELU, BF16 NHWC out (zsave: pre-activation)
 //   EPI 1: bias + GELU, then the 1x1 projection to one channel (weights wp,
 //          bias *bp), times oscale, FP32 potential out (zsave as EPI 0)
-//   EPI 2: backward data pass: (sum + add) * GELU'(zin), BF16 NHWC out
+//   EPI 2: backward data pass: sum * GELU'(zin), BF16 NHWC out
 struct ConvArgs {
     const unsigned short *in, *w;
     const float *bias;
-    const unsigned short *add;
+    const unsigned short *x2t, *bxt;
     unsigned short *out, *zsave;
     const unsigned short *zin;
     const float *wp, *bp;
@@ -248,23 +252,27 @@ __global__ __launch_bounds__(256) void sur_conv_kernel(ConvArgs A) {
         __syncthreads();
         if (t + (int)gridDim.x < ntile) load_win(t + gridDim.x, nxt);  // in flight during the products
         const int x0 = (t % ntx) * TS, y0 = (t / ntx) * TS;
-        // the epilogue's operands (added sum, saved pre-activations) of this
-        // lane's outputs, loaded now so the products hide their latency
-        uint2 pad[4][4], pzin[4][4];
+        // the epilogue's saved pre-activations and the spectral operands of
+        // this lane's outputs, loaded now so the products hide their latency
+        uint2 pzin[4][4];
+        bf16x8 sa[4][4], sb;
+        const bool spec = EPI != 1 && A.x2t != nullptr;  // uniform
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             const int gx = x0 + j, gy = y0 + 4 * wave + c;
             const bool ok = gx < xd && gy < yd;
             const long pix = ((long)z * yd + gy) * xd + gx;
+            // rows and columns beyond the film read its last ones (discarded outputs)
+            const long srow = (long)z * yd + (gy < yd ? gy : yd - 1);
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const long o = pix * C + 16 * r + 4 * g;
-                pad[c][r] = make_uint2(0u, 0u);
                 pzin[c][r] = make_uint2(0u, 0u);
-                if (EPI != 1 && A.add && ok) pad[c][r] = *reinterpret_cast<const uint2 *>(A.add + o);
                 if (EPI == 2 && ok) pzin[c][r] = *reinterpret_cast<const uint2 *>(A.zin + o);
+                if (spec) sa[c][r] = *reinterpret_cast<const bf16x8 *>(A.x2t + (srow * C + 16 * r + j) * 32 + 8 * g);
             }
         }
+        if (spec) sb = *reinterpret_cast<const bf16x8 *>(A.bxt + (long)(x0 + j < xd ? x0 + j : xd - 1) * 32 + 8 * g);
         f32x4 acc[4][4];
 #pragma unroll
         for (int r = 0; r < 4; r++)
@@ -290,6 +298,12 @@ __global__ __launch_bounds__(256) void sur_conv_kernel(ConvArgs A) {
                     for (int c = 0; c < 4; c++) acc[r][c] = mfma_bf16(a[r], b[c], acc[r][c]);
             }
         }
+        if (spec) {
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) acc[r][c] = mfma_bf16(sa[c][r], sb, acc[r][c]);
+        }
         // epilogue: lane (j, g) holds channels 16 r + 4 g + i of pixel (tile row 4 wave + c, column j)
 #pragma unroll
         for (int c = 0; c < 4; c++) {
@@ -300,17 +314,16 @@ __global__ __launch_bounds__(256) void sur_conv_kernel(ConvArgs A) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const long o = pix * C + 16 * r + 4 * g;
-                float ad[4], zz[4] = {0.f, 0.f, 0.f, 0.f}, v[4];
-                unpack4(pad[c][r], ad);
+                float zz[4] = {0.f, 0.f, 0.f, 0.f}, v[4];
                 if (EPI == 2) {
                     unpack4(pzin[c][r], zz);
 #pragma unroll
-                    for (int i = 0; i < 4; i++) v[i] = ok ? (acc[r][c][i] + ad[i]) * gelu_grad(zz[i]) : 0.0f;
+                    for (int i = 0; i < 4; i++) v[i] = ok ? acc[r][c][i] * gelu_grad(zz[i]) : 0.0f;
                 } else {
 #pragma unroll
                     for (int i = 0; i < 4; i++) {
                         const int co = 16 * r + 4 * g + i;
-                        zz[i] = acc[r][c][i] + A.bias[co] + ad[i];
+                        zz[i] = acc[r][c][i] + A.bias[co];
                         v[i] = gelu(zz[i]);
                         if (EPI == 1) proj += bf2f(f2bf(v[i])) * A.wp[co];
                     }
@@ -789,11 +802,21 @@ int launch_conv(int epi, const SurrogatePlan &sp, const ConvArgs &a, hipStream_t
     return 0;
 }
 
+// X2^T [z][y][64 c][32 k] (BF16) of X2 [z][y][32 k][64 c] (FP32): one (z, y)
+// row per workgroup through LDS
+__global__ __launch_bounds__(256) void sur_x2t_kernel(const float *__restrict__ x2, unsigned short *__restrict__ x2t) {
+    __shared__ float t[32][C + 1];
+    const float *src = x2 + (long)blockIdx.x * 32 * C;
+    for (int e = threadIdx.x; e < 32 * C; e += 256) t[e / C][e % C] = src[e];
+    __syncthreads();
+    unsigned short *dst = x2t + (long)blockIdx.x * 32 * C;
+    for (int e = threadIdx.x; e < 32 * C; e += 256) dst[e] = f2bf(t[e % 32][e / 32]);
+}
+
 // The spectral branch's transform chain of the BF16 NHWC array `in`: X1 = Bx in,
 // hbuf = By X1, gbuf = mix(hbuf) (transposed mixing when tr), X2 = By^T gbuf,
-// out = Bx^T X2 (BF16 NHWC).
-int spectral(SurrogatePlan &sp, const unsigned short *in, float *hbuf, float *gbuf, bool tr, unsigned short *out,
-             hipStream_t s) {
+// then X2^T in BF16 (sp.d_x2t) for the conv kernel, which adds Bx^T X2.
+int spectral(SurrogatePlan &sp, const unsigned short *in, float *hbuf, float *gbuf, bool tr, hipStream_t s) {
     const int X = sp.g.xd, Y = sp.g.yd, Z = sp.g.zd;
     {
         GemmArgs a{};  // X1[z][y][kx][c] = sum_x Bx[kx][x] in[z][y][x][c]
@@ -822,14 +845,8 @@ int spectral(SurrogatePlan &sp, const unsigned short *in, float *hbuf, float *gb
         a.M = Y; a.N = 32 * C; a.K = 32;
         if (gemm<unsigned short, float, float>(a, Z, s)) return 1;
     }
-    {
-        GemmArgs a{};  // out[z][y][x][c] = sum_kx Bx[kx][x] X2[z][y][kx][c]  (BF16 NHWC)
-        a.A = sp.d_bx; a.sam = 1; a.sak = X; a.sab = 0;
-        a.B = sp.d_x2; a.sbk = C; a.sbn = 1; a.sbb = 32L * C;
-        a.C = out; a.scm = C; a.scn = 1; a.scb = (long)X * C;
-        a.M = X; a.N = C; a.K = 32;
-        if (gemm<unsigned short, float, unsigned short>(a, Z * Y, s)) return 1;
-    }
+    hipLaunchKernelGGL(sur_x2t_kernel, dim3(Z * Y), dim3(256), 0, s, sp.d_x2, sp.d_x2t);
+    MMS_HIP(hipGetLastError());
     return 0;
 }
 
@@ -950,20 +967,24 @@ int SurrogatePlan::init(const Grid &g_, int zf_, const float *weights, bool trai
             }
         return b;
     };
-    std::vector<unsigned short> bx = basis(g.xd), by = basis(g.yd);
+    std::vector<unsigned short> bx = basis(g.xd), by = basis(g.yd), bxt(bx.size());
+    for (int k = 0; k < 32; k++)
+        for (int x = 0; x < g.xd; x++) bxt[(size_t)x * 32 + k] = bx[(size_t)k * g.xd + x];
     MMS_HIP(hipMalloc(&d_bx, bx.size() * 2));
+    MMS_HIP(hipMalloc(&d_bxt, bxt.size() * 2));
+    MMS_HIP(hipMemcpyAsync(d_bxt, bxt.data(), bxt.size() * 2, hipMemcpyHostToDevice, s));
     MMS_HIP(hipMalloc(&d_by, by.size() * 2));
     MMS_HIP(hipMemcpyAsync(d_bx, bx.data(), bx.size() * 2, hipMemcpyHostToDevice, s));
     MMS_HIP(hipMemcpyAsync(d_by, by.data(), by.size() * 2, hipMemcpyHostToDevice, s));
     const size_t act = sizeof(unsigned short) * npix * C;
     MMS_HIP(hipMalloc(&d_h0, act));
     MMS_HIP(hipMalloc(&d_h1, act));
-    MMS_HIP(hipMalloc(&d_spec, act));
     d_h2 = d_h0;  // inference: conv2's output overwrites the lift's
     MMS_HIP(hipMalloc(&d_x1, sizeof(float) * (size_t)g.zd * g.yd * 32 * C));
     MMS_HIP(hipMalloc(&d_hm, sizeof(float) * (size_t)g.zd * nmode * C));
     MMS_HIP(hipMalloc(&d_gm, sizeof(float) * (size_t)g.zd * nmode * C));
     MMS_HIP(hipMalloc(&d_x2, sizeof(float) * (size_t)g.zd * g.yd * 32 * C));
+    MMS_HIP(hipMalloc(&d_x2t, sizeof(unsigned short) * (size_t)g.zd * g.yd * 32 * C));
     MMS_HIP(hipMalloc(&d_part, sizeof(double) * 2 * 1024));
     if (train) {
         MMS_HIP(hipMalloc(&d_h2, act));
@@ -1012,12 +1033,12 @@ int SurrogatePlan::pack(hipStream_t s) {
 void SurrogatePlan::release() {
     if (d_h2 == d_h0) d_h2 = nullptr;
     void *bufs[] = {d_param, d_lift, d_w[0], d_w[1], d_w[2], d_wt[0], d_wt[1], d_wt[2], d_wm, d_bx, d_by,
-                    d_h0,    d_h1,   d_h2,   d_spec, d_z[0], d_z[1],  d_z[2],  d_z[3],  d_dza, d_dzb,
+                    d_h0,    d_h1,   d_h2,   d_bxt,  d_x2t,  d_z[0], d_z[1],  d_z[2],  d_z[3],  d_dza, d_dzb,
                     d_x1,    d_hm,   d_gm,   d_x2,   d_dh,   d_wpart, d_hpart, d_hsum,  d_part};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     d_param = d_x1 = d_hm = d_gm = d_x2 = d_dh = d_wpart = d_hpart = d_hsum = nullptr;
-    d_lift = d_wm = d_bx = d_by = d_h0 = d_h1 = d_h2 = d_spec = d_dza = d_dzb = nullptr;
+    d_lift = d_wm = d_bx = d_by = d_bxt = d_x2t = d_h0 = d_h1 = d_h2 = d_dza = d_dzb = nullptr;
     for (int l = 0; l < 3; l++) d_w[l] = d_wt[l] = nullptr;
     for (int l = 0; l < 4; l++) d_z[l] = nullptr;
     d_part = nullptr;
@@ -1038,8 +1059,8 @@ int surrogate_potential(SurrogatePlan &sp, const float *rho, float *phi, float r
     a.in = sp.d_h0; a.w = sp.d_w[0]; a.bias = P + OFF_W1 + CONVN; a.out = sp.d_h1; a.zsave = sp.d_z[1];
     if (launch_conv(0, sp, a, s)) return 1;
     // spectral branch of h1
-    if (spectral(sp, sp.d_h1, sp.d_hm, sp.d_gm, false, sp.d_spec, s)) return 1;
-    a.in = sp.d_h1; a.w = sp.d_w[1]; a.bias = P + OFF_W2 + CONVN; a.add = sp.d_spec; a.out = sp.d_h2;
+    if (spectral(sp, sp.d_h1, sp.d_hm, sp.d_gm, false, s)) return 1;
+    a.in = sp.d_h1; a.w = sp.d_w[1]; a.bias = P + OFF_W2 + CONVN; a.x2t = sp.d_x2t; a.bxt = sp.d_bxt; a.out = sp.d_h2;
     a.zsave = sp.d_z[2];
     if (launch_conv(0, sp, a, s)) return 1;
     a = ConvArgs{};
@@ -1078,16 +1099,16 @@ int surrogate_backward(SurrogatePlan &sp, const float *rho, const float *phi_s, 
     if (launch_conv(2, sp, a, s)) return 1;
     // conv2: dW2 from (dz2, h1); the spectral branch: dWm and its data gradient
     if (launch_wgrad(sp, sp.d_dzb, sp.d_h1, grad + OFF_W2, s)) return 1;
-    if (spectral(sp, sp.d_dzb, sp.d_gm, sp.d_dh, true, sp.d_spec, s)) return 1;
+    if (spectral(sp, sp.d_dzb, sp.d_gm, sp.d_dh, true, s)) return 1;
     hipLaunchKernelGGL(sur_mix_wgrad_kernel, dim3(sp.nmode), dim3(256), 0, s, sp.d_gm, sp.d_hm, grad + OFF_WM,
                        sp.nmode, g.zd);
     MMS_HIP(hipGetLastError());
     // dz1 = (convT(dz2) + spectral^T(dz2)) GELU'(z1)
-    a.in = sp.d_dzb; a.w = sp.d_wt[1]; a.add = sp.d_spec; a.zin = sp.d_z[1]; a.out = sp.d_dza;
+    a.in = sp.d_dzb; a.w = sp.d_wt[1]; a.x2t = sp.d_x2t; a.bxt = sp.d_bxt; a.zin = sp.d_z[1]; a.out = sp.d_dza;
     if (launch_conv(2, sp, a, s)) return 1;
     // conv1: dW1 from (dz1, h0); dz0 = convT(dz1) GELU'(z0)
     if (launch_wgrad(sp, sp.d_dza, sp.d_h0, grad + OFF_W1, s)) return 1;
-    a.in = sp.d_dza; a.w = sp.d_wt[0]; a.add = nullptr; a.zin = sp.d_z[0]; a.out = sp.d_dzb;
+    a.in = sp.d_dza; a.w = sp.d_wt[0]; a.x2t = nullptr; a.bxt = nullptr; a.zin = sp.d_z[0]; a.out = sp.d_dzb;
     if (launch_conv(2, sp, a, s)) return 1;
     // lift
     const int ntile = ((g.xd + TS - 1) / TS) * ((g.yd + TS - 1) / TS) * g.zd;

@@ -81,7 +81,7 @@ def forward_t(p, rho, zf, rscale, oscale):
     Wm = _bf(p["mode"]).reshape(32, 32, C, C)                              # [ky, kx, co, ci]
     G = torch.einsum("lkoi,zlki->zlko", Wm, H)
     X2 = torch.einsum("ly,zlko->zyko", By, _bf(G))
-    S = _bf(torch.einsum("kx,zyko->zyxo", Bx, _bf(X2))).permute(0, 3, 1, 2)  # [z, c, y, x]
+    S = torch.einsum("kx,zyko->zyxo", Bx, _bf(X2)).permute(0, 3, 1, 2)  # [z, c, y, x], FP32 (added in the conv's sums)
     h2 = _bf(gelu(conv(h1, _bf(p["w2"]), p["b2"], padding=1) + S))
     h3 = _bf(gelu(conv(h2, _bf(p["w3"]), p["b3"], padding=1)))
     return (torch.einsum("zcyx,c->zyx", h3, p["wp"]) + p["bp"]) * np.float32(oscale)

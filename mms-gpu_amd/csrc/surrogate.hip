@@ -47,10 +47,13 @@ typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 __device__ __forceinline__ f32x4 mfma_bf16(bf16x8 a, bf16x8 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
-__device__ __forceinline__ unsigned short f2bf(float x) {  // round to nearest even
-    unsigned u = __float_as_uint(x);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
+// round to nearest even (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ unsigned short f2bf(float x) { return __builtin_bit_cast(unsigned short, (__bf16)x); }
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned pack2(float a, float b) {
+    return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2{a, b}), bf16x2));
 }
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 // GELU, tanh form: 0.5 x (1 + tanh(u)) = x s, s = 1 / (1 + exp(-2u)), u =
@@ -66,9 +69,7 @@ __device__ __forceinline__ float gelu_grad(float x) {
     const float s = gelu_s(x);
     return s + 2.0f * x * s * (1.0f - s) * 0.7978845608f * (1.0f + 0.134145f * x * x);
 }
-__device__ __forceinline__ uint2 pack4(const float (&v)[4]) {
-    return make_uint2(f2bf(v[0]) | (unsigned)f2bf(v[1]) << 16, f2bf(v[2]) | (unsigned)f2bf(v[3]) << 16);
-}
+__device__ __forceinline__ uint2 pack4(const float (&v)[4]) { return make_uint2(pack2(v[0], v[1]), pack2(v[2], v[3])); }
 __device__ __forceinline__ void unpack4(uint2 u, float (&v)[4]) {
     v[0] = bf2f((unsigned short)(u.x & 0xFFFF));
     v[1] = bf2f((unsigned short)(u.x >> 16));
@@ -204,6 +205,12 @@ constexpr int WCH = HS * HS * 8;                 // 16-byte chunks of a window
 constexpr int WPT = (WCH + 255) / 256;           // per thread (11)
 constexpr size_t CONV_LDS = sizeof(unsigned short) * ((size_t)HS * HS * CP + 9 * C * CP);
 
+// Software-pipelined over the tiles a workgroup walks: the epilogue of tile
+// t - 1 (held in a second set of accumulators) is computed between the
+// k steps of tile t, so its VALU work (GELU, conversions) and stores issue
+// while the matrix cores run; the stores go through buffer resources whose
+// range check discards the lanes outside the film (and every store of the
+// first, empty, held tile).
 template <int EPI>
 __global__ __launch_bounds__(256) void sur_conv_kernel(ConvArgs A) {
     extern __shared__ __attribute__((aligned(16))) unsigned short lds[];
@@ -239,6 +246,71 @@ __global__ __launch_bounds__(256) void sur_conv_kernel(ConvArgs A) {
         }
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j = lane & 15, g = lane >> 4;
+    // per-lane constants of the epilogue: channels 16 r + 4 g + i
+    float bias_c[4][4], wp_c[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            bias_c[r][i] = EPI == 2 ? 0.0f : A.bias[16 * r + 4 * g + i];
+            wp_c[r][i] = EPI == 1 ? A.wp[16 * r + 4 * g + i] : 0.0f;
+        }
+    const float bp = EPI == 1 ? A.bp[0] : 0.0f;
+    // buffer resources over the outputs (num_records 0 when absent: every store discarded)
+    const long npix = (long)gridDim.z * yd * xd;
+    const int nact = (int)(npix * C * 2), nphi = (int)(npix * 4);
+    const __amdgpu_buffer_rsrc_t r_out = __builtin_amdgcn_make_buffer_rsrc(A.out, (short)0, A.out ? nact : 0, 0x00020000);
+    const __amdgpu_buffer_rsrc_t r_zs = __builtin_amdgcn_make_buffer_rsrc(A.zsave, (short)0, A.zsave ? nact : 0, 0x00020000);
+    const __amdgpu_buffer_rsrc_t r_phi = __builtin_amdgcn_make_buffer_rsrc(A.phi, (short)0, A.phi ? nphi : 0, 0x00020000);
+    const bool spec = EPI != 1 && A.x2t != nullptr;  // uniform
+    // the held tile: accumulators, origin, saved pre-activations; `have` false
+    // for the empty tile before the first (its stores are discarded)
+    f32x4 hacc[4][4];
+    uint2 hz[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            hacc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+            hz[c][r] = make_uint2(0u, 0u);
+        }
+    int hx0 = 0, hy0 = 0;
+    bool have = false;
+    float proj = 0.0f;
+    // one 4-channel block (r, c) of the held tile's epilogue
+    auto epi_block = [&](int r, int c) {
+        const int gx = hx0 + j, gy = hy0 + 4 * wave + c;
+        const bool ok = have && gx < xd && gy < yd;
+        const long pix = ((long)z * yd + gy) * xd + gx;
+        const int bo = ok ? (int)((pix * C + 16 * r + 4 * g) * 2) : nact;  // byte offset, out of range if not ok
+        float zz[4], v[4];
+        if (EPI == 2) {
+            unpack4(hz[c][r], zz);
+#pragma unroll
+            for (int i = 0; i < 4; i++) v[i] = hacc[r][c][i] * gelu_grad(zz[i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                zz[i] = hacc[r][c][i] + bias_c[r][i];
+                v[i] = gelu(zz[i]);
+                if (EPI == 1) proj += bf2f(f2bf(v[i])) * wp_c[r][i];
+            }
+            const uint2 zp = pack4(zz);
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, zp), r_zs, bo, 0, 0);
+        }
+        if (EPI != 1) {
+            const uint2 vp = pack4(v);
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, vp), r_out, bo, 0, 0);
+        }
+        if (EPI == 1 && r == 3) {
+            // the 64 channels of pixel j are spread over lanes j, j + 16, j + 32, j + 48
+            proj += __shfl_xor(proj, 16);
+            proj += __shfl_xor(proj, 32);
+            const int po = (ok && g == 0) ? (int)(pix * 4) : nphi;
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, (proj + bp) * A.oscale), r_phi, po, 0, 0);
+            proj = 0.0f;
+        }
+    };
     uint4 nxt[WPT];
     int t = blockIdx.x;
     if (t < ntile) load_win(t, nxt);
@@ -252,11 +324,10 @@ __global__ __launch_bounds__(256) void sur_conv_kernel(ConvArgs A) {
         __syncthreads();
         if (t + (int)gridDim.x < ntile) load_win(t + gridDim.x, nxt);  // in flight during the products
         const int x0 = (t % ntx) * TS, y0 = (t / ntx) * TS;
-        // the epilogue's saved pre-activations and the spectral operands of
-        // this lane's outputs, loaded now so the products hide their latency
+        // the saved pre-activations and the spectral operands of this tile,
+        // loaded now so the products hide their latency
         uint2 pzin[4][4];
         bf16x8 sa[4][4], sb;
-        const bool spec = EPI != 1 && A.x2t != nullptr;  // uniform
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             const int gx = x0 + j, gy = y0 + 4 * wave + c;
@@ -266,9 +337,8 @@ __global__ __launch_bounds__(256) void sur_conv_kernel(ConvArgs A) {
             const long srow = (long)z * yd + (gy < yd ? gy : yd - 1);
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const long o = pix * C + 16 * r + 4 * g;
                 pzin[c][r] = make_uint2(0u, 0u);
-                if (EPI == 2 && ok) pzin[c][r] = *reinterpret_cast<const uint2 *>(A.zin + o);
+                if (EPI == 2 && ok) pzin[c][r] = *reinterpret_cast<const uint2 *>(A.zin + pix * C + 16 * r + 4 * g);
                 if (spec) sa[c][r] = *reinterpret_cast<const bf16x8 *>(A.x2t + (srow * C + 16 * r + j) * 32 + 8 * g);
             }
         }
@@ -296,6 +366,8 @@ __global__ __launch_bounds__(256) void sur_conv_kernel(ConvArgs A) {
                 for (int r = 0; r < 4; r++)
 #pragma unroll
                     for (int c = 0; c < 4; c++) acc[r][c] = mfma_bf16(a[r], b[c], acc[r][c]);
+                const int ks = 2 * tap + kb;
+                if (ks < 16) epi_block(ks & 3, ks >> 2);  // the held tile, block by block
             }
         }
         if (spec) {
@@ -304,41 +376,20 @@ __global__ __launch_bounds__(256) void sur_conv_kernel(ConvArgs A) {
 #pragma unroll
                 for (int c = 0; c < 4; c++) acc[r][c] = mfma_bf16(sa[c][r], sb, acc[r][c]);
         }
-        // epilogue: lane (j, g) holds channels 16 r + 4 g + i of pixel (tile row 4 wave + c, column j)
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const int gx = x0 + j, gy = y0 + 4 * wave + c;
-            const bool ok = gx < xd && gy < yd;
-            const long pix = ((long)z * yd + gy) * xd + gx;
-            float proj = 0.0f;
+        for (int r = 0; r < 4; r++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const long o = pix * C + 16 * r + 4 * g;
-                float zz[4] = {0.f, 0.f, 0.f, 0.f}, v[4];
-                if (EPI == 2) {
-                    unpack4(pzin[c][r], zz);
-#pragma unroll
-                    for (int i = 0; i < 4; i++) v[i] = ok ? acc[r][c][i] * gelu_grad(zz[i]) : 0.0f;
-                } else {
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const int co = 16 * r + 4 * g + i;
-                        zz[i] = acc[r][c][i] + A.bias[co];
-                        v[i] = gelu(zz[i]);
-                        if (EPI == 1) proj += bf2f(f2bf(v[i])) * A.wp[co];
-                    }
-                    if (A.zsave && ok) *reinterpret_cast<uint2 *>(A.zsave + o) = pack4(zz);
-                }
-                if (EPI != 1 && ok) *reinterpret_cast<uint2 *>(A.out + o) = pack4(v);
+            for (int c = 0; c < 4; c++) {
+                hacc[r][c] = acc[r][c];
+                hz[c][r] = pzin[c][r];
             }
-            if (EPI == 1) {
-                // the 64 channels of pixel j are spread over lanes j, j + 16, j + 32, j + 48
-                proj += __shfl_xor(proj, 16);
-                proj += __shfl_xor(proj, 32);
-                if (g == 0 && ok) A.phi[pix] = (proj + A.bp[0]) * A.oscale;
-            }
-        }
+        hx0 = x0;
+        hy0 = y0;
+        have = true;
     }
+    // the last tile's epilogue
+#pragma unroll
+    for (int ks = 0; ks < 16; ks++) epi_block(ks & 3, ks >> 2);
 }
 
 // ------------------------------------------------- weight gradients --
@@ -790,6 +841,8 @@ int launch_wgrad(SurrogatePlan &sp, const unsigned short *dz, const unsigned sho
 
 int launch_conv(int epi, const SurrogatePlan &sp, const ConvArgs &a, hipStream_t s) {
     const Grid &g = sp.g;
+    // the kernel's output buffer resources take 32-bit byte sizes
+    MMS_CHECK((long)g.n() * C * 2 < (1L << 31), "surrogate: grid too large for one activation buffer");
     const int ntile = ((g.xd + TS - 1) / TS) * ((g.yd + TS - 1) / TS);
     // one resident workgroup per CU per layer slice walks the tiles
     int per_layer = (256 + g.zd - 1) / g.zd;

@@ -176,7 +176,7 @@ __global__ __launch_bounds__(256) void sur_lift_kernel(const float *__restrict__
 
 // ------------------------------------------------------------- conv 3x3 --
 // 64 -> 64 channels on 16 x 16 pixel tiles.  A workgroup loads the nine
-// 64 x 64 weight taps into LDS once (row stride 72 bf16) and then walks
+// 64 x 64 weight taps into LDS once (row stride CP) and then walks
 // tiles blockIdx.x, blockIdx.x + gridDim.x, ... of layer blockIdx.z: the
 // next tile's 18 x 18 x 64 input window is loaded into registers while the
 // current one is multiplied, then stored to LDS.  Wave w computes D[64 co]
@@ -651,7 +651,13 @@ struct GemmArgs {
     int M, N, K;
 };
 template <typename T> __device__ __forceinline__ unsigned short to_bf(T v);
-template <> __device__ __forceinline__ unsigned short to_bf<float>(float v) { return f2bf(v); }
+// integer round to nearest even (the same bits as f2bf): with the hardware
+// conversion the compiler waits for each staged load inside its bounds branch
+template <> __device__ __forceinline__ unsigned short to_bf<float>(float v) {
+    unsigned u = __float_as_uint(v);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
 template <> __device__ __forceinline__ unsigned short to_bf<unsigned short>(unsigned short v) { return v; }
 
 template <typename TA, typename TB, typename TC>

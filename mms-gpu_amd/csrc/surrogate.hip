@@ -58,10 +58,11 @@ __device__ __forceinline__ unsigned pack2(float a, float b) {
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 // GELU, tanh form: 0.5 x (1 + tanh(u)) = x s, s = 1 / (1 + exp(-2u)), u =
 // sqrt(2/pi) (x + 0.044715 x^3), with the hardware exp2 and reciprocal
-// (within a few ulp of the library tanh form; large |x| gives x or -0)
+// (within a few ulp of the library tanh form; large |x| gives x or -0):
+// exp(-2u) = 2^(x (a + b x^2)), a = -2 sqrt(2/pi) log2(e), b = 0.044715 a
 __device__ __forceinline__ float gelu_s(float x) {
-    const float u2 = -1.5957691216f * (x + 0.044715f * x * x * x);
-    return __builtin_amdgcn_rcpf(1.0f + __expf(u2));
+    const float q = x * fmaf(x * x, -0.10294324f, -2.3022082f);
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(q));
 }
 __device__ __forceinline__ float gelu(float x) { return x * gelu_s(x); }
 // d/dx of the same form: s + 2 x s (1 - s) sqrt(2/pi) (1 + 3 * 0.044715 x^2)
@@ -219,16 +220,37 @@ __global__ __launch_bounds__(256) void sur_conv_kernel(ConvArgs A) {
     const int xd = A.xd, yd = A.yd;
     const unsigned short *__restrict__ in = A.in;
     const int z = blockIdx.z, ntx = (xd + TS - 1) / TS, nty = (yd + TS - 1) / TS, ntile = ntx * nty;
+    // window position (sy << 16 | sx) of this thread's chunk q, -1 past the
+    // window: computed once (the backward pass, short of registers, each time)
+    constexpr bool HOIST = EPI != 2;
+    int wpos[WPT];
+#pragma unroll
+    for (int q = 0; q < WPT; q++) {
+        const int e = threadIdx.x + 256 * q, p = e >> 3;
+        wpos[q] = (HOIST && e < WCH) ? ((p / HS) << 16) | (p % HS) : -1;
+    }
     // the window chunks of tile t into registers (zero outside the film)
     auto load_win = [&](int t, uint4 (&v)[WPT]) {
         const int x0 = (t % ntx) * TS, y0 = (t / ntx) * TS;
+        if constexpr (HOIST) {
+            const unsigned short *base = in + (((long)z * yd + y0 - 1) * xd + x0 - 1) * C + 8 * (threadIdx.x & 7);
 #pragma unroll
-        for (int q = 0; q < WPT; q++) {
-            const int e = threadIdx.x + 256 * q, p = e >> 3, ch = e & 7, sy = p / HS, sx = p % HS;
-            const int gx = x0 - 1 + sx, gy = y0 - 1 + sy;
-            v[q] = make_uint4(0u, 0u, 0u, 0u);
-            if (e < WCH && gx >= 0 && gx < xd && gy >= 0 && gy < yd)
-                v[q] = *reinterpret_cast<const uint4 *>(in + (((long)z * yd + gy) * xd + gx) * C + 8 * ch);
+            for (int q = 0; q < WPT; q++) {
+                const int sy = wpos[q] >> 16, sx = wpos[q] & 0xFFFF;
+                const int gx = x0 - 1 + sx, gy = y0 - 1 + sy;
+                v[q] = make_uint4(0u, 0u, 0u, 0u);
+                if (wpos[q] >= 0 && gx >= 0 && gx < xd && gy >= 0 && gy < yd)
+                    v[q] = *reinterpret_cast<const uint4 *>(base + (sy * xd + sx) * C);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < WPT; q++) {
+                const int e = threadIdx.x + 256 * q, p = e >> 3, ch = e & 7, sy = p / HS, sx = p % HS;
+                const int gx = x0 - 1 + sx, gy = y0 - 1 + sy;
+                v[q] = make_uint4(0u, 0u, 0u, 0u);
+                if (e < WCH && gx >= 0 && gx < xd && gy >= 0 && gy < yd)
+                    v[q] = *reinterpret_cast<const uint4 *>(in + (((long)z * yd + gy) * xd + gx) * C + 8 * ch);
+            }
         }
     };
     {   // weights: 18 chunks per thread, all loads in flight before the stores

@@ -158,6 +158,7 @@ struct SurrogatePlan {
     Grid g{};
     int zf = 1;
     int nmode = 0;
+    int xpad = 0;  // row length of the zero-padded Bx (d_bx [32][xpad])
     bool train = false;
     float *d_param = nullptr;
     unsigned short *d_lift = nullptr, *d_wm = nullptr, *d_bx = nullptr, *d_by = nullptr, *d_bxt = nullptr;

@@ -894,18 +894,105 @@ __global__ __launch_bounds__(256) void sur_x2t_kernel(const float *__restrict__ 
     for (int e = threadIdx.x; e < 32 * C; e += 256) dst[e] = f2bf(t[e % 32][e / 32]);
 }
 
+// Forward transform over x of every (z, y) row of a BF16 NHWC array:
+// X1[row][k][c] = sum_x Bx[k][x] in[row][x][c] (FP32).  Persistent
+// workgroups walk (row, 128-pixel chunk) in order.  A chunk's 128 x 64 values
+// are loaded as 16-byte pieces two chunks ahead and stored transposed into
+// LDS as [c][x] (four consecutive x per 8-byte store), so that the MFMA
+// operands are 16-byte reads; wave w takes the chunk's 32 pixels 32w..32w+31
+// (one k step) for all 64 channels and both 16-mode tiles, its Bx operand read
+// from the zero-padded basis [32][xpad] in global memory.  The four waves'
+// partial sums of a row are added in LDS in wave order.
+constexpr int XFC = 128, XFS = XFC + 8;
+__global__ __launch_bounds__(256) void sur_xfwd_kernel(const unsigned short *__restrict__ in,
+                                                       const unsigned short *__restrict__ bxp, float *__restrict__ x1,
+                                                       int X, int xpad, int rows) {
+    __shared__ __attribute__((aligned(16))) unsigned short hT[C][XFS];
+    __shared__ __attribute__((aligned(16))) float red[4][32 * C];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j = lane & 15, g = lane >> 4;
+    const int xq = threadIdx.x >> 3, cc = threadIdx.x & 7;  // this thread's pieces: x = 4 xq + u, channels 8 cc..
+    const int nch = (X + XFC - 1) / XFC;
+    const int nmine = rows > (int)blockIdx.x ? (rows - 1 - (int)blockIdx.x) / (int)gridDim.x + 1 : 0;
+    const int total = nmine * nch;  // (row, chunk) steps of this workgroup
+    auto load = [&](int s, uint4 (&v)[4]) {
+        if (s >= total) return;
+        const int row = blockIdx.x + (s / nch) * gridDim.x, x0 = (s % nch) * XFC;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int x = x0 + 4 * xq + u;
+            v[u] = make_uint4(0u, 0u, 0u, 0u);
+            if (x < X) v[u] = *reinterpret_cast<const uint4 *>(in + ((long)row * X + x) * C + 8 * cc);
+        }
+    };
+    uint4 va[4], vb[4];
+    load(0, va);
+    load(1, vb);
+    f32x4 acc[4][2];
+    for (int s = 0; s < total; s++) {
+        const int x0 = (s % nch) * XFC;
+        if (x0 == 0) {
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++) acc[ct][0] = acc[ct][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+        __syncthreads();  // the previous chunk's operand reads are done
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            uint2 w2;
+            const unsigned sh = (t & 1) * 16;
+            const unsigned e0 = ((t >> 1) == 0 ? va[0].x : (t >> 1) == 1 ? va[0].y : (t >> 1) == 2 ? va[0].z : va[0].w) >> sh;
+            const unsigned e1 = ((t >> 1) == 0 ? va[1].x : (t >> 1) == 1 ? va[1].y : (t >> 1) == 2 ? va[1].z : va[1].w) >> sh;
+            const unsigned e2 = ((t >> 1) == 0 ? va[2].x : (t >> 1) == 1 ? va[2].y : (t >> 1) == 2 ? va[2].z : va[2].w) >> sh;
+            const unsigned e3 = ((t >> 1) == 0 ? va[3].x : (t >> 1) == 1 ? va[3].y : (t >> 1) == 2 ? va[3].z : va[3].w) >> sh;
+            w2.x = (e0 & 0xFFFFu) | (e1 << 16);
+            w2.y = (e2 & 0xFFFFu) | (e3 << 16);
+            *reinterpret_cast<uint2 *>(&hT[8 * cc + t][4 * xq]) = w2;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; u++) va[u] = vb[u];
+        load(s + 2, vb);  // two chunks ahead, in flight during the products
+        const int xb = x0 + 32 * wave + 8 * g;  // this lane's 8 pixels of the k step
+        bf16x8 b[2];
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++)
+            b[nt] = *reinterpret_cast<const bf16x8 *>(bxp + (long)(16 * nt + j) * xpad + xb);
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&hT[16 * ct + j][32 * wave + 8 * g]);
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++) acc[ct][nt] = mfma_bf16(a, b[nt], acc[ct][nt]);
+        }
+        if (x0 + XFC >= X) {
+            // row done: D[c][k] of lane (j, g) = channels 16 ct + 4 g + i, mode 16 nt + j
+            const int row = blockIdx.x + (s / nch) * gridDim.x;
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++)
+#pragma unroll
+                for (int nt = 0; nt < 2; nt++)
+                    *reinterpret_cast<f32x4 *>(&red[wave][(16 * nt + j) * C + 16 * ct + 4 * g]) = acc[ct][nt];
+            __syncthreads();
+            float *dst = x1 + (long)row * 32 * C;
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const int e = 4 * (threadIdx.x + 256 * q);
+                f32x4 v = *reinterpret_cast<const f32x4 *>(&red[0][e]);
+#pragma unroll
+                for (int w = 1; w < 4; w++) v += *reinterpret_cast<const f32x4 *>(&red[w][e]);
+                *reinterpret_cast<f32x4 *>(dst + e) = v;
+            }
+        }
+    }
+}
+
 // The spectral branch's transform chain of the BF16 NHWC array `in`: X1 = Bx in,
 // hbuf = By X1, gbuf = mix(hbuf) (transposed mixing when tr), X2 = By^T gbuf,
 // then X2^T in BF16 (sp.d_x2t) for the conv kernel, which adds Bx^T X2.
 int spectral(SurrogatePlan &sp, const unsigned short *in, float *hbuf, float *gbuf, bool tr, hipStream_t s) {
     const int X = sp.g.xd, Y = sp.g.yd, Z = sp.g.zd;
-    {
-        GemmArgs a{};  // X1[z][y][kx][c] = sum_x Bx[kx][x] in[z][y][x][c]
-        a.A = sp.d_bx; a.sam = X; a.sak = 1; a.sab = 0;
-        a.B = in; a.sbk = C; a.sbn = 1; a.sbb = (long)X * C;
-        a.C = sp.d_x1; a.scm = C; a.scn = 1; a.scb = 32L * C;
-        a.M = 32; a.N = C; a.K = X;
-        if (gemm<unsigned short, unsigned short, float>(a, Z * Y, s)) return 1;
+    {   // X1[z][y][kx][c] = sum_x Bx[kx][x] in[z][y][x][c]
+        const int rows = Z * Y, nb = rows < 512 ? rows : 512;
+        hipLaunchKernelGGL(sur_xfwd_kernel, dim3(nb), dim3(256), 0, s, in, sp.d_bx, sp.d_x1, X, sp.xpad, rows);
+        MMS_HIP(hipGetLastError());
     }
     {
         GemmArgs a{};  // H[z][ky][(kx, c)] = sum_y By[ky][y] X1[z][y][(kx, c)]
@@ -1048,9 +1135,14 @@ int SurrogatePlan::init(const Grid &g_, int zf_, const float *weights, bool trai
             }
         return b;
     };
-    std::vector<unsigned short> bx = basis(g.xd), by = basis(g.yd), bxt(bx.size());
+    std::vector<unsigned short> bx0 = basis(g.xd), by = basis(g.yd), bxt(bx0.size());
     for (int k = 0; k < 32; k++)
-        for (int x = 0; x < g.xd; x++) bxt[(size_t)x * 32 + k] = bx[(size_t)k * g.xd + x];
+        for (int x = 0; x < g.xd; x++) bxt[(size_t)x * 32 + k] = bx0[(size_t)k * g.xd + x];
+    // Bx rows zero-padded to a multiple of the forward transform's 128-pixel chunk
+    xpad = (g.xd + 127) / 128 * 128;
+    std::vector<unsigned short> bx((size_t)32 * xpad, 0);
+    for (int k = 0; k < 32; k++)
+        for (int x = 0; x < g.xd; x++) bx[(size_t)k * xpad + x] = bx0[(size_t)k * g.xd + x];
     MMS_HIP(hipMalloc(&d_bx, bx.size() * 2));
     MMS_HIP(hipMalloc(&d_bxt, bxt.size() * 2));
     MMS_HIP(hipMemcpyAsync(d_bxt, bxt.data(), bxt.size() * 2, hipMemcpyHostToDevice, s));

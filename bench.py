@@ -407,6 +407,44 @@ def surrogate_timing(mg, wl, w, args, plain_sps, weights=None):
 VALU_FP32_PEAK_TFLOPS = 78.6  # non-packed FP32 FMA (packed FP32 is not used by this build)
 
 
+STREAM_CODE = r"""
+import json, torch
+n = 256 * 1024 * 1024
+a = torch.ones(n, dtype=torch.float32, device="cuda")
+b = torch.ones_like(a)
+c = torch.empty_like(a)
+out = {}
+for name, fn, nbytes in (("copy", lambda: c.copy_(a), 8 * n), ("add", lambda: torch.add(a, b, out=c), 12 * n)):
+    for _ in range(3):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(16):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    out[name + "_GBs"] = round(nbytes / (e0.elapsed_time(e1) / 16 * 1e-3) / 1e9, 1)
+print(json.dumps(out))
+"""
+
+
+def hbm_stream_rates():
+    """Practical HBM rates of plain streaming kernels on this GPU, for
+    context beside the published 8 TB/s: PyTorch's device copy (c = a) and
+    add (c = a + b) over 1 GiB FP32 arrays, bytes moved / time (HIP events,
+    16 launches after 3 warm-up launches).  Run in a child process: PyTorch
+    brings its own HIP runtime, which does not share a process with the
+    library's."""
+    import subprocess
+    try:
+        r = subprocess.run([sys.executable, "-c", STREAM_CODE], capture_output=True, text=True, timeout=180)
+        return json.loads(r.stdout.strip().splitlines()[-1])
+    except (subprocess.SubprocessError, ValueError, IndexError) as e:
+        log(f"HBM streaming rates not measured: {e}")
+        return None
+
+
 def bench_c5(args, mg, wl):
     """512 independent films (16^2..96^2 x 3, discs / squares / U shapes,
     field ramp -50..+50 mT): direct sum below 4096 cells, FMM P = 4 above,
@@ -520,6 +558,7 @@ def main():
                     help="c3: the 1024x1024x4 film (default); c5: the 512-film sweep")
     ap.add_argument("--serial", action="store_true",
                     help="run every kernel in order on one stream (MMS_GPU_SERIAL=1), e.g. under a kernel trace")
+    ap.add_argument("--no-stream", action="store_true", help="skip the plain streaming-kernel HBM rates")
     ap.add_argument("--pmc-calibration", action="store_true",
                     help="end with plain device copies of 256 MiB and 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -630,6 +669,16 @@ def main():
     if args.pmc_calibration:
         pmc_calibration_copies()
 
+    stream = None
+    if not args.no_stream:
+        stream = hbm_stream_rates()
+    if stream:
+        stream["note"] = ("plain streaming kernels (PyTorch device copy and add over 1 GiB FP32 arrays): the "
+                          "practical HBM rate beside the published 8000 GB/s peak")
+        if roof and roof["unit"] == "GB/s":
+            roof["frac_of_stream_add"] = round(roof["achieved"] / stream["add_GBs"], 4)
+        log(f"HBM streaming rates: copy {stream['copy_GBs']} GB/s, add {stream['add_GBs']} GB/s")
+
     out = {
         "metric": "LLG RK4 steps/s at 1024²×4 cells, surrogate+FMM vs plain FMM at ≤1e-3 field err",
         "value": round(steps_per_s, 3),
@@ -657,6 +706,7 @@ def main():
                            "size, whatever its accuracy); `value` is the plain GPU FMM form"),
         "surrogate": sur,
         "roofline": roof,
+        "hbm_stream": stream,
         "kernels": others,
         "kernel_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},
         "field_error": err,

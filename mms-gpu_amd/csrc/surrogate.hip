@@ -212,7 +212,7 @@ constexpr size_t CONV_LDS = sizeof(unsigned short) * ((size_t)HS * HS * CP + 9 *
 // while the matrix cores run; the stores go through buffer resources whose
 // range check discards the lanes outside the film (and every store of the
 // first, empty, held tile).
-template <int EPI>
+template <int EPI, bool SPEC>
 __global__ __launch_bounds__(256) void sur_conv_kernel(ConvArgs A) {
     extern __shared__ __attribute__((aligned(16))) unsigned short lds[];
     unsigned short *win = lds;                 // [HS * HS][CP]
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(256) void sur_conv_kernel(ConvArgs A) {
     const __amdgpu_buffer_rsrc_t r_out = __builtin_amdgcn_make_buffer_rsrc(A.out, (short)0, A.out ? nact : 0, 0x00020000);
     const __amdgpu_buffer_rsrc_t r_zs = __builtin_amdgcn_make_buffer_rsrc(A.zsave, (short)0, A.zsave ? nact : 0, 0x00020000);
     const __amdgpu_buffer_rsrc_t r_phi = __builtin_amdgcn_make_buffer_rsrc(A.phi, (short)0, A.phi ? nphi : 0, 0x00020000);
-    const bool spec = EPI != 1 && A.x2t != nullptr;  // uniform
+    constexpr bool spec = SPEC && EPI != 1;  // the spectral operands are given (x2t, bxt)
     // the held tile: accumulators, origin, saved pre-activations; `have` false
     // for the empty tile before the first (its stores are discarded)
     f32x4 hacc[4][4];
@@ -390,13 +390,15 @@ __global__ __launch_bounds__(256) void sur_conv_kernel(ConvArgs A) {
                     for (int c = 0; c < 4; c++) acc[r][c] = mfma_bf16(a[r], b[c], acc[r][c]);
                 const int ks = 2 * tap + kb;
                 if (ks < 16) epi_block(ks & 3, ks >> 2);  // the held tile, block by block
+                if (spec && ks == 9) {
+                    // the spectral term, midway: its operands have arrived and the
+                    // loop does not end on these products
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+#pragma unroll
+                        for (int c = 0; c < 4; c++) acc[r][c] = mfma_bf16(sa[c][r], sb, acc[r][c]);
+                }
             }
-        }
-        if (spec) {
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-#pragma unroll
-                for (int c = 0; c < 4; c++) acc[r][c] = mfma_bf16(sa[c][r], sb, acc[r][c]);
         }
 #pragma unroll
         for (int r = 0; r < 4; r++)
@@ -408,6 +410,11 @@ __global__ __launch_bounds__(256) void sur_conv_kernel(ConvArgs A) {
         hx0 = x0;
         hy0 = y0;
         have = true;
+        // the next iteration starts by reading the accumulators these
+        // products wrote: 19 wait states after the last MFMA, which the
+        // compiler does not insert across the loop's back edge
+        __builtin_amdgcn_sched_barrier(0);
+        asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 2" ::: "memory");
     }
     // the last tile's epilogue
 #pragma unroll
@@ -876,9 +883,12 @@ int launch_conv(int epi, const SurrogatePlan &sp, const ConvArgs &a, hipStream_t
     int per_layer = (256 + g.zd - 1) / g.zd;
     if (per_layer > ntile) per_layer = ntile;
     dim3 grid(per_layer, 1, g.zd);
-    if (epi == 0) hipLaunchKernelGGL((sur_conv_kernel<0>), grid, dim3(256), CONV_LDS, s, a);
-    else if (epi == 1) hipLaunchKernelGGL((sur_conv_kernel<1>), grid, dim3(256), CONV_LDS, s, a);
-    else hipLaunchKernelGGL((sur_conv_kernel<2>), grid, dim3(256), CONV_LDS, s, a);
+    const bool spec = a.x2t != nullptr;
+    if (epi == 0 && spec) hipLaunchKernelGGL((sur_conv_kernel<0, true>), grid, dim3(256), CONV_LDS, s, a);
+    else if (epi == 0) hipLaunchKernelGGL((sur_conv_kernel<0, false>), grid, dim3(256), CONV_LDS, s, a);
+    else if (epi == 1) hipLaunchKernelGGL((sur_conv_kernel<1, false>), grid, dim3(256), CONV_LDS, s, a);
+    else if (spec) hipLaunchKernelGGL((sur_conv_kernel<2, true>), grid, dim3(256), CONV_LDS, s, a);
+    else hipLaunchKernelGGL((sur_conv_kernel<2, false>), grid, dim3(256), CONV_LDS, s, a);
     MMS_HIP(hipGetLastError());
     return 0;
 }
@@ -1171,9 +1181,11 @@ int SurrogatePlan::init(const Grid &g_, int zf_, const float *weights, bool trai
     }
     static bool attr = false;
     if (!attr) {
-        for (const void *k : {reinterpret_cast<const void *>(&sur_conv_kernel<0>),
-                              reinterpret_cast<const void *>(&sur_conv_kernel<1>),
-                              reinterpret_cast<const void *>(&sur_conv_kernel<2>)})
+        for (const void *k : {reinterpret_cast<const void *>(&sur_conv_kernel<0, false>),
+                              reinterpret_cast<const void *>(&sur_conv_kernel<0, true>),
+                              reinterpret_cast<const void *>(&sur_conv_kernel<1, false>),
+                              reinterpret_cast<const void *>(&sur_conv_kernel<2, false>),
+                              reinterpret_cast<const void *>(&sur_conv_kernel<2, true>)})
             MMS_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)CONV_LDS));
         MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&sur_wgrad_kernel),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)WGRAD_LDS));

@@ -10,7 +10,7 @@ of the last accumulator register.  This script scans the device assembly
 (v_accvgpr_read / v_accvgpr_mov) an AGPR that an MFMA near the back-edge
 branch wrote, with fewer than REQUIRED wait states between them.  Each
 instruction in between counts one wait state, each MFMA at least four (its
-issue occupies the matrix pipe for at least four cycles).
+issue occupies the matrix pipe for at least four cycles), each s_nop N N + 1.
 
     python hazard_lint.py kernel.s [more.s ...]     exit status 1 on a finding
 """
@@ -30,7 +30,10 @@ def agprs(tok):
 
 
 def cost(ins):
-    return MFMA_MIN_WAIT if ins.startswith("v_mfma") else 1
+    if ins.startswith("v_mfma"):
+        return MFMA_MIN_WAIT
+    m = re.match(r"s_nop\s+(\d+)", ins)
+    return int(m.group(1)) + 1 if m else 1  # s_nop N: N + 1 wait states
 
 
 def instructions(lines, start, stop, step):

@@ -593,14 +593,29 @@ __global__ __launch_bounds__(256) void sur_lift_wgrad_kernel(const float *__rest
 }
 
 // out[i] = sum_b part[b * stride + i], i < len (fixed order: deterministic)
+// out[i] = sum of part[b * stride + i] over the nb partials in a fixed
+// order: four interleaved sums (b mod 4, four loads in flight per thread)
+// added in order.  64 outputs per workgroup; launch with sum_blocks(len).
 __global__ __launch_bounds__(256) void sur_sum_kernel(const float *__restrict__ part, long stride, int nb, long len,
                                                       float *__restrict__ out) {
-    const long i = blockIdx.x * 256L + threadIdx.x;
-    if (i >= len) return;
+    __shared__ float red[4][64];
+    const int o = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const long i = blockIdx.x * 64L + o;
     float s = 0.0f;
-    for (int b = 0; b < nb; b++) s += part[(long)b * stride + i];
-    out[i] = s;
+    if (i < len) {
+        int b = q;
+        for (; b + 12 < nb; b += 16) {
+            const float v0 = part[(long)b * stride + i], v1 = part[(long)(b + 4) * stride + i];
+            const float v2 = part[(long)(b + 8) * stride + i], v3 = part[(long)(b + 12) * stride + i];
+            s += v0; s += v1; s += v2; s += v3;
+        }
+        for (; b < nb; b += 4) s += part[(long)b * stride + i];
+    }
+    red[q][o] = s;
+    __syncthreads();
+    if (q == 0 && i < len) out[i] = ((red[0][o] + red[1][o]) + red[2][o]) + red[3][o];
 }
+inline unsigned sum_blocks(long len) { return (unsigned)((len + 63) / 64); }
 
 // ---------------------------------------------------- loss and head --
 // Per film f of npf cells: loss_f = sum (phi_s - phi_t)^2 inv_norm[f]; with dz3
@@ -868,7 +883,7 @@ int launch_wgrad(SurrogatePlan &sp, const unsigned short *dz, const unsigned sho
     const int nb = ntile < SurrogatePlan::kPartBlocks ? ntile : SurrogatePlan::kPartBlocks;
     hipLaunchKernelGGL(sur_wgrad_kernel, dim3(nb), dim3(256), WGRAD_LDS, s, dz, a, sp.d_wpart, g.xd, g.yd, g.zd);
     MMS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(sur_sum_kernel, dim3((unsigned)((WGRAD_PART + 255) / 256)), dim3(256), 0, s, sp.d_wpart,
+    hipLaunchKernelGGL(sur_sum_kernel, dim3(sum_blocks(WGRAD_PART)), dim3(256), 0, s, sp.d_wpart,
                        WGRAD_PART, nb, WGRAD_PART, grad);
     MMS_HIP(hipGetLastError());
     return 0;
@@ -1272,7 +1287,7 @@ int surrogate_backward(SurrogatePlan &sp, const float *rho, const float *phi_s, 
     const int nf = g.zd / sp.zf;
     // loss and projection: dz3 -> d_dza, dwp / dbp / loss partials -> d_hsum
     if (surrogate_losses(sp, phi_s, phi_t, inv_norm, gscale, oscale, true, s)) return 1;
-    hipLaunchKernelGGL(sur_sum_kernel, dim3(1), dim3(256), 0, s, sp.d_hpart, (long)HPART, nf * HB, (long)HPART,
+    hipLaunchKernelGGL(sur_sum_kernel, dim3(sum_blocks(HPART)), dim3(256), 0, s, sp.d_hpart, (long)HPART, nf * HB, (long)HPART,
                        sp.d_hsum);
     MMS_HIP(hipGetLastError());
     MMS_HIP(hipMemcpyAsync(grad + OFF_WP, sp.d_hsum, sizeof(float) * (C + 1), hipMemcpyDeviceToDevice, s));
@@ -1301,7 +1316,7 @@ int surrogate_backward(SurrogatePlan &sp, const float *rho, const float *phi_s, 
     hipLaunchKernelGGL(sur_lift_wgrad_kernel, dim3(nb), dim3(256), 0, s, rho, sp.d_dzb, sp.d_wpart, g.xd, g.yd, g.zd,
                        sp.zf, rscale);
     MMS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(sur_sum_kernel, dim3((unsigned)((LGRAD_PART + 255) / 256)), dim3(256), 0, s, sp.d_wpart,
+    hipLaunchKernelGGL(sur_sum_kernel, dim3(sum_blocks(LGRAD_PART)), dim3(256), 0, s, sp.d_wpart,
                        LGRAD_PART, nb, LGRAD_PART, grad + OFF_L);
     MMS_HIP(hipGetLastError());
     return 0;

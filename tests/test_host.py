@@ -200,3 +200,17 @@ def test_cli_rejects_bad_arguments(tmp_path):
     (tmp_path / "bad.png").write_bytes(b"not a png")
     r = subprocess.run([CLI, "bad.png"], cwd=str(tmp_path), capture_output=True, text=True, timeout=60)
     assert r.returncode != 0 and "not a PNG" in r.stderr
+
+
+def test_committed_traffic_summary_serves_the_default_bench_line():
+    """bench.py's roofline.traffic for the default form reads the committed
+    rocprofv3 summary: every kernel class it prices has a line keyed by the
+    workload name the bench asks for."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    workload = "c3_leaf8_fp16x2_f16x2near"
+    for kernel in ("stage", "div", "m2l", "p2p", "upward", "downward", "m2m"):
+        t = bench.pmc_traffic(kernel, workload)
+        assert t is not None and t > 0, kernel

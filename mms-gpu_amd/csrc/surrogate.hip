@@ -949,9 +949,18 @@ __global__ __launch_bounds__(256) void sur_xfwd_kernel(const unsigned short *__r
             if (x < X) v[u] = *reinterpret_cast<const uint4 *>(in + ((long)row * X + x) * C + 8 * cc);
         }
     };
+    // this lane's Bx operands of step s (8 pixels of the k step of wave w)
+    auto loadb = [&](int s, bf16x8 (&b)[2]) {
+        if (s >= total) return;
+        const int xb = (s % nch) * XFC + 32 * wave + 8 * g;
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++) b[nt] = *reinterpret_cast<const bf16x8 *>(bxp + (long)(16 * nt + j) * xpad + xb);
+    };
     uint4 va[4], vb[4];
+    bf16x8 bcur[2], bnxt[2];
     load(0, va);
     load(1, vb);
+    loadb(0, bcur);
     f32x4 acc[4][2];
     for (int s = 0; s < total; s++) {
         const int x0 = (s % nch) * XFC;
@@ -976,17 +985,15 @@ __global__ __launch_bounds__(256) void sur_xfwd_kernel(const unsigned short *__r
 #pragma unroll
         for (int u = 0; u < 4; u++) va[u] = vb[u];
         load(s + 2, vb);  // two chunks ahead, in flight during the products
-        const int xb = x0 + 32 * wave + 8 * g;  // this lane's 8 pixels of the k step
-        bf16x8 b[2];
-#pragma unroll
-        for (int nt = 0; nt < 2; nt++)
-            b[nt] = *reinterpret_cast<const bf16x8 *>(bxp + (long)(16 * nt + j) * xpad + xb);
+        loadb(s + 1, bnxt);
 #pragma unroll
         for (int ct = 0; ct < 4; ct++) {
             const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&hT[16 * ct + j][32 * wave + 8 * g]);
 #pragma unroll
-            for (int nt = 0; nt < 2; nt++) acc[ct][nt] = mfma_bf16(a, b[nt], acc[ct][nt]);
+            for (int nt = 0; nt < 2; nt++) acc[ct][nt] = mfma_bf16(a, bcur[nt], acc[ct][nt]);
         }
+        bcur[0] = bnxt[0];
+        bcur[1] = bnxt[1];
         if (x0 + XFC >= X) {
             // row done: D[c][k] of lane (j, g) = channels 16 ct + 4 g + i, mode 16 nt + j
             const int row = blockIdx.x + (s / nch) * gridDim.x;

@@ -95,7 +95,23 @@ constexpr long OFF_L = 0, OFF_BL = OFF_L + 32L * C, OFF_W1 = OFF_BL + C, OFF_W2 
 constexpr long OFF_W[3] = {OFF_W1, OFF_W2, OFF_W3};
 
 // input planes of layer z at (gx, gy): own charge, layers at distance 1,
-// layers at distance >= 2 of the film of zf layers (times rscale)
+// layers at distance >= 2 of the film of zf layers (times rscale); the
+// film's charges at (gx, gy) are given in r[0 .. zf)
+template <int ZM>
+__device__ __forceinline__ void combine_planes(const float (&r)[ZM], int zl, int zf, float rscale, float (&v)[3]) {
+    float p0 = 0.0f, p1 = 0.0f, p2 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < ZM; k++) {
+        if (k >= zf) break;
+        const int d = k > zl ? k - zl : zl - k;
+        if (d == 0) p0 = r[k];
+        else if (d == 1) p1 += r[k];
+        else p2 += r[k];
+    }
+    v[0] = p0 * rscale;
+    v[1] = p1 * rscale;
+    v[2] = p2 * rscale;
+}
 __device__ __forceinline__ void input_planes(const float *__restrict__ rho, int z, int zf, long plane, long c2,
                                              float rscale, float (&v)[3]) {
     const int f0 = (z / zf) * zf, zl = z - f0;
@@ -112,10 +128,36 @@ __device__ __forceinline__ void input_planes(const float *__restrict__ rho, int 
     v[2] = p2 * rscale;
 }
 
-// the 18 x 18 window of the three input planes around the tile (x0, y0) of layer z
+// the 18 x 18 window of the three input planes around the tile (x0, y0) of
+// layer z; films of up to 8 layers issue every charge load of a thread's
+// pixels before combining them (the same sums in the same order)
 __device__ __forceinline__ void stage_planes(const float *__restrict__ rho, float (*in)[HS][HS], int z, int zf,
                                              int x0, int y0, int xd, int yd, float rscale) {
     const long plane = (long)xd * yd;
+    constexpr int ZM = 8, NP = (HS * HS + 255) / 256;
+    if (zf <= ZM) {
+        const int f0 = (z / zf) * zf, zl = z - f0;
+        float r[NP][ZM];
+#pragma unroll
+        for (int q = 0; q < NP; q++) {
+            const int e = threadIdx.x + 256 * q, sy = e / HS, sx = e % HS, gx = x0 - 1 + sx, gy = y0 - 1 + sy;
+            const bool ok = e < HS * HS && gx >= 0 && gx < xd && gy >= 0 && gy < yd;
+            const long c2 = (long)gy * xd + gx;
+#pragma unroll
+            for (int k = 0; k < ZM; k++) r[q][k] = (ok && k < zf) ? rho[(long)(f0 + k) * plane + c2] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < NP; q++) {
+            const int e = threadIdx.x + 256 * q;
+            if (e >= HS * HS) break;
+            float v[3];
+            combine_planes<ZM>(r[q], zl, zf, rscale, v);
+            in[0][e / HS][e % HS] = v[0];
+            in[1][e / HS][e % HS] = v[1];
+            in[2][e / HS][e % HS] = v[2];
+        }
+        return;
+    }
     for (int e = threadIdx.x; e < HS * HS; e += 256) {
         const int sy = e / HS, sx = e % HS, gx = x0 - 1 + sx, gy = y0 - 1 + sy;
         float v[3] = {0.0f, 0.0f, 0.0f};

@@ -463,6 +463,242 @@ __global__ __launch_bounds__(256) void sur_conv_kernel(ConvArgs A) {
     for (int ks = 0; ks < 16; ks++) epi_block(ks & 3, ks >> 2);
 }
 
+// Eight-wave form of the same convolution (512 threads, two waves per SIMD):
+// wave w takes pixel rows 4 (w & 3) .. + 3 of the tile and output channels
+// 32 (w >> 2) .. + 31, so each SIMD holds two waves whose products,
+// epilogue arithmetic and LDS reads interleave.  With EPI 1 the two channel
+// halves' projections of a pixel meet in LDS (half 0 + half 1) and are stored
+// at the start of the next tile, after its first barrier.
+constexpr int NT8 = 512, WPT8 = (WCH + NT8 - 1) / NT8;
+constexpr size_t CONV8_LDS = CONV_LDS + sizeof(float) * 2 * 2 * 4 * 4 * 16;
+template <int EPI, bool SPEC>
+__global__ __launch_bounds__(512) void sur_conv8_kernel(ConvArgs A) {
+    extern __shared__ __attribute__((aligned(16))) unsigned short lds[];
+    unsigned short *win = lds;                 // [HS * HS][CP]
+    unsigned short *wt = lds + HS * HS * CP;   // [9 * 64][CP]
+    float *pproj = reinterpret_cast<float *>(lds + HS * HS * CP + 9 * C * CP);  // [slot][half][pg][c][j]
+    const int xd = A.xd, yd = A.yd;
+    const unsigned short *__restrict__ in = A.in;
+    const int z = blockIdx.z, ntx = (xd + TS - 1) / TS, nty = (yd + TS - 1) / TS, ntile = ntx * nty;
+    const int tid = threadIdx.x;
+    // window position (sy << 16 | sx) of this thread's chunk q, -1 past the
+    // window: computed once (the spectral backward pass, short of registers,
+    // each time)
+    constexpr bool HOIST = !(EPI == 2 && SPEC);
+    int wpos[WPT8];
+#pragma unroll
+    for (int q = 0; q < WPT8; q++) {
+        const int e = tid + NT8 * q, p = e >> 3;
+        wpos[q] = (HOIST && e < WCH) ? ((p / HS) << 16) | (p % HS) : -1;
+    }
+    // the window chunks of tile t into registers (zero outside the film)
+    auto load_win = [&](int t, uint4 (&v)[WPT8]) {
+        const int x0 = (t % ntx) * TS, y0 = (t / ntx) * TS;
+        const unsigned short *base = in + (((long)z * yd + y0 - 1) * xd + x0 - 1) * C + 8 * (tid & 7);
+#pragma unroll
+        for (int q = 0; q < WPT8; q++) {
+            int pos = wpos[q];
+            if constexpr (!HOIST) {
+                const int e = tid + NT8 * q, p = e >> 3;
+                pos = e < WCH ? ((p / HS) << 16) | (p % HS) : -1;
+            }
+            const int sy = pos >> 16, sx = pos & 0xFFFF;
+            const int gx = x0 - 1 + sx, gy = y0 - 1 + sy;
+            v[q] = make_uint4(0u, 0u, 0u, 0u);
+            if (pos >= 0 && gx >= 0 && gx < xd && gy >= 0 && gy < yd)
+                v[q] = *reinterpret_cast<const uint4 *>(base + (sy * xd + sx) * C);
+        }
+    };
+    {   // weights: 9 chunks per thread, all loads in flight before the stores
+        constexpr int NW = 9 * C * 8 / NT8;
+        uint4 v[NW];
+#pragma unroll
+        for (int q = 0; q < NW; q++) {
+            const int e = tid + NT8 * q, row = e >> 3, ch = e & 7;
+            v[q] = *reinterpret_cast<const uint4 *>(A.w + row * C + 8 * ch);
+        }
+#pragma unroll
+        for (int q = 0; q < NW; q++) {
+            const int e = tid + NT8 * q, row = e >> 3, ch = e & 7;
+            *reinterpret_cast<uint4 *>(wt + row * CP + 8 * ch) = v[q];
+        }
+    }
+    const int lane = tid & 63, wave = tid >> 6, j = lane & 15, g = lane >> 4;
+    const int pg = wave & 3, hf = wave >> 2;  // pixel rows 4 pg .., channels 32 hf ..
+    // per-lane constants of the epilogue: channels 32 hf + 16 r + 4 g + i
+    float bias_c[2][4], wp_c[2][4];
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int co = 32 * hf + 16 * r + 4 * g + i;
+            bias_c[r][i] = EPI == 2 ? 0.0f : A.bias[co];
+            wp_c[r][i] = EPI == 1 ? A.wp[co] : 0.0f;
+        }
+    const float bp = EPI == 1 ? A.bp[0] : 0.0f;
+    // buffer resources over the outputs (num_records 0 when absent: every store discarded)
+    const long npix = (long)gridDim.z * yd * xd;
+    const int nact = (int)(npix * C * 2);
+    const __amdgpu_buffer_rsrc_t r_out = __builtin_amdgcn_make_buffer_rsrc(A.out, (short)0, A.out ? nact : 0, 0x00020000);
+    const __amdgpu_buffer_rsrc_t r_zs = __builtin_amdgcn_make_buffer_rsrc(A.zsave, (short)0, A.zsave ? nact : 0, 0x00020000);
+    constexpr bool spec = SPEC && EPI != 1;  // the spectral operands are given (x2t, bxt)
+    // the held tile: accumulators, origin, saved pre-activations; `have` false
+    // for the empty tile before the first (its stores are discarded)
+    f32x4 hacc[2][4];
+    uint2 hz[4][2];
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            hacc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+            hz[c][r] = make_uint2(0u, 0u);
+        }
+    int hx0 = 0, hy0 = 0;
+    bool have = false;
+    float proj = 0.0f;
+    int slot = 0;  // EPI 1: projection partials of the epilogue run in this iteration
+    // one 4-channel block (r, c) of the held tile's epilogue
+    auto epi_block = [&](int r, int c) {
+        const int gx = hx0 + j, gy = hy0 + 4 * pg + c;
+        const bool ok = have && gx < xd && gy < yd;
+        const long pix = ((long)z * yd + gy) * xd + gx;
+        const int bo = ok ? (int)((pix * C + 32 * hf + 16 * r + 4 * g) * 2) : nact;  // out of range if not ok
+        float zz[4], v[4];
+        if (EPI == 2) {
+            unpack4(hz[c][r], zz);
+#pragma unroll
+            for (int i = 0; i < 4; i++) v[i] = hacc[r][c][i] * gelu_grad(zz[i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                zz[i] = hacc[r][c][i] + bias_c[r][i];
+                v[i] = gelu(zz[i]);
+                if (EPI == 1) proj += bf2f(f2bf(v[i])) * wp_c[r][i];
+            }
+            const uint2 zp = pack4(zz);
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, zp), r_zs, bo, 0, 0);
+        }
+        if (EPI != 1) {
+            const uint2 vp = pack4(v);
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, vp), r_out, bo, 0, 0);
+        }
+        if (EPI == 1 && r == 1) {
+            // this half's 32 channels of pixel j: lanes j, j + 16, j + 32, j + 48
+            proj += __shfl_xor(proj, 16);
+            proj += __shfl_xor(proj, 32);
+            if (g == 0) pproj[(((slot * 2 + hf) * 4 + pg) * 4 + c) * 16 + j] = proj;
+            proj = 0.0f;
+        }
+    };
+    // EPI 1: the potential of the tile at (ex0, ey0) from the partials in slot sl
+    auto store_phi = [&](int sl, int ex0, int ey0) {
+        if (tid < 256) {
+            const int row = tid >> 4, col = tid & 15, gx = ex0 + col, gy = ey0 + row;
+            const float *pp = pproj + ((sl * 2) * 4 + (row >> 2)) * 64 + (row & 3) * 16 + col;
+            const float v = (pp[0] + pp[4 * 4 * 16] + bp) * A.oscale;  // half 0 + half 1
+            if (gx < xd && gy < yd) A.phi[((long)z * yd + gy) * xd + gx] = v;
+        }
+    };
+    int ex0 = 0, ey0 = 0;
+    bool ehave = false;  // EPI 1: partials of a real tile wait in slot 1 - slot
+    uint4 nxt[WPT8];
+    int t = blockIdx.x;
+    if (t < ntile) load_win(t, nxt);
+    for (; t < ntile; t += gridDim.x) {
+        __syncthreads();  // the previous tile's products are done with win; its partials are in LDS
+        if (EPI == 1 && ehave) store_phi(1 - slot, ex0, ey0);
+#pragma unroll
+        for (int q = 0; q < WPT8; q++) {
+            const int e = tid + NT8 * q;
+            if (e < WCH) *reinterpret_cast<uint4 *>(win + (e >> 3) * CP + 8 * (e & 7)) = nxt[q];
+        }
+        __syncthreads();
+        if (t + (int)gridDim.x < ntile) load_win(t + gridDim.x, nxt);  // in flight during the products
+        const int x0 = (t % ntx) * TS, y0 = (t / ntx) * TS;
+        // the saved pre-activations and the spectral operands of this tile,
+        // loaded now so the products hide their latency
+        uint2 pzin[4][2];
+        bf16x8 sa[4][2], sb;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int gx = x0 + j, gy = y0 + 4 * pg + c;
+            const bool ok = gx < xd && gy < yd;
+            const long pix = ((long)z * yd + gy) * xd + gx;
+            // rows and columns beyond the film read its last ones (discarded outputs)
+            const long srow = (long)z * yd + (gy < yd ? gy : yd - 1);
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                pzin[c][r] = make_uint2(0u, 0u);
+                if (EPI == 2 && ok) pzin[c][r] = *reinterpret_cast<const uint2 *>(A.zin + pix * C + 32 * hf + 16 * r + 4 * g);
+                if (spec)
+                    sa[c][r] = *reinterpret_cast<const bf16x8 *>(A.x2t + (srow * C + 32 * hf + 16 * r + j) * 32 + 8 * g);
+            }
+        }
+        if (spec) sb = *reinterpret_cast<const bf16x8 *>(A.bxt + (long)(x0 + j < xd ? x0 + j : xd - 1) * 32 + 8 * g);
+        f32x4 acc[2][4];
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++) {
+            const int dy = tap / 3, dx = tap % 3;
+#pragma unroll
+            for (int kb = 0; kb < 2; kb++) {
+                bf16x8 a[2], b[4];
+#pragma unroll
+                for (int r = 0; r < 2; r++)
+                    a[r] = *reinterpret_cast<const bf16x8 *>(wt + (tap * C + 32 * hf + 16 * r + j) * CP + 32 * kb + 8 * g);
+#pragma unroll
+                for (int c = 0; c < 4; c++)
+                    b[c] = *reinterpret_cast<const bf16x8 *>(win + ((4 * pg + c + dy) * HS + j + dx) * CP + 32 * kb + 8 * g);
+#pragma unroll
+                for (int r = 0; r < 2; r++)
+#pragma unroll
+                    for (int c = 0; c < 4; c++) acc[r][c] = mfma_bf16(a[r], b[c], acc[r][c]);
+                const int ks = 2 * tap + kb;
+                if (ks < 16 && (ks & 1) == 0) epi_block((ks >> 1) & 1, ks >> 2);  // the held tile, block by block
+                if (spec && ks == 9) {
+                    // the spectral term, midway: its operands have arrived and the
+                    // loop does not end on these products
+#pragma unroll
+                    for (int r = 0; r < 2; r++)
+#pragma unroll
+                        for (int c = 0; c < 4; c++) acc[r][c] = mfma_bf16(sa[c][r], sb, acc[r][c]);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                hacc[r][c] = acc[r][c];
+                hz[c][r] = pzin[c][r];
+            }
+        ex0 = hx0; ey0 = hy0; ehave = have;  // the tile whose partials this iteration wrote
+        slot = 1 - slot;
+        hx0 = x0;
+        hy0 = y0;
+        have = true;
+        // the next iteration starts by reading the accumulators these
+        // products wrote: 19 wait states after the last MFMA, which the
+        // compiler does not insert across the loop's back edge
+        __builtin_amdgcn_sched_barrier(0);
+        asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 2" ::: "memory");
+    }
+    // the last tiles: pending partials, then the held tile's epilogue
+    if (EPI == 1) {
+        __syncthreads();
+        if (ehave) store_phi(1 - slot, ex0, ey0);
+    }
+#pragma unroll
+    for (int b = 0; b < 8; b++) epi_block(b & 1, b >> 1);
+    if (EPI == 1) {
+        __syncthreads();
+        if (have) store_phi(slot, hx0, hy0);
+    }
+}
+
 // ------------------------------------------------- weight gradients --
 // The gradient tile dz (16 x 16 pixels x 64 channels, NHWC) transposed into
 // LDS as dzT[co][pixel] (row stride DZS): thread (row, half, channel block)
@@ -941,11 +1177,11 @@ int launch_conv(int epi, const SurrogatePlan &sp, const ConvArgs &a, hipStream_t
     if (per_layer > ntile) per_layer = ntile;
     dim3 grid(per_layer, 1, g.zd);
     const bool spec = a.x2t != nullptr;
-    if (epi == 0 && spec) hipLaunchKernelGGL((sur_conv_kernel<0, true>), grid, dim3(256), CONV_LDS, s, a);
-    else if (epi == 0) hipLaunchKernelGGL((sur_conv_kernel<0, false>), grid, dim3(256), CONV_LDS, s, a);
-    else if (epi == 1) hipLaunchKernelGGL((sur_conv_kernel<1, false>), grid, dim3(256), CONV_LDS, s, a);
-    else if (spec) hipLaunchKernelGGL((sur_conv_kernel<2, true>), grid, dim3(256), CONV_LDS, s, a);
-    else hipLaunchKernelGGL((sur_conv_kernel<2, false>), grid, dim3(256), CONV_LDS, s, a);
+    if (epi == 0 && spec) hipLaunchKernelGGL((sur_conv8_kernel<0, true>), grid, dim3(NT8), CONV8_LDS, s, a);
+    else if (epi == 0) hipLaunchKernelGGL((sur_conv8_kernel<0, false>), grid, dim3(NT8), CONV8_LDS, s, a);
+    else if (epi == 1) hipLaunchKernelGGL((sur_conv8_kernel<1, false>), grid, dim3(NT8), CONV8_LDS, s, a);
+    else if (spec) hipLaunchKernelGGL((sur_conv8_kernel<2, true>), grid, dim3(NT8), CONV8_LDS, s, a);
+    else hipLaunchKernelGGL((sur_conv8_kernel<2, false>), grid, dim3(NT8), CONV8_LDS, s, a);
     MMS_HIP(hipGetLastError());
     return 0;
 }
@@ -1245,12 +1481,12 @@ int SurrogatePlan::init(const Grid &g_, int zf_, const float *weights, bool trai
     }
     static bool attr = false;
     if (!attr) {
-        for (const void *k : {reinterpret_cast<const void *>(&sur_conv_kernel<0, false>),
-                              reinterpret_cast<const void *>(&sur_conv_kernel<0, true>),
-                              reinterpret_cast<const void *>(&sur_conv_kernel<1, false>),
-                              reinterpret_cast<const void *>(&sur_conv_kernel<2, false>),
-                              reinterpret_cast<const void *>(&sur_conv_kernel<2, true>)})
-            MMS_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)CONV_LDS));
+        for (const void *k : {reinterpret_cast<const void *>(&sur_conv8_kernel<0, false>),
+                              reinterpret_cast<const void *>(&sur_conv8_kernel<0, true>),
+                              reinterpret_cast<const void *>(&sur_conv8_kernel<1, false>),
+                              reinterpret_cast<const void *>(&sur_conv8_kernel<2, false>),
+                              reinterpret_cast<const void *>(&sur_conv8_kernel<2, true>)})
+            MMS_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)CONV8_LDS));
         MMS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&sur_wgrad_kernel),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)WGRAD_LDS));
         attr = true;

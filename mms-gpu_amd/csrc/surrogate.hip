@@ -222,9 +222,10 @@ __global__ __launch_bounds__(256) void sur_lift_kernel(const float *__restrict__
 // 64 x 64 weight taps into LDS once (row stride CP) and then walks
 // tiles blockIdx.x, blockIdx.x + gridDim.x, ... of layer blockIdx.z: the
 // next tile's 18 x 18 x 64 input window is loaded into registers while the
-// current one is multiplied, then stored to LDS.  Wave w computes D[64 co]
-// [64 pixels of tile rows 4w..4w+3] over 18 k steps (tap, 32 input
-// channels).  w: [9 taps][64 co][64 ci] bf16.  With x2t the spectral
+// current one is multiplied, then stored to LDS.  Each of the eight waves
+// computes D[32 co][64 pixels] (a quarter of the tile's rows, half its output
+// channels) over 18 k steps (tap, 32 input channels).  w: [9 taps][64 co]
+// [64 ci] bf16.  With x2t the spectral
 // branch's inverse transform is added in the same accumulators: S[co][x] =
 // sum_k X2[z][y][k][co] Bx[k][x], one 32-deep MFMA per 16 x 16 output tile
 // from X2^T [z][y][64 co][32 k] and Bx^T [x][32 k] (BF16, global memory).
@@ -245,7 +246,6 @@ struct ConvArgs {
     int xd, yd;
 };
 constexpr int WCH = HS * HS * 8;                 // 16-byte chunks of a window
-constexpr int WPT = (WCH + 255) / 256;           // per thread (11)
 constexpr size_t CONV_LDS = sizeof(unsigned short) * ((size_t)HS * HS * CP + 9 * C * CP);
 
 // Software-pipelined over the tiles a workgroup walks: the epilogue of tile
@@ -253,220 +253,10 @@ constexpr size_t CONV_LDS = sizeof(unsigned short) * ((size_t)HS * HS * CP + 9 *
 // k steps of tile t, so its VALU work (GELU, conversions) and stores issue
 // while the matrix cores run; the stores go through buffer resources whose
 // range check discards the lanes outside the film (and every store of the
-// first, empty, held tile).
-template <int EPI, bool SPEC>
-__global__ __launch_bounds__(256) void sur_conv_kernel(ConvArgs A) {
-    extern __shared__ __attribute__((aligned(16))) unsigned short lds[];
-    unsigned short *win = lds;                 // [HS * HS][CP]
-    unsigned short *wt = lds + HS * HS * CP;   // [9 * 64][CP]
-    const int xd = A.xd, yd = A.yd;
-    const unsigned short *__restrict__ in = A.in;
-    const int z = blockIdx.z, ntx = (xd + TS - 1) / TS, nty = (yd + TS - 1) / TS, ntile = ntx * nty;
-    // window position (sy << 16 | sx) of this thread's chunk q, -1 past the
-    // window: computed once (the backward pass, short of registers, each time)
-    constexpr bool HOIST = EPI != 2;
-    int wpos[WPT];
-#pragma unroll
-    for (int q = 0; q < WPT; q++) {
-        const int e = threadIdx.x + 256 * q, p = e >> 3;
-        wpos[q] = (HOIST && e < WCH) ? ((p / HS) << 16) | (p % HS) : -1;
-    }
-    // the window chunks of tile t into registers (zero outside the film)
-    auto load_win = [&](int t, uint4 (&v)[WPT]) {
-        const int x0 = (t % ntx) * TS, y0 = (t / ntx) * TS;
-        if constexpr (HOIST) {
-            const unsigned short *base = in + (((long)z * yd + y0 - 1) * xd + x0 - 1) * C + 8 * (threadIdx.x & 7);
-#pragma unroll
-            for (int q = 0; q < WPT; q++) {
-                const int sy = wpos[q] >> 16, sx = wpos[q] & 0xFFFF;
-                const int gx = x0 - 1 + sx, gy = y0 - 1 + sy;
-                v[q] = make_uint4(0u, 0u, 0u, 0u);
-                if (wpos[q] >= 0 && gx >= 0 && gx < xd && gy >= 0 && gy < yd)
-                    v[q] = *reinterpret_cast<const uint4 *>(base + (sy * xd + sx) * C);
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < WPT; q++) {
-                const int e = threadIdx.x + 256 * q, p = e >> 3, ch = e & 7, sy = p / HS, sx = p % HS;
-                const int gx = x0 - 1 + sx, gy = y0 - 1 + sy;
-                v[q] = make_uint4(0u, 0u, 0u, 0u);
-                if (e < WCH && gx >= 0 && gx < xd && gy >= 0 && gy < yd)
-                    v[q] = *reinterpret_cast<const uint4 *>(in + (((long)z * yd + gy) * xd + gx) * C + 8 * ch);
-            }
-        }
-    };
-    {   // weights: 18 chunks per thread, all loads in flight before the stores
-        constexpr int NW = 9 * C * 8 / 256;
-        uint4 v[NW];
-#pragma unroll
-        for (int q = 0; q < NW; q++) {
-            const int e = threadIdx.x + 256 * q, row = e >> 3, ch = e & 7;
-            v[q] = *reinterpret_cast<const uint4 *>(A.w + row * C + 8 * ch);
-        }
-#pragma unroll
-        for (int q = 0; q < NW; q++) {
-            const int e = threadIdx.x + 256 * q, row = e >> 3, ch = e & 7;
-            *reinterpret_cast<uint4 *>(wt + row * CP + 8 * ch) = v[q];
-        }
-    }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j = lane & 15, g = lane >> 4;
-    // per-lane constants of the epilogue: channels 16 r + 4 g + i
-    float bias_c[4][4], wp_c[4][4];
-#pragma unroll
-    for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            bias_c[r][i] = EPI == 2 ? 0.0f : A.bias[16 * r + 4 * g + i];
-            wp_c[r][i] = EPI == 1 ? A.wp[16 * r + 4 * g + i] : 0.0f;
-        }
-    const float bp = EPI == 1 ? A.bp[0] : 0.0f;
-    // buffer resources over the outputs (num_records 0 when absent: every store discarded)
-    const long npix = (long)gridDim.z * yd * xd;
-    const int nact = (int)(npix * C * 2), nphi = (int)(npix * 4);
-    const __amdgpu_buffer_rsrc_t r_out = __builtin_amdgcn_make_buffer_rsrc(A.out, (short)0, A.out ? nact : 0, 0x00020000);
-    const __amdgpu_buffer_rsrc_t r_zs = __builtin_amdgcn_make_buffer_rsrc(A.zsave, (short)0, A.zsave ? nact : 0, 0x00020000);
-    const __amdgpu_buffer_rsrc_t r_phi = __builtin_amdgcn_make_buffer_rsrc(A.phi, (short)0, A.phi ? nphi : 0, 0x00020000);
-    constexpr bool spec = SPEC && EPI != 1;  // the spectral operands are given (x2t, bxt)
-    // the held tile: accumulators, origin, saved pre-activations; `have` false
-    // for the empty tile before the first (its stores are discarded)
-    f32x4 hacc[4][4];
-    uint2 hz[4][4];
-#pragma unroll
-    for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            hacc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
-            hz[c][r] = make_uint2(0u, 0u);
-        }
-    int hx0 = 0, hy0 = 0;
-    bool have = false;
-    float proj = 0.0f;
-    // one 4-channel block (r, c) of the held tile's epilogue
-    auto epi_block = [&](int r, int c) {
-        const int gx = hx0 + j, gy = hy0 + 4 * wave + c;
-        const bool ok = have && gx < xd && gy < yd;
-        const long pix = ((long)z * yd + gy) * xd + gx;
-        const int bo = ok ? (int)((pix * C + 16 * r + 4 * g) * 2) : nact;  // byte offset, out of range if not ok
-        float zz[4], v[4];
-        if (EPI == 2) {
-            unpack4(hz[c][r], zz);
-#pragma unroll
-            for (int i = 0; i < 4; i++) v[i] = hacc[r][c][i] * gelu_grad(zz[i]);
-        } else {
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                zz[i] = hacc[r][c][i] + bias_c[r][i];
-                v[i] = gelu(zz[i]);
-                if (EPI == 1) proj += bf2f(f2bf(v[i])) * wp_c[r][i];
-            }
-            const uint2 zp = pack4(zz);
-            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, zp), r_zs, bo, 0, 0);
-        }
-        if (EPI != 1) {
-            const uint2 vp = pack4(v);
-            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, vp), r_out, bo, 0, 0);
-        }
-        if (EPI == 1 && r == 3) {
-            // the 64 channels of pixel j are spread over lanes j, j + 16, j + 32, j + 48
-            proj += __shfl_xor(proj, 16);
-            proj += __shfl_xor(proj, 32);
-            const int po = (ok && g == 0) ? (int)(pix * 4) : nphi;
-            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, (proj + bp) * A.oscale), r_phi, po, 0, 0);
-            proj = 0.0f;
-        }
-    };
-    uint4 nxt[WPT];
-    int t = blockIdx.x;
-    if (t < ntile) load_win(t, nxt);
-    for (; t < ntile; t += gridDim.x) {
-        __syncthreads();  // the previous tile's products are done with win
-#pragma unroll
-        for (int q = 0; q < WPT; q++) {
-            const int e = threadIdx.x + 256 * q;
-            if (e < WCH) *reinterpret_cast<uint4 *>(win + (e >> 3) * CP + 8 * (e & 7)) = nxt[q];
-        }
-        __syncthreads();
-        if (t + (int)gridDim.x < ntile) load_win(t + gridDim.x, nxt);  // in flight during the products
-        const int x0 = (t % ntx) * TS, y0 = (t / ntx) * TS;
-        // the saved pre-activations and the spectral operands of this tile,
-        // loaded now so the products hide their latency
-        uint2 pzin[4][4];
-        bf16x8 sa[4][4], sb;
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const int gx = x0 + j, gy = y0 + 4 * wave + c;
-            const bool ok = gx < xd && gy < yd;
-            const long pix = ((long)z * yd + gy) * xd + gx;
-            // rows and columns beyond the film read its last ones (discarded outputs)
-            const long srow = (long)z * yd + (gy < yd ? gy : yd - 1);
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                pzin[c][r] = make_uint2(0u, 0u);
-                if (EPI == 2 && ok) pzin[c][r] = *reinterpret_cast<const uint2 *>(A.zin + pix * C + 16 * r + 4 * g);
-                if (spec) sa[c][r] = *reinterpret_cast<const bf16x8 *>(A.x2t + (srow * C + 16 * r + j) * 32 + 8 * g);
-            }
-        }
-        if (spec) sb = *reinterpret_cast<const bf16x8 *>(A.bxt + (long)(x0 + j < xd ? x0 + j : xd - 1) * 32 + 8 * g);
-        f32x4 acc[4][4];
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-#pragma unroll
-            for (int c = 0; c < 4; c++) acc[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int tap = 0; tap < 9; tap++) {
-            const int dy = tap / 3, dx = tap % 3;
-#pragma unroll
-            for (int kb = 0; kb < 2; kb++) {
-                bf16x8 a[4], b[4];
-#pragma unroll
-                for (int r = 0; r < 4; r++)
-                    a[r] = *reinterpret_cast<const bf16x8 *>(wt + (tap * C + 16 * r + j) * CP + 32 * kb + 8 * g);
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    const int py = 4 * wave + c, px = j;
-                    b[c] = *reinterpret_cast<const bf16x8 *>(win + ((py + dy) * HS + px + dx) * CP + 32 * kb + 8 * g);
-                }
-#pragma unroll
-                for (int r = 0; r < 4; r++)
-#pragma unroll
-                    for (int c = 0; c < 4; c++) acc[r][c] = mfma_bf16(a[r], b[c], acc[r][c]);
-                const int ks = 2 * tap + kb;
-                if (ks < 16) epi_block(ks & 3, ks >> 2);  // the held tile, block by block
-                if (spec && ks == 9) {
-                    // the spectral term, midway: its operands have arrived and the
-                    // loop does not end on these products
-#pragma unroll
-                    for (int r = 0; r < 4; r++)
-#pragma unroll
-                        for (int c = 0; c < 4; c++) acc[r][c] = mfma_bf16(sa[c][r], sb, acc[r][c]);
-                }
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-                hacc[r][c] = acc[r][c];
-                hz[c][r] = pzin[c][r];
-            }
-        hx0 = x0;
-        hy0 = y0;
-        have = true;
-        // the next iteration starts by reading the accumulators these
-        // products wrote: 19 wait states after the last MFMA, which the
-        // compiler does not insert across the loop's back edge
-        __builtin_amdgcn_sched_barrier(0);
-        asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 2" ::: "memory");
-    }
-    // the last tile's epilogue
-#pragma unroll
-    for (int ks = 0; ks < 16; ks++) epi_block(ks & 3, ks >> 2);
-}
-
-// Eight-wave form of the same convolution (512 threads, two waves per SIMD):
-// wave w takes pixel rows 4 (w & 3) .. + 3 of the tile and output channels
-// 32 (w >> 2) .. + 31, so each SIMD holds two waves whose products,
-// epilogue arithmetic and LDS reads interleave.  With EPI 1 the two channel
+// first, empty, held tile).  512 threads, two waves per SIMD: wave w takes
+// pixel rows 4 (w & 3) .. + 3 of the tile and output channels 32 (w >> 2)
+// .. + 31, so each SIMD holds two waves whose products, epilogue arithmetic
+// and LDS reads interleave.  With EPI 1 the two channel
 // halves' projections of a pixel meet in LDS (half 0 + half 1) and are stored
 // at the start of the next tile, after its first barrier.
 constexpr int NT8 = 512, WPT8 = (WCH + NT8 - 1) / NT8;

@@ -281,10 +281,13 @@ __global__ __launch_bounds__(512) void sur_conv8_kernel(ConvArgs A) {
         const int e = tid + NT8 * q, p = e >> 3;
         wpos[q] = (HOIST && e < WCH) ? ((p / HS) << 16) | (p % HS) : -1;
     }
-    // the window chunks of tile t into registers (zero outside the film)
+    // the window chunks of tile t into registers: buffer loads whose range
+    // check returns zeros for chunks outside the film (no branches)
+    const int nin = (int)((long)gridDim.z * yd * xd * C * 2);
+    const __amdgpu_buffer_rsrc_t r_in = __builtin_amdgcn_make_buffer_rsrc(const_cast<unsigned short *>(in), (short)0, nin, 0x00020000);
     auto load_win = [&](int t, uint4 (&v)[WPT8]) {
         const int x0 = (t % ntx) * TS, y0 = (t / ntx) * TS;
-        const unsigned short *base = in + (((long)z * yd + y0 - 1) * xd + x0 - 1) * C + 8 * (tid & 7);
+        const int base = ((z * yd + y0 - 1) * xd + x0 - 1) * C * 2 + 16 * (tid & 7);  // bytes (may be negative)
 #pragma unroll
         for (int q = 0; q < WPT8; q++) {
             int pos = wpos[q];
@@ -294,9 +297,9 @@ __global__ __launch_bounds__(512) void sur_conv8_kernel(ConvArgs A) {
             }
             const int sy = pos >> 16, sx = pos & 0xFFFF;
             const int gx = x0 - 1 + sx, gy = y0 - 1 + sy;
-            v[q] = make_uint4(0u, 0u, 0u, 0u);
-            if (pos >= 0 && gx >= 0 && gx < xd && gy >= 0 && gy < yd)
-                v[q] = *reinterpret_cast<const uint4 *>(base + (sy * xd + sx) * C);
+            const bool ok = pos >= 0 && gx >= 0 && gx < xd && gy >= 0 && gy < yd;
+            const int off = ok ? base + (sy * xd + sx) * C * 2 : nin;
+            v[q] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r_in, off, 0, 0));
         }
     };
     {   // weights: 9 chunks per thread, all loads in flight before the stores

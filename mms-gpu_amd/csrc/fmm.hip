@@ -465,6 +465,20 @@ __global__ __launch_bounds__(256) void up_coarse_kernel(float *__restrict__ Mall
     float *M = Mall + (long)f * film_stride;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j16 = lane & 15, g = lane >> 4;
     float *const b0 = lds, *const b1 = lds + 256 * SB;  // level l goes to b1 if l is odd, else b0
+    // the M2M operator fragments of the four child positions, held in
+    // registers for every level (at most 32 slots: 64 VGPRs); larger blocks
+    // read them inside the product loop
+    constexpr bool HOLD = KP <= 32;
+    f32x4 avh[HOLD ? 4 : 1][HOLD ? G / 4 : 1][RT];
+    if constexpr (HOLD) {
+#pragma unroll
+        for (int ci = 0; ci < 4; ci++)
+#pragma unroll
+            for (int u4 = 0; u4 < G / 4; u4++)
+#pragma unroll
+                for (int r = 0; r < RT; r++)
+                    avh[ci][u4][r] = *reinterpret_cast<const f32x4 *>(m2mfrag + (((long)ci * RT + r) * 64 + lane) * G + 4 * u4);
+    }
     int s = 1 << Ltop;
     for (int l = Ltop - 1; l >= 2; l--) {
         const int sp = s >> 1, np = sp * sp, ntp = (np + 15) / 16;
@@ -477,18 +491,27 @@ __global__ __launch_bounds__(256) void up_coarse_kernel(float *__restrict__ Mall
             f32x4 acc[RT];
 #pragma unroll
             for (int r = 0; r < RT; r++) acc[r] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 1
+            // the four children's vectors, every load issued before the products
+            f32x4 bv[4][G / 4];
+#pragma unroll
             for (int ci = 0; ci < 4; ci++) {
                 const int cx = 2 * px + (ci & 1), cy = 2 * py + (ci >> 1);
                 const float *bsrc = from_hbm ? M + lo.off[l + 1] + (((long)z * s + cy) * s + cx) * KP
                                              : ch + (cy * s + cx) * SB;
 #pragma unroll
+                for (int u4 = 0; u4 < G / 4; u4++)
+                    bv[ci][u4] = ok ? *reinterpret_cast<const f32x4 *>(bsrc + g * G + 4 * u4) : f32x4{0.f, 0.f, 0.f, 0.f};
+            }
+#pragma unroll
+            for (int ci = 0; ci < 4; ci++) {
+#pragma unroll
                 for (int u4 = 0; u4 < G / 4; u4++) {
                     f32x4 av[RT];
 #pragma unroll
                     for (int r = 0; r < RT; r++)
-                        av[r] = *reinterpret_cast<const f32x4 *>(m2mfrag + (((long)ci * RT + r) * 64 + lane) * G + 4 * u4);
-                    const f32x4 b = ok ? *reinterpret_cast<const f32x4 *>(bsrc + g * G + 4 * u4) : f32x4{0.f, 0.f, 0.f, 0.f};
+                        av[r] = HOLD ? avh[HOLD ? ci : 0][HOLD ? u4 : 0][r]
+                                     : *reinterpret_cast<const f32x4 *>(m2mfrag + (((long)ci * RT + r) * 64 + lane) * G + 4 * u4);
+                    const f32x4 b = bv[ci][u4];
 #pragma unroll
                     for (int u = 0; u < 4; u++)
 #pragma unroll

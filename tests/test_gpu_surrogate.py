@@ -24,8 +24,11 @@ def _weights(mg, seed):
     return w
 
 
-@pytest.mark.parametrize("dims", [(48, 40, 2), (64, 64, 3), (33, 17, 1)])
+@pytest.mark.parametrize("dims", [(48, 40, 2), (64, 64, 3), (33, 17, 1), (24, 20, 10)])
 def test_surrogate_forward_matches_fp32_reference(mg, dims):
+    """One forward pass against the fp32 restatement; the 10-layer film
+    stages its input planes through the path for films of more than 8
+    layers."""
     import surrogate_ref
     from mms_gpu import workloads as wl
     xd, yd, zd = dims

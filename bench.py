@@ -565,8 +565,9 @@ def main():
     if args.serial:
         os.environ["MMS_GPU_SERIAL"] = "1"
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md, 'How the path would shard')", file=sys.stderr)
+        print("bench.py runs on one GPU only: the library has no multi-GPU path "
+              "(films, layers or subtrees would be sharded across processes; none is implemented)",
+              file=sys.stderr)
         sys.exit(2)
 
     import mms_gpu as mg

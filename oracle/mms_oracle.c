@@ -573,7 +573,7 @@ void or_llg_mprime(const float *M, const float *H, float alfa, float gamma, floa
 }
 
 /* external field: the reference ramp (ode_functions.cpp:61-66) when hext is
- * NULL, else a constant vector in A/m (the BASELINE configs' uniform fields) */
+ * NULL, else a constant vector in A/m (the dot film's and the sweep's uniform fields) */
 static v3 ext_field(float t, float mu_0, const float *hext) {
     if (hext) return v3_make(hext[0], hext[1], hext[2]);
     double amp = (t < 1e-9) ? (-0.3 / 1e-9 * (double)t + 0.3) : 0.0;
